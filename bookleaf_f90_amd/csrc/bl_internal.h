@@ -1,0 +1,129 @@
+// bl_internal.h -- handle layout and launch helpers of the BookLeaf HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bookleaf_hip.h"
+
+namespace bl {
+
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+
+void set_error(const std::string& msg);
+
+#define BL_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::bl::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return BL_FAILURE;                                                          \
+    }                                                                             \
+  } while (0)
+
+inline unsigned nblocks(int64_t n, int bs = kBlock) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+// dt reduction results written by the last workgroup of the getdt kernels
+struct DtResult {
+  double cfl_min;   // min over elements of rscratch11 (len^2/cs2)
+  double div_max;   // max over elements of |div u|/V (TINY if none larger)
+  int32_t cfl_idx;  // 1-based, first index attaining the min
+  int32_t div_idx;  // 1-based, 0 if none exceeds TINY
+  int32_t negvol;   // largest 1-based index with V < 0 (0 if none)
+  int32_t pad;
+};
+
+// device-resident timestep state of the fused step (bl_run / bl_run_graph)
+struct DevTimestep {
+  double time, dt, dts, dt_prev;
+  int32_t nstep, idtel, sdt, done;   // done: time_end reached or error
+  int32_t ierr, ierr_step, err_kind, pad;  // err_kind: 1 negvol, 2 cfl, 3 dtmin
+};
+
+// per-launch kernel timing with HIP events on the handle's stream
+struct TimingClass {
+  std::string name;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct Handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bl_sizes_t sz{};
+  bl_config_t cfg{};
+  // per-region / per-material tables (host + device)
+  int32_t nreg = 0, nmat = 0;
+  std::vector<double> kappareg, pmeritreg;
+  std::vector<int32_t> zdtnotreg, zmidlength, eos_itype;
+  std::vector<double> eos_param;
+  double* d_kappareg = nullptr;
+  double* d_pmeritreg = nullptr;
+  int32_t* d_zdtnotreg = nullptr;
+  int32_t* d_zmidlength = nullptr;
+  int32_t* d_eos_itype = nullptr;
+  double* d_eos_param = nullptr;
+  bool ideal_single = false;  // every material 1..nmat ideal gas: fast EOS path
+  // dataAPI field mirror
+  void* f[BL_NFIELD + 1] = {};
+  int32_t kind[BL_NFIELD + 1] = {};
+  int64_t count[BL_NFIELD + 1] = {};
+  // node -> corner CSR maps (corner id = 4*(iel-1) + (j-1))
+  int32_t* d_hyd_ptr = nullptr;  // order (corner j, element in ielsort1 order)
+  int32_t* d_hyd_idx = nullptr;
+  int32_t* d_ale_ptr = nullptr;  // order (element in ielsort2 order, corner)
+  int32_t* d_ale_idx = nullptr;
+  int32_t* d_el_ptr = nullptr;   // order (element 1..nel, corner)
+  int32_t* d_el_idx = nullptr;
+  bool maps_built = false;
+  // reductions
+  double* d_part = nullptr;  // 4 doubles per workgroup
+  int32_t* d_parti = nullptr;
+  uint32_t* d_ticket = nullptr;  // 4 counters
+  DtResult* d_dtres = nullptr;
+  DtResult* h_dtres = nullptr;  // pinned
+  int64_t npart = 0;
+  // fused-step state
+  DevTimestep* d_ts = nullptr;
+  DevTimestep* h_ts = nullptr;  // pinned
+  double* d_hist = nullptr;     // dt history (3 values per step)
+  int64_t hist_cap = 0;
+  double* d_tmp[8] = {};        // fused-step temporaries (nel or nnd sized)
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_ale = false;
+  // timing
+  bool timing = false;
+  std::vector<TimingClass> tclass;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_class;  // class of each event pair in the pool
+  int ev_used = 0;
+
+  double* R(int id) const { return (double*)f[id]; }
+  int32_t* I(int id) const { return (int32_t*)f[id]; }
+  uint8_t* Z(int id) const { return (uint8_t*)f[id]; }
+};
+
+// timing helpers (bl_api.hip)
+int timing_begin(Handle* h, const char* cls);
+void timing_end(Handle* h, int slot);
+
+}  // namespace bl
+
+struct bl_handle : public bl::Handle {};
+
+// launch a kernel on the handle's stream, timed by class name when enabled
+#define BL_LAUNCH(h, cls, kern, grid, block, ...)                                  \
+  do {                                                                             \
+    int slot_ = (h)->timing ? ::bl::timing_begin((h), (cls)) : -1;                 \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, (h)->stream, __VA_ARGS__); \
+    if (slot_ >= 0) ::bl::timing_end((h), slot_);                                  \
+    hipError_t le_ = hipGetLastError();                                            \
+    if (le_ != hipSuccess) {                                                       \
+      ::bl::set_error(std::string("launch ") + #kern + ": " + hipGetErrorString(le_)); \
+      return BL_FAILURE;                                                           \
+    }                                                                              \
+  } while (0)

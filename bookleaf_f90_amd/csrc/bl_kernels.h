@@ -1,0 +1,100 @@
+// bl_kernels.h -- deterministic workgroup reductions used by the dt controls.
+//
+// hydro_kn_getdtcfl picks the FIRST element attaining the minimum of
+// rscratch11 (strict '<' scan, hydro_kn_getdt.f90:68-72); hydro_kn_getdtdiv
+// picks the FIRST element attaining the maximum, and only if it exceeds TINY
+// (strict '>' scan from TINY, hydro_kn_getdt.f90:104-117).  Both are total
+// orders on (value, index), so a tree reduction in any order returns the
+// same element as the serial scan: wave64 shuffles, then LDS across the
+// waves of the workgroup, then the last workgroup over the partials.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+
+#include "bl_internal.h"
+
+namespace bl {
+
+__device__ __forceinline__ void better_min(double& v, int& i, double v2, int i2) {
+  if (v2 < v || (v2 == v && i2 < i)) { v = v2; i = i2; }
+}
+__device__ __forceinline__ void better_max(double& v, int& i, double v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
+}
+
+template <bool IS_MIN>
+__device__ __forceinline__ void wave_reduce(double& v, int& i) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double v2 = __shfl_xor(v, off, 64);
+    int i2 = __shfl_xor(i, off, 64);
+    if (IS_MIN) better_min(v, i, v2, i2);
+    else better_max(v, i, v2, i2);
+  }
+}
+
+// reduce over the workgroup; thread 0 writes the partial of this workgroup
+template <bool IS_MIN>
+__device__ __forceinline__ void block_reduce_store(double v, int i, double* part, int32_t* parti) {
+  __shared__ double sv[kBlock / 64];
+  __shared__ int si[kBlock / 64];
+  wave_reduce<IS_MIN>(v, i);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sv[wid] = v; si[wid] = i; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv = sv[0];
+    int bi = si[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      if (IS_MIN) better_min(bv, bi, sv[w], si[w]);
+      else better_max(bv, bi, sv[w], si[w]);
+    }
+    part[blockIdx.x] = bv;
+    parti[blockIdx.x] = bi;
+  }
+}
+
+__device__ __forceinline__ void block_argmin_first(double v, int i, double* part, int32_t* parti) {
+  if (!(v == v)) v = DBL_MAX;  // a NaN never wins the strict '<' scan
+  block_reduce_store<true>(v, i, part, parti);
+}
+__device__ __forceinline__ void block_argmax_first(double v, int i, double* part, int32_t* parti) {
+  if (!(v == v)) v = -1.0;  // a NaN never wins the strict '>' scan
+  block_reduce_store<false>(v, i, part, parti);
+}
+
+// run by the whole last workgroup: DIV partials in part[0,na), CFL partials in
+// part[2na, 2na+nc)
+__device__ __forceinline__ void final_dt_reduce(int na, int nc, const double* part,
+                                                const int32_t* parti, DtResult* res) {
+  __shared__ double sv[kBlock / 64];
+  __shared__ int si[kBlock / 64];
+  // DIV (max)
+  double v = -1.0;
+  int i = 0x7fffffff;
+  for (int k = threadIdx.x; k < na; k += blockDim.x) better_max(v, i, part[k], parti[k]);
+  wave_reduce<false>(v, i);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sv[wid] = v; si[wid] = i; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better_max(sv[0], si[0], sv[w], si[w]);
+    if (sv[0] > DBL_MIN) { res->div_max = sv[0]; res->div_idx = si[0]; }
+    else { res->div_max = DBL_MIN; res->div_idx = 0; }
+  }
+  __syncthreads();
+  // CFL (min)
+  v = DBL_MAX;
+  i = 0x7fffffff;
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) better_min(v, i, part[2 * na + k], parti[2 * na + k]);
+  wave_reduce<true>(v, i);
+  if (lane == 0) { sv[wid] = v; si[wid] = i; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better_min(sv[0], si[0], sv[w], si[w]);
+    res->cfl_min = sv[0];
+    res->cfl_idx = si[0];
+  }
+}
+
+}  // namespace bl

@@ -1,0 +1,335 @@
+"""Host-side mirror of the reference's hot-path drivers over the C-ABI.
+
+``BookLeaf(problem)`` owns a device-resident dataAPI registry on the current
+HIP device (one MI355X) and exposes one method per reference driver with the
+reference's name (hydro_getdt, hydro_getforce, ..., ale_advect), plus the
+solver sequence of src/infrastructure/solver (getdt / lagstep / alestep) and
+the device-resident multi-step runner.  There is no CPU fallback: the HIP
+library must be built (``python -m bookleaf_f90_amd.build``) or construction
+fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_LIB = None
+LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbookleaf_hip.so")
+
+
+class BookLeafError(RuntimeError):
+    def __init__(self, msg, err=None):
+        super().__init__(msg)
+        self.err = err
+        self.ierr = err.ierr if err is not None else None
+
+
+def load_library(path=LIBPATH):
+    """Load libbookleaf_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise ImportError("libbookleaf_hip.so not built: run python -m bookleaf_f90_amd.build")
+        lib = C.CDLL(path)
+        abi.declare(lib, "bl")
+        P, vp = C.POINTER, C.c_void_p
+        lib.bl_create.argtypes = [P(abi.Sizes), P(vp)]
+        lib.bl_create.restype = C.c_int
+        lib.bl_destroy.argtypes = [vp]
+        lib.bl_set_config.argtypes = [vp, P(abi.Config)]
+        lib.bl_set_regions.argtypes = [vp, C.c_int32, P(C.c_double), P(C.c_double),
+                                       P(C.c_int32), P(C.c_int32)]
+        lib.bl_set_eos.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_double)]
+        lib.bl_field_info.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_int64)]
+        lib.bl_field_device_ptr.argtypes = [vp, C.c_int32]
+        lib.bl_field_device_ptr.restype = vp
+        lib.bl_upload.argtypes = [vp, C.c_int32, vp, C.c_int64]
+        lib.bl_download.argtypes = [vp, C.c_int32, vp, C.c_int64]
+        lib.bl_build_maps.argtypes = [vp]
+        lib.bl_init_mesh.argtypes = [vp]
+        lib.bl_synchronize.argtypes = [vp]
+        lib.bl_run_graph.argtypes = lib.bl_run.argtypes
+        lib.bl_timing_enable.argtypes = [vp, C.c_int32]
+        lib.bl_timing_reset.argtypes = [vp]
+        lib.bl_timing_query.argtypes = [vp, C.c_char_p, P(C.c_double), P(C.c_int64)]
+        lib.bl_timing_names.argtypes = [vp, C.c_char_p, C.c_int32]
+        lib.bl_last_error.restype = C.c_char_p
+        _LIB = lib
+    return _LIB
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class DriverState:
+    """Common driver-level interface of the HIP library and the CPU oracle.
+
+    Subclasses set ``self.lib``, ``self.prefix`` and ``self.h`` and implement
+    ``get``/``set`` for dataAPI fields.
+    """
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + "_" + name)
+
+    def _check(self, rc, what, err=None):
+        if rc != abi.SUCCESS:
+            msg = what
+            if err is not None and err.ierr != 0:
+                msg += ": " + err.message + " (ierr=%d)" % err.ierr
+            raise BookLeafError(msg, err)
+
+    def _setup(self, problem):
+        """inf_init order (inf_init.F90:53-101) with the problem's mesh and ICs."""
+        p = problem
+        self.problem = p
+        self.sizes = p.sizes()
+        for name in ("ielnd", "ielreg", "ielmat", "indtype", "ndx", "ndy"):
+            self.set(name, getattr(p, name))
+        self._check(self._f("init_mesh")(self.h), "init_mesh")
+        for name in ("ndu", "ndv"):
+            self.set(name, getattr(p, name))
+        self.set("eldensity", p.density)
+        self.set("elenergy", p.energy)
+        err = abi.Error()
+        self._check(self._f("geometry_getgeometry")(self.h, C.byref(err)), "getgeometry", err)
+        self._check(self._f("init_state")(self.h), "init_state")
+        self._check(self._f("hydro_init")(self.h), "hydro_init")
+        self.ts = abi.Timestep()
+        self.ts.time = p.config["time_start"]
+        self.ts.dt = p.config["dt_initial"]
+        self.ts.dts = self.ts.dt
+        self.ts.nstep = 0
+        self.ts.idtel = 0
+        self.ts.zcorrector = 0
+        self.ts.sdt = abi.DT_INITIAL
+
+    # ---- drivers (src/packages/*/driver) -------------------------------
+    def hydro_getdt(self):
+        cfl, div, err = abi.Dt(), abi.Dt(), abi.Error()
+        self._check(self._f("hydro_getdt")(self.h, C.byref(cfl), C.byref(div), C.byref(err)),
+                    "hydro_getdt", err)
+        return cfl, div
+
+    def hydro_setpredictor(self):
+        self._check(self._f("hydro_setpredictor")(self.h), "hydro_setpredictor")
+
+    def hydro_setcorrector(self):
+        self._check(self._f("hydro_setcorrector")(self.h), "hydro_setcorrector")
+
+    def hydro_getforce(self):
+        self._check(self._f("hydro_getforce")(self.h, C.byref(self.ts)), "hydro_getforce")
+
+    def hydro_getstate(self):
+        err = abi.Error()
+        self._check(self._f("hydro_getstate")(self.h, C.byref(self.ts), C.byref(err)),
+                    "hydro_getstate", err)
+
+    def hydro_getenergy(self):
+        self._check(self._f("hydro_getenergy")(self.h, C.byref(self.ts)), "hydro_getenergy")
+
+    def hydro_geteos(self):
+        self._check(self._f("hydro_geteos")(self.h), "hydro_geteos")
+
+    def hydro_initacceleration(self):
+        self._check(self._f("hydro_initacceleration")(self.h), "hydro_initacceleration")
+
+    def hydro_scatteracceleration(self):
+        self._check(self._f("hydro_scatteracceleration")(self.h), "hydro_scatteracceleration")
+
+    def hydro_getacceleration(self):
+        self._check(self._f("hydro_getacceleration")(self.h), "hydro_getacceleration")
+
+    def hydro_applyacceleration(self):
+        self._check(self._f("hydro_applyacceleration")(self.h, C.byref(self.ts)),
+                    "hydro_applyacceleration")
+
+    def geometry_getgeometry(self):
+        err = abi.Error()
+        self._check(self._f("geometry_getgeometry")(self.h, C.byref(err)), "getgeometry", err)
+
+    def ale_active(self):
+        return bool(self._f("ale_active")(self.h, C.byref(self.ts)))
+
+    def ale_getmeshstatus(self):
+        self._check(self._f("ale_getmeshstatus")(self.h), "ale_getmeshstatus")
+
+    def ale_getfluxvolume(self):
+        self._check(self._f("ale_getfluxvolume")(self.h, C.byref(self.ts)), "ale_getfluxvolume")
+
+    def ale_advect(self):
+        err = abi.Error()
+        self._check(self._f("ale_advect")(self.h, C.byref(self.ts), C.byref(err)),
+                    "ale_advect", err)
+
+    def ale_hydroupdate(self):
+        err = abi.Error()
+        self._check(self._f("ale_hydroupdate")(self.h, C.byref(err)), "ale_hydroupdate", err)
+
+    # ---- time package (time_dr_advance.f90:38-132) -----------------------
+    def time_select(self, cfl, div):
+        c = self.problem.config
+        cand = [(c["dt_max"], -1, abi.DT_MAXIMUM), (c["dt_g"] * self.ts.dt, -1, abi.DT_GROWTH)]
+        if self.ts.nstep == 0:
+            cand.append((c["dt_initial"], -1, abi.DT_INITIAL))
+        cand += [(cfl.rdt, cfl.idt, cfl.sdt), (div.rdt, div.idt, div.sdt)]
+        best = cand[0]
+        for cnd in cand[1:]:
+            if cnd[0] < best[0]:
+                best = cnd
+        self.ts.nstep += 1
+        self.ts.dt, self.ts.idtel, self.ts.sdt = best
+        self.ts.time = self.ts.time + self.ts.dt
+        if self.ts.dt < c["dt_min"]:
+            raise BookLeafError("ERROR: dt < dt_min")
+
+    # ---- solver sequence (src/infrastructure/solver) ----------------------
+    def getdt(self):
+        cfl, div = self.hydro_getdt()
+        self.time_select(cfl, div)
+
+    def lagstep(self):
+        self.ts.zcorrector = 0
+        self.ts.dts = 0.5 * self.ts.dt
+        self.hydro_setpredictor()
+        self.hydro_getforce()
+        self.hydro_getstate()
+        self.hydro_getenergy()
+        self.hydro_geteos()
+        self.ts.zcorrector = 1
+        self.ts.dts = self.ts.dt
+        self.hydro_setcorrector()
+        self.hydro_getforce()
+        self.hydro_initacceleration()
+        self.hydro_scatteracceleration()
+        self.hydro_getacceleration()
+        self.hydro_applyacceleration()
+        self.hydro_getstate()
+        self.hydro_getenergy()
+        self.hydro_geteos()
+
+    def alestep(self):
+        if not self.ale_active():
+            return
+        self.ale_getmeshstatus()
+        self.ale_getfluxvolume()
+        self.ale_advect()
+        self.ale_hydroupdate()
+
+    def step(self):
+        self.getdt()
+        self.lagstep()
+        self.alestep()
+
+    def run(self, nsteps, stop_at_time_end=False, runner="run"):
+        """Run up to nsteps full steps in the library; returns the per-step
+        (dt, idtel, control) history."""
+        dt = np.zeros(max(nsteps, 1))
+        idt = np.zeros(max(nsteps, 1), np.int32)
+        ctl = np.zeros(max(nsteps, 1), np.int32)
+        taken = C.c_int32(0)
+        err = abi.Error()
+        rc = self._f(runner)(self.h, nsteps, int(bool(stop_at_time_end)), C.byref(self.ts),
+                             _dptr(dt), _iptr(idt), _iptr(ctl), C.byref(taken), C.byref(err))
+        n = taken.value
+        self.last_history = (dt[:n].copy(), idt[:n].copy(), ctl[:n].copy())
+        self._check(rc, runner, err)
+        return self.last_history
+
+    def totals(self):
+        t = abi.Totals()
+        self._check(self._f("totals")(self.h, C.byref(t)), "totals")
+        return t
+
+    def state(self, names=None):
+        names = names or abi.FIELDS
+        return {n: self.get(n) for n in names}
+
+
+class BookLeaf(DriverState):
+    """Device-resident BookLeaf state on the current HIP device."""
+
+    def __init__(self, problem):
+        self.lib = load_library()
+        self.prefix = "bl"
+        sz = problem.sizes()
+        h = C.c_void_p()
+        if self.lib.bl_create(C.byref(sz), C.byref(h)) != abi.SUCCESS:
+            raise BookLeafError("bl_create: " + self.lib.bl_last_error().decode())
+        self.h = h
+        cfg = problem.cconfig()
+        self._check(self.lib.bl_set_config(self.h, C.byref(cfg)), "set_config")
+        self._check(self.lib.bl_set_regions(
+            self.h, problem.nreg, _dptr(np.ascontiguousarray(problem.kappareg, np.float64)),
+            _dptr(np.ascontiguousarray(problem.pmeritreg, np.float64)),
+            _iptr(np.ascontiguousarray(problem.zdtnotreg, np.int32)),
+            _iptr(np.ascontiguousarray(problem.zmidlength, np.int32))), "set_regions")
+        self._check(self.lib.bl_set_eos(
+            self.h, problem.nmat, _iptr(np.ascontiguousarray(problem.eos_itype, np.int32)),
+            _dptr(np.ascontiguousarray(problem.eos_param, np.float64))), "set_eos")
+        self._setup(problem)
+
+    def _check(self, rc, what, err=None):
+        if rc != abi.SUCCESS:
+            msg = what + ": " + self.lib.bl_last_error().decode()
+            if err is not None and err.ierr != 0:
+                msg += " [" + err.message + ", ierr=%d]" % err.ierr
+            raise BookLeafError(msg, err)
+
+    def _info(self, name):
+        kind, count = C.c_int32(), C.c_int64()
+        self.lib.bl_field_info(self.h, abi.ID[name], C.byref(kind), C.byref(count))
+        return kind.value, count.value
+
+    def get(self, name):
+        kind, count = self._info(name)
+        out = np.empty(count, abi.DTYPE[kind])
+        self._check(self.lib.bl_download(self.h, abi.ID[name], out.ctypes.data, count),
+                    "download " + name)
+        return out
+
+    def set(self, name, values):
+        kind, count = self._info(name)
+        a = np.ascontiguousarray(values, abi.DTYPE[kind])
+        self._check(self.lib.bl_upload(self.h, abi.ID[name], a.ctypes.data, len(a)),
+                    "upload " + name)
+
+    def device_ptr(self, name):
+        return self.lib.bl_field_device_ptr(self.h, abi.ID[name])
+
+    def synchronize(self):
+        self._check(self.lib.bl_synchronize(self.h), "synchronize")
+
+    def timing(self, on=True):
+        self._check(self.lib.bl_timing_enable(self.h, int(on)), "timing_enable")
+
+    def timing_reset(self):
+        self._check(self.lib.bl_timing_reset(self.h), "timing_reset")
+
+    def timing_query(self, name):
+        ms, n = C.c_double(), C.c_int64()
+        self._check(self.lib.bl_timing_query(self.h, name.encode(), C.byref(ms), C.byref(n)),
+                    "timing_query")
+        return ms.value, n.value
+
+    def timing_names(self):
+        buf = C.create_string_buffer(4096)
+        self.lib.bl_timing_names(self.h, buf, 4096)
+        return [s for s in buf.value.decode().split(",") if s]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,289 @@
+"""Problem set-up on the host: structured quad meshes, boundary node types,
+initial conditions and configuration, in the dataAPI layout.
+
+This replaces the reference's deck reader / mesh generator / IC set-up
+(src/packages/setup, out of the hot path) for the workloads the hot path is
+measured on.  Conventions follow the reference:
+
+* node types (setup_mesh.F90:1379-1584): interior nodes carry their region
+  number (> 0); boundary nodes carry -bc with SLIPX=1, SLIPY=2, WALL=3,
+  FREE=8 (setup_mesh.F90:324-341); corners joining SLIPX and SLIPY become -3
+  (setup_mesh.F90:886-1000);
+* element corners are counter-clockwise, face f joins corners f and f+1
+  (dataAPI_params.f90:27-28 IFCND);
+* energy given per unit mass; Sedov's deposit uses energy_scale='mass'
+  semantics e = E/(rho*V) (setup_IC.f90:621-625);
+* defaults of *_cf_defaults (global_config.f90:37-40, hydro_config.f90:43-49,
+  eos_config.f90:42-43, time_config.f90:37-42, ale_config.f90:42-49).
+"""
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+from . import abi
+
+BC = {"SLIPX": 1, "SLIPY": 2, "WALL": 3, "FREE": 8}
+
+DEFAULT_CONFIG = dict(
+    zcut=1.0e-8, zerocut=1.0e-40, dencut=1.0e-6, accut=1.0e-6,
+    cvisc1=0.5, cvisc2=0.75, cfl_sf=0.5, div_sf=0.25, zhg=0, zsp=0, ztq=0, pad0=0,
+    ccut=1.0e-6, pcut=1.0e-8,
+    time_start=0.0, time_end=1.0, dt_g=1.02, dt_min=1.0e-8, dt_max=1.0e-1,
+    dt_initial=1.0e-5,
+    mintime=np.finfo(np.float64).max, maxtime=np.finfo(np.float64).max, sf=0.5,
+    adv_type=1, zexist=0, zon=0, zeul=0)
+
+
+@dataclass
+class Problem:
+    name: str
+    ielnd: np.ndarray          # (4*nel) int32, 1-based, corners of element e at 4e..4e+3
+    ndx: np.ndarray
+    ndy: np.ndarray
+    indtype: np.ndarray        # (nnd) int32
+    ielreg: np.ndarray         # (nel) int32 1-based
+    ielmat: np.ndarray         # (nel) int32 1-based
+    density: np.ndarray
+    energy: np.ndarray
+    ndu: np.ndarray
+    ndv: np.ndarray
+    nmat: int
+    nreg: int
+    eos_itype: np.ndarray      # (nmat+1) int32, [0] = void
+    eos_param: np.ndarray      # (6*(nmat+1)) float64, material m at [6m:6m+6]
+    kappareg: np.ndarray = None
+    pmeritreg: np.ndarray = None
+    zdtnotreg: np.ndarray = None
+    zmidlength: np.ndarray = None
+    config: dict = field(default_factory=lambda: dict(DEFAULT_CONFIG))
+    nsteps: int = 1
+    stop_at_time_end: int = 0
+    dims: tuple = None
+
+    def __post_init__(self):
+        if self.kappareg is None:
+            self.kappareg = np.zeros(self.nreg)
+        if self.pmeritreg is None:
+            self.pmeritreg = np.zeros(self.nreg)
+        if self.zdtnotreg is None:
+            self.zdtnotreg = np.zeros(self.nreg, np.int32)
+        if self.zmidlength is None:
+            self.zmidlength = np.zeros(self.nreg, np.int32)
+        # hydro_cf_rationalise (hydro_config.f90:349,368): zhg/zsp from tables
+        self.config["zhg"] = int(np.any(self.kappareg > 0.0))
+        self.config["zsp"] = int(np.any(self.pmeritreg > 0.0))
+        # ale_cf_rationalise (ale_config.f90:539-551): Eulerian implies ALE on
+        if self.config.get("zeul"):
+            self.config["zexist"] = 1
+            self.config["mintime"] = -np.finfo(np.float64).max
+            self.config["maxtime"] = np.finfo(np.float64).max
+
+    @property
+    def nel(self):
+        return len(self.ielreg)
+
+    @property
+    def nnd(self):
+        return len(self.ndx)
+
+    def sizes(self):
+        s = abi.Sizes()
+        s.nel = s.nel1 = s.nel2 = self.nel
+        s.nnd = s.nnd1 = s.nnd2 = self.nnd
+        s.nsz = max(self.nel, self.nnd)
+        s.nmat, s.nreg = self.nmat, self.nreg
+        s.ncp = s.nmx = s.mcp = s.mmx = 0
+        return s
+
+    def cconfig(self):
+        c = abi.Config()
+        for k, v in self.config.items():
+            setattr(c, k, v)
+        return c
+
+
+def _corner_code(a, b):
+    """Corner BC code from the two adjacent side codes (setup_mesh.F90:886-1000)."""
+    s = {a, b}
+    if s == {1, 2}:
+        return 3
+    if 8 in s and len(s) == 2:
+        (o,) = s - {8}
+        return o
+    return max(a, b)
+
+
+def structured_mesh(nx, ny, x0, x1, y0, y1, bcs=("SLIPY", "SLIPX", "SLIPY", "SLIPX"),
+                    jitter=0.0, seed=12345):
+    """nx*ny quads on [x0,x1]x[y0,y1]; bcs = (bottom, right, top, left).
+
+    Node (i,j) has 1-based id 1+i+j*(nx+1); element (i,j) has id 1+i+j*nx with
+    corners (i,j),(i+1,j),(i+1,j+1),(i,j+1).  Interior nodes are displaced by
+    U(-jitter,jitter)*cell width in x and y (numpy default_rng(seed)).
+    """
+    xs = x0 + (x1 - x0) * np.arange(nx + 1, dtype=np.float64) / nx
+    ys = y0 + (y1 - y0) * np.arange(ny + 1, dtype=np.float64) / ny
+    X, Y = np.meshgrid(xs, ys)                   # (ny+1, nx+1)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        hx, hy = (x1 - x0) / nx, (y1 - y0) / ny
+        jx = rng.uniform(-jitter, jitter, size=(ny - 1, nx - 1)) * hx
+        jy = rng.uniform(-jitter, jitter, size=(ny - 1, nx - 1)) * hy
+        X[1:-1, 1:-1] += jx
+        Y[1:-1, 1:-1] += jy
+    ndx = np.ascontiguousarray(X.ravel())
+    ndy = np.ascontiguousarray(Y.ravel())
+    i = np.arange(nx)
+    j = np.arange(ny)
+    I, J = np.meshgrid(i, j)
+    n00 = (I + J * (nx + 1)).ravel()
+    ielnd = np.stack([n00, n00 + 1, n00 + 1 + (nx + 1), n00 + (nx + 1)], axis=1)
+    ielnd = (ielnd + 1).astype(np.int32).ravel()
+    bottom, right, top, left = (BC[b] for b in bcs)
+    code = np.zeros((ny + 1, nx + 1), np.int32)
+    code[0, :] = bottom
+    code[-1, :] = top
+    code[:, 0] = left
+    code[:, -1] = right
+    code[0, 0] = _corner_code(bottom, left)
+    code[0, -1] = _corner_code(bottom, right)
+    code[-1, 0] = _corner_code(top, left)
+    code[-1, -1] = _corner_code(top, right)
+    indtype = np.where(code > 0, -code, 1).astype(np.int32).ravel()
+    return ielnd, ndx, ndy, indtype
+
+
+def element_volume(ielnd, ndx, ndy, e):
+    """V of element e (0-based) with the exact operation order of
+    geometry_kn_getiso/getvolume (geometry_kn_volume.f90:49-85)."""
+    n = [int(ielnd[4 * e + k]) - 1 for k in range(4)]
+    x = [float(ndx[k]) for k in n]
+    y = [float(ndy[k]) for k in n]
+    a1 = 0.25 * (-x[0] + x[1] + x[2] - x[3])
+    a3 = 0.25 * (-x[0] - x[1] + x[2] + x[3])
+    b1 = 0.25 * (-y[0] + y[1] + y[2] - y[3])
+    b3 = 0.25 * (-y[0] - y[1] + y[2] + y[3])
+    return 4.0 * (a1 * b3 - a3 * b1)
+
+
+def centroids(ielnd, ndx, ndy):
+    n = ielnd.reshape(-1, 4) - 1
+    return ndx[n].mean(axis=1), ndy[n].mean(axis=1)
+
+
+def ideal_eos(gammas):
+    nmat = len(gammas)
+    itype = np.zeros(nmat + 1, np.int32)
+    param = np.zeros(abi.NEOSPARAM * (nmat + 1))
+    for m, g in enumerate(gammas, start=1):
+        itype[m] = 1
+        param[abi.NEOSPARAM * m] = g
+    return itype, param
+
+
+def sod(nx=200, ny=20, length=1.0, height=0.1, time_end=0.2, nsteps=None, eulerian=False,
+        **cfg):
+    """Sod shock tube: left rho=1 p=1, right rho=0.125 p=0.1, gamma=1.4, u=0;
+    interface at mid-length; SLIPY bottom/top, SLIPX ends.  Regions 1 (left)
+    and 2 (right), one material."""
+    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, length, 0.0, height)
+    xc, _ = centroids(ielnd, ndx, ndy)
+    left = xc < 0.5 * length
+    nel = nx * ny
+    density = np.where(left, 1.0, 0.125)
+    energy = np.where(left, 2.5, 2.0)   # e = p/((gamma-1) rho)
+    itype, param = ideal_eos([1.4])
+    config = dict(DEFAULT_CONFIG, time_end=time_end, **cfg)
+    if eulerian:
+        config["zeul"] = 1
+    return Problem(
+        name="sod_%dx%d%s" % (nx, ny, "_eul" if eulerian else ""),
+        ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+        ielreg=np.where(left, 1, 2).astype(np.int32), ielmat=np.ones(nel, np.int32),
+        density=density, energy=energy, ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)),
+        nmat=1, nreg=2, eos_itype=itype, eos_param=param, config=config,
+        nsteps=nsteps if nsteps is not None else 100000,
+        stop_at_time_end=1 if nsteps is None else 0, dims=(nx, ny))
+
+
+def sedov(n=1024, length=1.2, jitter=0.1, seed=12345, energy_total=0.25, gamma=1.4,
+          nsteps=500, kappa=0.0, **cfg):
+    """Sedov blast on [0,L]^2, rho=1, E deposited in the corner element at the
+    origin as e = E/(rho*V1), e = 0 elsewhere; interior nodes jittered by
+    U(-jitter,jitter) cell widths; reflective walls (SLIPY bottom/top, SLIPX
+    left/right)."""
+    ielnd, ndx, ndy, indtype = structured_mesh(n, n, 0.0, length, 0.0, length,
+                                               jitter=jitter, seed=seed)
+    nel = n * n
+    density = np.ones(nel)
+    energy = np.zeros(nel)
+    v1 = element_volume(ielnd, ndx, ndy, 0)
+    energy[0] = energy_total / (1.0 * v1)
+    itype, param = ideal_eos([gamma])
+    config = dict(DEFAULT_CONFIG, time_end=1.0e30, **cfg)
+    return Problem(
+        name="sedov_%dx%d" % (n, n), ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+        ielreg=np.ones(nel, np.int32), ielmat=np.ones(nel, np.int32), density=density,
+        energy=energy, ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)), nmat=1, nreg=1,
+        eos_itype=itype, eos_param=param, kappareg=np.array([kappa]), config=config,
+        nsteps=nsteps, dims=(n, n))
+
+
+def sod2mat(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps=300,
+            eulerian=True, **cfg):
+    """Two-material Sod on [0,L]x[0,L*ny/nx] (square cells): material 1
+    (gamma 1.4, rho 1, p 1) left of x = L/2 + A sin(2 pi k y/H + phi), material
+    2 (gamma 1.67, rho 0.125, p 0.1) right of it, k and phi drawn from the
+    seed.  Each element takes the material of its centroid (pure cells)."""
+    height = length * ny / nx
+    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, length, 0.0, height)
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(1, 4))
+    phi = float(rng.uniform(0.0, 2.0 * math.pi))
+    xc, yc = centroids(ielnd, ndx, ndy)
+    left = xc < 0.5 * length + amplitude * np.sin(2.0 * math.pi * k * yc / height + phi)
+    nel = nx * ny
+    g1, g2 = 1.4, 1.67
+    density = np.where(left, 1.0, 0.125)
+    energy = np.where(left, 1.0 / ((g1 - 1.0) * 1.0), 0.1 / ((g2 - 1.0) * 0.125))
+    itype, param = ideal_eos([g1, g2])
+    config = dict(DEFAULT_CONFIG, time_end=1.0e30, **cfg)
+    if eulerian:
+        config["zeul"] = 1
+    mat = np.where(left, 1, 2).astype(np.int32)
+    return Problem(
+        name="sod2mat_%dx%d" % (nx, ny), ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+        ielreg=mat.copy(), ielmat=mat, density=density, energy=energy,
+        ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)), nmat=2, nreg=2, eos_itype=itype,
+        eos_param=param, config=config, nsteps=nsteps, dims=(nx, ny))
+
+
+def noh(n=30, length=1.0, gamma=5.0 / 3.0, nsteps=100, **cfg):
+    """Noh implosion on a Cartesian quarter [0,L]^2: rho=1, e=0, radial inward
+    velocity of unit speed (the kinematic 'radial' IC of input/noh_small),
+    SLIPY bottom, SLIPX left, FREE top/right.  Strong convergence makes the
+    divergence control of hydro_kn_getdtdiv active."""
+    ielnd, ndx, ndy, indtype = structured_mesh(n, n, 0.0, length, 0.0, length,
+                                               bcs=("SLIPY", "FREE", "FREE", "SLIPX"))
+    r = np.sqrt(ndx * ndx + ndy * ndy)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ndu = np.where(r > 0.0, -ndx / r, 0.0)
+        ndv = np.where(r > 0.0, -ndy / r, 0.0)
+    nel = n * n
+    itype, param = ideal_eos([gamma])
+    config = dict(DEFAULT_CONFIG, time_end=0.6, dt_initial=1.0e-4, dt_max=1.0e-2)
+    config.update(cfg)
+    return Problem(
+        name="noh_%dx%d" % (n, n), ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+        ielreg=np.ones(nel, np.int32), ielmat=np.ones(nel, np.int32), density=np.ones(nel),
+        energy=np.zeros(nel), ndu=ndu, ndv=ndv, nmat=1, nreg=1, eos_itype=itype,
+        eos_param=param, config=config, nsteps=nsteps, dims=(n, n))
+
+
+def sod_deck_small(eulerian=False):
+    """The reference deck input/sod_sm_lag_small (or _eul_small): 100x25 on
+    [0,100]x[0,25], dt_initial 1e-3, time_end 20.  The mesh is a uniform
+    structured mesh rather than the deck's SOR-generated one."""
+    return sod(nx=100, ny=25, length=100.0, height=25.0, time_end=20.0,
+               eulerian=eulerian, dt_initial=1.0e-3)

@@ -1,0 +1,257 @@
+/*
+ * bookleaf_hip.h -- C-ABI of the MI355X (gfx950) implementation of BookLeaf's
+ * Lagrangian step and ALE remap.
+ *
+ * The library keeps a device-resident mirror of the dataAPI field registry
+ * (ids and shapes of src/dataAPI/dataAPI_id.f90:26-111 and
+ * src/utilities/data/data.f90:169-430) and exposes one entry point per
+ * reference driver on the hot path, so a Fortran host can replace the bodies
+ * of hydro_dr_* / ale_dr_* / geometry_dr_* / eos_dr_* / density_dr_* with
+ * iso_c_binding calls into this library and link the rest of the reference
+ * unchanged.  All arguments are plain C types: int32 sizes, FP64 scalars and
+ * raw host pointers.  Integer index fields (ielnd, ielel, ielfc, ielsort*)
+ * hold the reference's 1-based values unchanged.
+ *
+ * Conventions
+ *   - every function returns BL_SUCCESS (0) or BL_FAILURE (-1)
+ *     (dataAPI_params.f90:31-34); where an error_t is passed it is filled like
+ *     the reference fills it (ierr, iout = BL_HALT_ALL/BL_HALT_SINGLE, serr).
+ *   - calls are stream-ordered on the handle's own HIP stream; calls that
+ *     return a host scalar (getdt, getgeometry's negative-volume check,
+ *     downloads) synchronise that stream first.
+ *   - one partition per GPU: nel1=nel2=nel, nnd1=nnd2=nnd, and every Typhon
+ *     exchange of the reference is a no-op (nproc=1).
+ */
+#ifndef BOOKLEAF_HIP_H
+#define BOOKLEAF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- dataAPI_params.f90:25-34 ---------------------------------------- */
+#define BL_NCORN 4
+#define BL_NFACE 4
+#define BL_SUCCESS 0
+#define BL_FAILURE (-1)
+#define BL_HALT_ALL 1
+#define BL_HALT_SINGLE 2
+#define BL_SLEN 130
+#define BL_NEOSPARAM 6 /* setup_read.f90:518 eos_param(6,LN) */
+
+/* ---- field ids: dataAPI_id.f90:26-74 (1-based, identical numbering) --- */
+enum bl_field_id {
+  BL_ELDENSITY = 1, BL_ELENERGY = 2, BL_ELPRESSURE = 3, BL_ELCS2 = 4,
+  BL_ELVOLUME = 5, BL_A1 = 6, BL_A2 = 7, BL_A3 = 8, BL_B1 = 9, BL_B2 = 10,
+  BL_B3 = 11, BL_CNWT = 12, BL_CNX = 13, BL_CNY = 14,
+  BL_RSCRATCH21 = 15, BL_RSCRATCH22 = 16, BL_RSCRATCH23 = 17,
+  BL_RSCRATCH24 = 18, BL_RSCRATCH25 = 19, BL_RSCRATCH26 = 20,
+  BL_RSCRATCH27 = 21, BL_RSCRATCH28 = 22,
+  BL_CNVISCX = 23, BL_CNVISCY = 24, BL_ELVISC = 25, BL_ELMASS = 26,
+  BL_CNMASS = 27, BL_RSCRATCH11 = 28, BL_RSCRATCH12 = 29, BL_RSCRATCH13 = 30,
+  BL_RSCRATCH14 = 31, BL_RSCRATCH15 = 32, BL_RSCRATCH16 = 33,
+  BL_RSCRATCH17 = 34, BL_NDX = 35, BL_NDY = 36, BL_NDU = 37, BL_NDV = 38,
+  BL_INDTYPE = 39, BL_IELSORT1 = 40, BL_IELND = 41, BL_IELEL = 42,
+  BL_IELFC = 43, BL_IELMAT = 44, BL_IELREG = 45, BL_IELSORT2 = 46,
+  BL_SPMASS = 47, BL_ISCRATCH11 = 48, BL_ZSCRATCH11 = 49,
+  BL_IELLOCGLOB = 50, BL_INDLOCGLOB = 51,
+  BL_CPDENSITY = 52, BL_CPENERGY = 53, BL_CPPRESSURE = 54, BL_CPCS2 = 55,
+  BL_CPVOLUME = 56, BL_FRVOLUME = 57, BL_CPMASS = 58, BL_FRMASS = 59,
+  BL_CPVISCX = 60, BL_CPVISCY = 61, BL_CPVISC = 62, BL_CPA1 = 63,
+  BL_CPA3 = 64, BL_CPB1 = 65, BL_CPB3 = 66, BL_ICPMAT = 67, BL_ICPNEXT = 68,
+  BL_ICPPREV = 69, BL_IMXEL = 70, BL_IMXFCP = 71, BL_IMXNCP = 72,
+  BL_ICPSCRATCH11 = 73, BL_ICPSCRATCH12 = 74, BL_RCPSCRATCH11 = 75,
+  BL_RCPSCRATCH21 = 76, BL_RCPSCRATCH22 = 77, BL_RCPSCRATCH23 = 78,
+  BL_RCPSCRATCH24 = 79,
+  BL_NFIELD = 79
+};
+
+/* element type of a field, as registered by data.f90 (rlk / ink / lak) */
+enum bl_field_kind { BL_KIND_R64 = 1, BL_KIND_I32 = 2, BL_KIND_BOOL = 3 };
+
+/* dt control names of time_dr_advance.f90:38-72 and hydro_kn_getdt.f90:82,121 */
+enum bl_dt_control {
+  BL_DT_MAXIMUM = 1, BL_DT_GROWTH = 2, BL_DT_INITIAL = 3, BL_DT_CFL = 4,
+  BL_DT_DIV = 5
+};
+
+/* sizes_t, dataAPI_types.f90:41-44 */
+typedef struct bl_sizes {
+  int32_t nel, nel1, nel2, nnd, nnd1, nnd2, nsz, nmat, nreg, ncp, nmx, mcp,
+      mmx;
+} bl_sizes_t;
+
+/* Scalar configuration.  Fields mirror global_t (global_types.f90:25-27),
+ * hydro_t (hydro_types.f90:28-39), eos_t (eos_types.f90), time_t
+ * (time_types.f90) and ale_t (ale_types.f90:27-42).  Logical flags are int32
+ * 0/1.  Per-region and per-material tables are passed by bl_set_regions and
+ * bl_set_eos. */
+typedef struct bl_config {
+  double zcut, zerocut, dencut, accut;          /* global_t */
+  double cvisc1, cvisc2, cfl_sf, div_sf;        /* hydro_t  */
+  int32_t zhg, zsp, ztq, pad0;                  /* hydro_t  */
+  double ccut, pcut;                            /* eos_t    */
+  double time_start, time_end, dt_g, dt_min, dt_max, dt_initial; /* time_t */
+  double mintime, maxtime, sf;                  /* ale_t    */
+  int32_t adv_type, zexist, zon, zeul;          /* ale_t    */
+} bl_config_t;
+
+/* timestep_t, dataAPI_types.f90:46-52 (sdt/mdt strings are replaced by the
+ * bl_dt_control code; the host derives mdt from ielmat(idtel)) */
+typedef struct bl_timestep {
+  double time, dt, dts;
+  int32_t nstep, idtel, zcorrector, sdt;
+} bl_timestep_t;
+
+/* dt_t, dataAPI_types.f90:54-60 (without the linked-list pointer) */
+typedef struct bl_dt {
+  double rdt;
+  int32_t idt, sdt;
+} bl_dt_t;
+
+/* error_t, dataAPI_types.f90:76-79 */
+typedef struct bl_error {
+  int32_t ierr, iout;
+  char serr[BL_SLEN];
+} bl_error_t;
+
+/* per-step conservation totals in the sense of hydro_kn_calc_shortprint
+ * (hydro_kn_print.F90:56-110), summed over the whole mesh */
+typedef struct bl_totals {
+  double volume, mass, ie, ke, pressure_min, pressure_max, density_min,
+      density_max;
+} bl_totals_t;
+
+typedef struct bl_handle bl_handle_t;
+
+/* ---- lifetime (data.f90:129-167 data_set / data_setinit) -------------- */
+/* Allocate the device mirror of every dataAPI field for these sizes on the
+ * current HIP device.  Real fields are filled with RINIT=-2e12, integer
+ * fields with IINIT=-2e9 and logical fields with .FALSE. (data.f90:107-109),
+ * as the reference registry does.  sizes->nsz is recomputed as
+ * MAX(nel2,nnd2) (data.f90:165). */
+int bl_create(const bl_sizes_t* sizes, bl_handle_t** out);
+int bl_destroy(bl_handle_t* h);
+int bl_set_config(bl_handle_t* h, const bl_config_t* cfg);
+/* hydro_t per-region tables: kappareg, pmeritreg, zdtnotreg, zmidlength
+ * (hydro_types.f90:32-34); length nreg each, logicals as int32. */
+int bl_set_regions(bl_handle_t* h, int32_t nreg, const double* kappareg,
+                   const double* pmeritreg, const int32_t* zdtnotreg,
+                   const int32_t* zmidlength);
+/* eos_t tables after eos_cf_rationalise (eos_config.f90:163-226): itype and
+ * param for materials 0..nmat, param stored (BL_NEOSPARAM, 0:nmat). */
+int bl_set_eos(bl_handle_t* h, int32_t nmat, const int32_t* itype,
+               const double* param);
+
+/* ---- field transfer (the dh(id)%raddr / %iaddr / %zaddr arrays) ------- */
+int bl_field_info(bl_handle_t* h, int32_t id, int32_t* kind, int64_t* count);
+void* bl_field_device_ptr(bl_handle_t* h, int32_t id);
+int bl_upload(bl_handle_t* h, int32_t id, const void* host, int64_t count);
+int bl_download(bl_handle_t* h, int32_t id, void* host, int64_t count);
+/* Build the node->corner CSR maps from ielnd, ielsort1, ielsort2 (upload
+ * those first).  Two orderings: hydro (corner jj, then element in ielsort1
+ * order, hydro_kn_getacceleration.f90:53-72) and ALE (element in ielsort2
+ * order, then corner, ale_kn_advect.f90:96-124). */
+int bl_build_maps(bl_handle_t* h);
+/* Host-side mesh initialisation for hosts that do not run the reference's
+ * own init: ielel/ielfc from ielnd (init_kn_connectivity, init_kn.f90:82-91,
+ * utils_kn_connectivity.f90:30-170), corner node types (init_kn_nodetype,
+ * init_kn.f90:93-113), ielsort1/2 = identity (init_dr_utils.f90:64-69), then
+ * bl_build_maps.  Upload ielnd and indtype first. */
+int bl_init_mesh(bl_handle_t* h);
+int bl_synchronize(bl_handle_t* h);
+
+/* ---- initialisation drivers ------------------------------------------ */
+/* init_dr_state: elmass=rho*V, cnmass=rho*cnwt, then EOS
+ * (init_dr_utils.f90:93-117; init_kn.f90:32-48) */
+int bl_init_state(bl_handle_t* h);
+/* hydro_dr_init: zero viscosity, sub-zonal masses if zsp
+ * (hydro_dr_init.f90:37-57) */
+int bl_hydro_init(bl_handle_t* h);
+
+/* ---- Lagrangian step drivers ----------------------------------------- */
+/* hydro_dr_getdt (hydro_dr_getdt.f90:39-79): corner gather of ndu/ndv,
+ * edge viscosity, cs2 += visc, CFL and divergence controls. */
+int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div,
+                   bl_error_t* err);
+/* hydro_dr_setpredictor / setcorrector (hydro_dr_set.f90:31-51) */
+int bl_hydro_setpredictor(bl_handle_t* h);
+int bl_hydro_setcorrector(bl_handle_t* h);
+/* hydro_dr_getforce (hydro_dr_getforce.f90:40-91) */
+int bl_hydro_getforce(bl_handle_t* h, const bl_timestep_t* ts);
+/* hydro_dr_getstate (hydro_dr_getstate.f90:33-51): vertex update with
+ * 0.5*dt, geometry, density */
+int bl_hydro_getstate(bl_handle_t* h, const bl_timestep_t* ts,
+                      bl_error_t* err);
+/* hydro_dr_getenergy (hydro_dr_getenergy.f90:35-64) */
+int bl_hydro_getenergy(bl_handle_t* h, const bl_timestep_t* ts);
+/* hydro_dr_geteos / eos_dr_geteos (eos_dr_geteos.f90:35-66) */
+int bl_hydro_geteos(bl_handle_t* h);
+/* hydro_dr_getacceleration.f90:44-112, split as the reference splits it */
+int bl_hydro_initacceleration(bl_handle_t* h);
+int bl_hydro_scatteracceleration(bl_handle_t* h);
+int bl_hydro_getacceleration(bl_handle_t* h);
+int bl_hydro_applyacceleration(bl_handle_t* h, const bl_timestep_t* ts);
+
+/* ---- shared utilities ------------------------------------------------- */
+/* geometry_dr_getgeometry (geometry_dr_volume.f90:40-96); on a negative
+ * volume err->ierr = -iel of the LAST such element, iout=HALT_SINGLE */
+int bl_geometry_getgeometry(bl_handle_t* h, bl_error_t* err);
+/* density_dr_getdensity (density_dr_getdensity.f90:32-46) */
+int bl_density_getdensity(bl_handle_t* h);
+
+/* ---- ALE step drivers ------------------------------------------------- */
+/* ale_active (ale_dr_utils.f90:35-51) */
+int bl_ale_active(bl_handle_t* h, const bl_timestep_t* ts);
+int bl_ale_getmeshstatus(bl_handle_t* h);                  /* ale_dr_getmeshstatus.f90:32-49 */
+int bl_ale_getfluxvolume(bl_handle_t* h, const bl_timestep_t* ts); /* ale_dr_getfluxvolume.f90:37-71 */
+int bl_ale_advect(bl_handle_t* h, const bl_timestep_t* ts,
+                  bl_error_t* err);                         /* ale_dr_advect.f90:56-100 */
+int bl_ale_hydroupdate(bl_handle_t* h, bl_error_t* err);   /* ale_dr_update.f90:33-51 */
+
+/* ---- whole-step residency --------------------------------------------- */
+/* Run up to nsteps complete steps (inf_solver_getdt + inf_solver_lagstep +
+ * inf_solver_alestep, inf_control_solver.f90:53-69) with the dt selection of
+ * time_dr_calc/time_dr_end (time_dr_advance.f90:38-132) on the device and no
+ * host synchronisation inside the batch.  ts is read at entry and written at
+ * exit.  If stop_at_time_end != 0 the steps after the one that reaches
+ * time_end (time_dr_utils.f90:57-69) do nothing.  Optional history arrays
+ * (length nsteps, may be NULL) receive dt, idtel and the bl_dt_control code
+ * of every step taken.  *steps_taken receives the number of steps taken.
+ * Errors (negative volume, negative CFL, dt < dt_min) stop the batch at the
+ * failing step and are reported through err exactly as the host drivers
+ * would report them. */
+int bl_run(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end,
+           bl_timestep_t* ts, double* dt_hist, int32_t* idt_hist,
+           int32_t* ctl_hist, int32_t* steps_taken, bl_error_t* err);
+/* Same as bl_run, replaying one captured hipGraph per step (identical
+ * kernels and arguments; dt never leaves the device). */
+int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end,
+                 bl_timestep_t* ts, double* dt_hist, int32_t* idt_hist,
+                 int32_t* ctl_hist, int32_t* steps_taken, bl_error_t* err);
+
+/* Conservation totals over all elements (hydro_kn_print.F90:56-110):
+ * volume, mass of elements with m > dencut*V, IE = sum m*e,
+ * KE = sum_c 0.5*rho*cnwt_c*|u_c|^2 with u_c gathered from ndu/ndv. */
+int bl_totals(bl_handle_t* h, bl_totals_t* out);
+
+/* Kernel timing with HIP events on the handle's stream: average duration
+ * (ms) of the named kernel class over the launches recorded since the last
+ * bl_timing_reset. */
+int bl_timing_enable(bl_handle_t* h, int32_t on);
+int bl_timing_reset(bl_handle_t* h);
+int bl_timing_query(bl_handle_t* h, const char* name, double* avg_ms,
+                    int64_t* launches);
+/* comma-separated list of the kernel classes seen so far */
+int bl_timing_names(bl_handle_t* h, char* buf, int32_t buflen);
+
+/* last error message of the library (thread-local), for diagnostics */
+const char* bl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOOKLEAF_HIP_H */

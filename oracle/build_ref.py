@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Build the reference's own Fortran hot path into oracle/_ref (test oracle).
+
+TEST INFRASTRUCTURE ONLY.  Compiles the serial (no MPI, no OpenMP, local
+mesh set-up) source set of the reference *where it lies* under
+/root/reference/src with AMD flang, in module-dependency order, and links it
+with oracle/ref/ref_driver.f90 into oracle/_ref/bookleaf_ref.  Nothing is
+copied out of the reference tree; outputs go only to oracle/_ref/.
+
+The source set follows the reference's object lists for
+``NO_MPI=1 NO_OMP=1 LSETUP=1`` (src/utilities/makefile.UTILITIES:41-49,
+src/packages/io/driver/makefile.IODR, src/packages/setup/makefile.SETUP):
+T1 serial Typhon stubs instead of T3/MPI, no Silo/TyphonIO/METIS, no
+problem-specific MOD, and our own host program in place of
+inf_control_main.f90.  Flags: -O2 -ffp-contract=off (no FMA contraction),
+the same association rules as the HIP build.
+
+usage: python oracle/build_ref.py [--force]
+"""
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+SRC = "/root/reference/src"
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FFLAGS = ["-O2", "-ffp-contract=off", "-DNOMPI", "-DNOOMP", "-DLSETUP"]
+
+EXCLUDE_DIRS = ("mods", "utilities/cutils", "utilities/putils")
+EXCLUDE_FILES = {
+    "inf_control_main.f90",      # replaced by oracle/ref/ref_driver.f90
+    "io_dr_silo.f90", "io_kn_silo.f90",  # SILO only
+    "setup_metis.f90",           # METIS only
+    "io_dr_tio.F90",             # TyphonIO only (not in makefile.IODR)
+    "hydro_kn_getclosure.f90",   # not in makefile.HYDROKN (unbuildable stub)
+}
+INTRINSIC = {"iso_c_binding", "iso_fortran_env", "omp_lib", "mpi",
+             "ieee_arithmetic", "ieee_exceptions"}
+
+
+def reference_sources():
+    files = []
+    for root, _, names in os.walk(SRC):
+        rel = os.path.relpath(root, SRC)
+        if rel.startswith(EXCLUDE_DIRS):
+            continue
+        for n in names:
+            if not n.lower().endswith(".f90"):
+                continue
+            if n in EXCLUDE_FILES or n.startswith("t3_"):
+                continue
+            files.append(os.path.join(root, n))
+    return sorted(files)
+
+
+MOD_RE = re.compile(r"^\s*module\s+(\w+)\s*$", re.I)
+USE_RE = re.compile(r"^\s*use\s*(?:,\s*intrinsic\s*::)?\s*(\w+)", re.I)
+
+
+def scan(path):
+    defs, uses = set(), set()
+    with open(path, errors="replace") as f:
+        for line in f:
+            line = line.split("!")[0]
+            m = MOD_RE.match(line)
+            if m and m.group(1).lower() != "procedure":
+                defs.add(m.group(1).lower())
+            m = USE_RE.match(line)
+            if m:
+                uses.add(m.group(1).lower())
+    return defs, uses - INTRINSIC
+
+
+def order(files):
+    info = {f: scan(f) for f in files}
+    provider = {}
+    for f, (defs, _) in info.items():
+        for d in defs:
+            provider[d] = f
+    done, out = set(), []
+
+    def visit(f, stack):
+        if f in done:
+            return
+        if f in stack:
+            raise RuntimeError("module cycle at " + f)
+        stack.add(f)
+        for u in sorted(info[f][1]):
+            p = provider.get(u)
+            if p is not None and p != f:
+                visit(p, stack)
+        stack.discard(f)
+        done.add(f)
+        out.append(f)
+
+    for f in files:
+        visit(f, set())
+    return out
+
+
+def run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise SystemExit(1)
+
+
+def main():
+    force = "--force" in sys.argv
+    if not os.path.isdir(SRC):
+        print("reference source not present; oracle/_ref not built")
+        return 0
+    objdir = os.path.join(OUT, "obj")
+    moddir = os.path.join(OUT, "mod")
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(moddir, exist_ok=True)
+    exe = os.path.join(OUT, "bookleaf_ref")
+    files = order(reference_sources())
+    driver = os.path.join(HERE, "ref", "ref_driver.f90")
+    objs = []
+    newest = 0.0
+    for f in files + [driver]:
+        base = os.path.splitext(os.path.basename(f))[0]
+        obj = os.path.join(objdir, base + ".o")
+        objs.append(obj)
+        if force or not os.path.exists(obj) or os.path.getmtime(obj) < os.path.getmtime(f):
+            run([FLANG, "-c"] + FFLAGS + ["-module-dir", moddir, "-I", moddir,
+                                         f, "-o", obj])
+        newest = max(newest, os.path.getmtime(obj))
+    if force or not os.path.exists(exe) or os.path.getmtime(exe) < newest:
+        run([FLANG, "-O2"] + objs + ["-o", exe])
+    print("built", exe, "from", len(files), "reference sources")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
