@@ -18,8 +18,15 @@
 
 namespace bl {
 
+// Inside a captured step (ts != null) the kernels do nothing when the step is
+// gated off or ALE is inactive (ale_active, ale_dr_utils.f90:35-51), and take
+// dt from the device timestep.
+#define ALE_GATE(ts) \
+  if ((ts) && ((ts)->skip || !(ts)->ale_on)) return
+
 // ale_kn_getmeshstatus (Eulerian: indstatus = 2)
-__global__ void k_meshstatus(int nnd, int32_t* __restrict__ st) {
+__global__ void k_meshstatus(int nnd, int32_t* __restrict__ st, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nnd) st[i] = 2;
 }
@@ -29,7 +36,9 @@ __global__ void k_meshstatus(int nnd, int32_t* __restrict__ st) {
 __global__ void k_fluxvol_nodes(int nnd, int zeul, double dt, double* __restrict__ st4,
                                 double* __restrict__ st5, double* __restrict__ st2,
                                 double* __restrict__ st3, double* __restrict__ ndx,
-                                double* __restrict__ ndy) {
+                                double* __restrict__ ndy, const DevTimestep* ts) {
+  ALE_GATE(ts);
+  if (ts) dt = ts->dt;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnd) return;
   double u = st4[i], v = st5[i];
@@ -45,7 +54,8 @@ __global__ void k_fluxvol_nodes(int nnd, int zeul, double dt, double* __restrict
 __global__ void k_fluxvol_faces(int nel, double cut, const int32_t* __restrict__ ielnd,
                                 const double* __restrict__ x0, const double* __restrict__ y0,
                                 const double* __restrict__ x1, const double* __restrict__ y1,
-                                double* __restrict__ fcdv) {
+                                double* __restrict__ fcdv, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   int4 nd4 = reinterpret_cast<const int4*>(ielnd)[e];
@@ -118,7 +128,9 @@ __device__ __forceinline__ NbFc load_nbfc(int e, const int32_t* ielel, const int
 __global__ void k_el1(int nel, int id1, int id2, const int32_t* __restrict__ ielel,
                       const int32_t* __restrict__ ielfc, const double* __restrict__ fcdv,
                       const double* __restrict__ cnwt, const double* __restrict__ rho,
-                      double* __restrict__ rwork1, double* __restrict__ fcdm) {
+                      double* __restrict__ rwork1, double* __restrict__ fcdm,
+                      const DevTimestep* ts) {
+  ALE_GATE(ts);
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
@@ -138,7 +150,9 @@ __global__ void k_el2(int nel, int id1, int id2, double zerocut, double dencut,
                       double* __restrict__ cutm, double* __restrict__ vpr,
                       double* __restrict__ mpr, double* __restrict__ dpr,
                       double* __restrict__ vol, double* __restrict__ mass,
-                      double* __restrict__ rho, double* __restrict__ flux) {
+                      double* __restrict__ rho, double* __restrict__ flux,
+                      const DevTimestep* ts) {
+  ALE_GATE(ts);
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
@@ -163,7 +177,8 @@ __global__ void k_el3(int nel, int id1, int id2, const int32_t* __restrict__ iel
                       const int32_t* __restrict__ ielfc, const double* __restrict__ flux,
                       const double* __restrict__ mpr, const double* __restrict__ mass,
                       const double* __restrict__ cutm, double* __restrict__ rwork1,
-                      double* __restrict__ en) {
+                      double* __restrict__ en, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
@@ -182,7 +197,8 @@ __global__ void k_nd1(int nel, int id1, int id2, const int32_t* __restrict__ iel
                       double* __restrict__ cnu, double* __restrict__ cnv,
                       double* __restrict__ cnm0, double* __restrict__ cndv,
                       double* __restrict__ cndm, double* __restrict__ cnflux,
-                      double* __restrict__ cnmass) {
+                      double* __restrict__ cnmass, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
@@ -229,7 +245,8 @@ __global__ void k_nd2(int nnd, const int32_t* __restrict__ ptr, const int32_t* _
                       const double* __restrict__ cnflux, double* __restrict__ ndv0,
                       double* __restrict__ ndv1, double* __restrict__ ndm0,
                       double* __restrict__ ndm1, double* __restrict__ cutv,
-                      double* __restrict__ cutm) {
+                      double* __restrict__ cutm, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnd) return;
   double v0 = 0.0, v1 = 0.0, m0 = 0.0;
@@ -251,7 +268,8 @@ __global__ void k_nd2(int nnd, const int32_t* __restrict__ ptr, const int32_t* _
 __global__ void k_fluxndvl(int nel, const int32_t* __restrict__ ielel,
                            const int32_t* __restrict__ ielfc, const double* __restrict__ cb,
                            const double* __restrict__ cdb, const double* __restrict__ var,
-                           double* __restrict__ cnflux) {
+                           double* __restrict__ cnflux, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
@@ -321,7 +339,8 @@ __global__ void k_updatend(int nnd, const int32_t* __restrict__ ptr,
                            const double* __restrict__ base1, double* __restrict__ base1_out,
                            const double* __restrict__ cut,
                            const double* __restrict__ cnflux, double* __restrict__ ndflux,
-                           double* __restrict__ var) {
+                           double* __restrict__ var, const DevTimestep* ts) {
+  ALE_GATE(ts);
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnd) return;
   double s = 0.0;
@@ -335,6 +354,46 @@ __global__ void k_updatend(int nnd, const int32_t* __restrict__ ptr,
   if (act && b1 > cut[n]) var[n] = (var[n] * base0[n] + s) / b1;
 }
 
+// ale_dr_hydroupdate in one pass (ale_dr_update.f90:33-51): geometry on the
+// remapped mesh (gather, getiso, getvolume, checkvolume), rho = m/V, EOS
+__global__ void k_ale_update(int nel, const int32_t* __restrict__ ielnd,
+                             const double* __restrict__ ndx, const double* __restrict__ ndy,
+                             double* __restrict__ cnx, double* __restrict__ cny,
+                             double* __restrict__ a1, double* __restrict__ a2,
+                             double* __restrict__ a3, double* __restrict__ b1,
+                             double* __restrict__ b2, double* __restrict__ b3,
+                             double* __restrict__ cnwt, double* __restrict__ vol,
+                             const double* __restrict__ mass, double* __restrict__ rho,
+                             const double* __restrict__ en, double* __restrict__ pre,
+                             double* __restrict__ cs2, const int32_t* __restrict__ imat,
+                             const int32_t* __restrict__ itype,
+                             const double* __restrict__ param, double pcut, double ccut,
+                             DtResult* __restrict__ res, const DevTimestep* ts) {
+  ALE_GATE(ts);
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
+  double x[4] = {ndx[nd.x - 1], ndx[nd.y - 1], ndx[nd.z - 1], ndx[nd.w - 1]};
+  double y[4] = {ndy[nd.x - 1], ndy[nd.y - 1], ndy[nd.z - 1], ndy[nd.w - 1]};
+  Iso g;
+  getiso(x, y, g);
+  reinterpret_cast<double4*>(cnx)[e] = make_double4(x[0], x[1], x[2], x[3]);
+  reinterpret_cast<double4*>(cny)[e] = make_double4(y[0], y[1], y[2], y[3]);
+  a1[e] = g.a1; a2[e] = g.a2; a3[e] = g.a3;
+  b1[e] = g.b1; b2[e] = g.b2; b3[e] = g.b3;
+  reinterpret_cast<double4*>(cnwt)[e] = make_double4(g.wt[0], g.wt[1], g.wt[2], g.wt[3]);
+  vol[e] = g.vol;
+  if (g.vol < 0.0) atomicMax(&res->negvol, e + 1);
+  double r = mass[e] / g.vol;
+  rho[e] = r;
+  int m = imat[e];
+  m = m > 0 ? m : 0;
+  const double* p = param + BL_NEOSPARAM * m;
+  double ee = en[e];
+  pre[e] = eos_pressure(itype[m], p, r, ee, pcut);
+  cs2[e] = eos_cs2(itype[m], p, r, ee, pcut, ccut);
+}
+
 }  // namespace bl
 
 using namespace bl;
@@ -346,29 +405,50 @@ int bl_ale_active(bl_handle_t* h, const bl_timestep_t* ts) {
   return (ts->time >= h->cfg.mintime) && (ts->time <= h->cfg.maxtime);
 }
 
-int bl_ale_getmeshstatus(bl_handle_t* h) {
-  if (!h) return BL_FAILURE;
+}  // extern "C"
+
+namespace bl {
+
+static int meshstatus(bl_handle_t* h, const DevTimestep* dts) {
   if (h->cfg.zeul) {
     int nnd = h->sz.nnd;
-    BL_LAUNCH(h, "ale_meshstatus", k_meshstatus, nblocks(nnd), kBlock, nnd, h->I(BL_ISCRATCH11));
+    BL_LAUNCH(h, "ale_meshstatus", k_meshstatus, nblocks(nnd), kBlock, nnd,
+              h->I(BL_ISCRATCH11), dts);
   }
   return BL_SUCCESS;
 }
 
-int bl_ale_getfluxvolume(bl_handle_t* h, const bl_timestep_t* ts) {
-  if (!h || !ts) return BL_FAILURE;
+static int fluxvolume(bl_handle_t* h, double dt, const DevTimestep* dts) {
   int nnd = h->sz.nnd, nel = h->sz.nel;
   BL_LAUNCH(h, "ale_fluxvol_nodes", k_fluxvol_nodes, nblocks(nnd), kBlock, nnd, h->cfg.zeul,
-            ts->dt, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), h->R(BL_RSCRATCH12),
-            h->R(BL_RSCRATCH13), h->R(BL_NDX), h->R(BL_NDY));
+            dt, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), h->R(BL_RSCRATCH12),
+            h->R(BL_RSCRATCH13), h->R(BL_NDX), h->R(BL_NDY), dts);
   BL_LAUNCH(h, "ale_fluxvol_faces", k_fluxvol_faces, nblocks(nel), kBlock, nel,
             h->cfg.zerocut, h->I(BL_IELND), h->R(BL_RSCRATCH12), h->R(BL_RSCRATCH13),
-            h->R(BL_NDX), h->R(BL_NDY), h->R(BL_RSCRATCH21));
+            h->R(BL_NDX), h->R(BL_NDY), h->R(BL_RSCRATCH21), dts);
   return BL_SUCCESS;
 }
 
+}  // namespace bl
+
+extern "C" {
+
+int bl_ale_getmeshstatus(bl_handle_t* h) {
+  if (!h) return BL_FAILURE;
+  return meshstatus(h, nullptr);
+}
+
+int bl_ale_getfluxvolume(bl_handle_t* h, const bl_timestep_t* ts) {
+  if (!h || !ts) return BL_FAILURE;
+  return fluxvolume(h, ts->dt, nullptr);
+}
+
+}  // extern "C"
+
+namespace bl {
+
 // lazily allocated nsz-sized temporaries owned by the handle
-double* tmp_buffer(bl_handle_t* h, int k) {
+double* tmp_buffer(Handle* h, int k) {
   if (!h->d_tmp[k]) {
     if (hipMalloc(&h->d_tmp[k], sizeof(double) * std::max<int64_t>(h->sz.nsz, 1)) != hipSuccess) {
       set_error("out of device memory for temporaries");
@@ -378,66 +458,89 @@ double* tmp_buffer(bl_handle_t* h, int k) {
   return h->d_tmp[k];
 }
 
-static int advectel(bl_handle_t* h, int id1, int id2) {
+static int advectel(bl_handle_t* h, int id1, int id2, const DevTimestep* dts) {
   int nel = h->sz.nel;
   unsigned nb = nblocks(nel);
   // 0-based face passes
   int a = id1 - 1, b = id2 - 1;
   BL_LAUNCH(h, "ale_el1", k_el1, nb, kBlock, nel, a, b, h->I(BL_IELEL), h->I(BL_IELFC),
             h->R(BL_RSCRATCH21), h->R(BL_CNWT), h->R(BL_ELDENSITY), h->R(BL_RSCRATCH24),
-            h->R(BL_RSCRATCH22));
+            h->R(BL_RSCRATCH22), dts);
   BL_LAUNCH(h, "ale_el2", k_el2, nb, kBlock, nel, a, b, h->cfg.zerocut, h->cfg.dencut,
             h->I(BL_IELEL), h->I(BL_IELFC), h->R(BL_RSCRATCH22), h->R(BL_RSCRATCH24),
             h->R(BL_CNMASS), h->R(BL_ELENERGY), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH15),
             h->R(BL_RSCRATCH16), h->R(BL_RSCRATCH11), h->R(BL_RSCRATCH12), h->R(BL_RSCRATCH13),
-            h->R(BL_ELVOLUME), h->R(BL_ELMASS), h->R(BL_ELDENSITY), h->R(BL_RSCRATCH23));
+            h->R(BL_ELVOLUME), h->R(BL_ELMASS), h->R(BL_ELDENSITY), h->R(BL_RSCRATCH23), dts);
   BL_LAUNCH(h, "ale_el3", k_el3, nb, kBlock, nel, a, b, h->I(BL_IELEL), h->I(BL_IELFC),
             h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH12), h->R(BL_ELMASS), h->R(BL_RSCRATCH16),
-            h->R(BL_RSCRATCH24), h->R(BL_ELENERGY));
+            h->R(BL_RSCRATCH24), h->R(BL_ELENERGY), dts);
   return BL_SUCCESS;
 }
 
-static int advectnd(bl_handle_t* h, int id1, int id2) {
+static int advectnd(bl_handle_t* h, int id1, int id2, const DevTimestep* dts) {
   int nel = h->sz.nel, nnd = h->sz.nnd;
   unsigned nbe = nblocks(nel), nbn = nblocks(nnd);
   int a = id1 - 1, b = id2 - 1;
   BL_LAUNCH(h, "ale_nd1", k_nd1, nbe, kBlock, nel, a, b, h->I(BL_IELND), h->I(BL_IELEL),
             h->I(BL_IELFC), h->R(BL_NDU), h->R(BL_NDV), h->R(BL_RSCRATCH21),
             h->R(BL_RSCRATCH22), h->R(BL_RSCRATCH27), h->R(BL_RSCRATCH28), h->R(BL_RSCRATCH26),
-            h->R(BL_RSCRATCH24), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH23), h->R(BL_CNMASS));
+            h->R(BL_RSCRATCH24), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH23), h->R(BL_CNMASS), dts);
   double* ndm1 = tmp_buffer(h, 0);
   if (!ndm1) return BL_FAILURE;
   BL_LAUNCH(h, "ale_nd2", k_nd2, nbn, kBlock, nnd, h->d_ale_ptr, h->d_ale_idx, h->cfg.zerocut,
             h->cfg.dencut, h->R(BL_RSCRATCH11), h->R(BL_ELVOLUME), h->R(BL_RSCRATCH26),
             h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH13), h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH12),
-            ndm1, h->R(BL_RSCRATCH15), h->R(BL_RSCRATCH16));
+            ndm1, h->R(BL_RSCRATCH15), h->R(BL_RSCRATCH16), dts);
   BL_LAUNCH(h, "ale_fluxndvl", k_fluxndvl, nbe, kBlock, nel, h->I(BL_IELEL), h->I(BL_IELFC),
-            h->R(BL_RSCRATCH26), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH27), h->R(BL_RSCRATCH23));
+            h->R(BL_RSCRATCH26), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH27), h->R(BL_RSCRATCH23),
+            dts);
   BL_LAUNCH(h, "ale_updatend", k_updatend, nbn, kBlock, nnd, h->d_el_ptr, h->d_el_idx, -1,
             h->I(BL_ISCRATCH11), h->I(BL_INDTYPE), h->Z(BL_ZSCRATCH11), 0, h->R(BL_RSCRATCH12),
             (const double*)ndm1, h->R(BL_RSCRATCH11), h->R(BL_RSCRATCH16), h->R(BL_RSCRATCH23),
-            h->R(BL_RSCRATCH22), h->R(BL_NDU));
+            h->R(BL_RSCRATCH22), h->R(BL_NDU), dts);
   BL_LAUNCH(h, "ale_fluxndvl", k_fluxndvl, nbe, kBlock, nel, h->I(BL_IELEL), h->I(BL_IELFC),
-            h->R(BL_RSCRATCH26), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH28), h->R(BL_RSCRATCH23));
+            h->R(BL_RSCRATCH26), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH28), h->R(BL_RSCRATCH23),
+            dts);
   BL_LAUNCH(h, "ale_updatend", k_updatend, nbn, kBlock, nnd, h->d_el_ptr, h->d_el_idx, -2,
             h->I(BL_ISCRATCH11), h->I(BL_INDTYPE), h->Z(BL_ZSCRATCH11), 1, h->R(BL_RSCRATCH12),
             (const double*)h->R(BL_RSCRATCH11), (double*)nullptr, h->R(BL_RSCRATCH16),
-            h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH22), h->R(BL_NDV));
+            h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH22), h->R(BL_NDV), dts);
   return BL_SUCCESS;
 }
+
+// the ALE step inside a captured step: mesh status, flux volumes, unsplit
+// advection (adv_type=1) and the hydro update, gated by the device timestep
+int ale_step_device(Handle* hb, const DevTimestep* d_ts) {
+  bl_handle_t* h = static_cast<bl_handle_t*>(hb);
+  if (meshstatus(h, d_ts) || fluxvolume(h, 0.0, d_ts) || advectel(h, 1, 2, d_ts) ||
+      advectnd(h, 1, 2, d_ts))
+    return BL_FAILURE;
+  int nel = h->sz.nel;
+  BL_LAUNCH(h, "ale_update", k_ale_update, nblocks(nel), kBlock, nel, h->I(BL_IELND),
+            h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1), h->R(BL_A2),
+            h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
+            h->R(BL_ELVOLUME), h->R(BL_ELMASS), h->R(BL_ELDENSITY), h->R(BL_ELENERGY),
+            h->R(BL_ELPRESSURE), h->R(BL_ELCS2), h->I(BL_IELMAT), h->d_eos_itype,
+            h->d_eos_param, h->cfg.pcut, h->cfg.ccut, h->d_dtres, d_ts);
+  return BL_SUCCESS;
+}
+
+}  // namespace bl
+
+extern "C" {
 
 int bl_ale_advect(bl_handle_t* h, const bl_timestep_t* ts, bl_error_t* err) {
   if (!h || !ts) return BL_FAILURE;
   if (!h->maps_built) { set_error("bl_ale_advect: maps not built"); return BL_FAILURE; }
   switch (h->cfg.adv_type) {
     case 1:
-      if (advectel(h, 1, 2) || advectnd(h, 1, 2)) return BL_FAILURE;
+      if (advectel(h, 1, 2, nullptr) || advectnd(h, 1, 2, nullptr)) return BL_FAILURE;
       break;
     case 2: {
       int ii = (ts->nstep + 1) % 2;
       int i1 = 1 + ii, i2 = 2 - ii, i3 = i2 - i1;
       for (ii = i1; (i3 > 0) ? (ii <= i2) : (ii >= i2); ii += i3)
-        if (advectel(h, ii, ii) || advectnd(h, ii, ii)) return BL_FAILURE;
+        if (advectel(h, ii, ii, nullptr) || advectnd(h, ii, ii, nullptr)) return BL_FAILURE;
       break;
     }
     default:

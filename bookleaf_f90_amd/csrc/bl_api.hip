@@ -117,6 +117,11 @@ int timing_begin(Handle* h, const char* cls) {
 
 void timing_end(Handle* h, int slot) { hipEventRecord(h->ev_pool[2 * slot + 1], h->stream); }
 
+void invalidate_graph(Handle* h) {
+  for (auto& g : h->graph_exec)
+    if (g) { hipGraphExecDestroy(g); g = nullptr; }
+}
+
 // ---------------------------------------------------------------------------
 // element / node kernels of the Lagrangian step
 // ---------------------------------------------------------------------------
@@ -488,15 +493,13 @@ __global__ void k_getdt_b(int nel, double zerocut, double cvisc1, double cvisc2,
   ellength[e] = zdtnotreg[ireg] ? DBL_MIN : minlength(x, y, zcut, zmidlength[ireg] != 0);
 }
 
-// (c) hydro_dr_getcs2 (cs2 += elvisc) and the CFL ratio with its min reduction;
-//     the last workgroup combines both reductions (hydro_kn_getdt.f90:67-80)
+// (c) hydro_dr_getcs2 (cs2 += elvisc) and the CFL ratio with its min reduction
+//     (hydro_kn_getdt.f90:50-73)
 __global__ void k_getdt_c(int nel, const int32_t* __restrict__ ielreg,
                           const int32_t* __restrict__ zdtnotreg,
                           const double* __restrict__ elvisc, double* __restrict__ cs2,
                           const double* __restrict__ ellength, double* __restrict__ rs11,
-                          double* __restrict__ part, int32_t* __restrict__ parti,
-                          uint32_t* __restrict__ ticket, int nblk_a,
-                          DtResult* __restrict__ res) {
+                          double* __restrict__ part, int32_t* __restrict__ parti, int nblk_a) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   double r = DBL_MAX;
   if (e < nel) {
@@ -506,15 +509,14 @@ __global__ void k_getdt_c(int nel, const int32_t* __restrict__ ielreg,
     rs11[e] = r;
   }
   block_argmin_first(r, e + 1, part + 2 * nblk_a, parti + 2 * nblk_a);
-  // last workgroup: combine the per-workgroup partials of (a) and (c)
-  __shared__ bool last;
-  __threadfence();
-  if (threadIdx.x == 0) last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  final_dt_reduce(nblk_a, gridDim.x, part, parti, res);
-  if (threadIdx.x == 0) *ticket = 0u;
+}
+
+// (d) one workgroup combines the per-workgroup partials of (a) and (c); a
+//     separate launch instead of a "last workgroup" ticket, which would need
+//     a device-scope fence in every workgroup (an L2 write-back per XCD)
+__global__ void k_dt_final(int nblk_a, int nblk_c, const double* __restrict__ part,
+                           const int32_t* __restrict__ parti, DtResult* __restrict__ res) {
+  final_dt_reduce(nblk_a, nblk_c, part, parti, res);
 }
 
 }  // namespace bl
@@ -594,8 +596,15 @@ int bl_create(const bl_sizes_t* sizes, bl_handle_t** out) {
 int bl_destroy(bl_handle_t* h) {
   if (!h) return BL_SUCCESS;
   if (h->stream) hipStreamSynchronize(h->stream);
-  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
-  for (int id = 1; id <= BL_NFIELD; ++id) if (h->f[id]) hipFree(h->f[id]);
+  invalidate_graph(h);
+  const bool sets = h->nodeset[1][0] != nullptr;
+  for (int id = 1; id <= BL_NFIELD; ++id) {
+    bool node = id == BL_NDX || id == BL_NDY || id == BL_NDU || id == BL_NDV;
+    if (h->f[id] && !(sets && node)) hipFree(h->f[id]);
+  }
+  if (sets)
+    for (int s = 0; s < 2; ++s)
+      for (int k = 0; k < 4; ++k) hipFree(h->nodeset[s][k]);
   void* bufs[] = {h->d_kappareg, h->d_pmeritreg, h->d_zdtnotreg, h->d_zmidlength,
                   h->d_eos_itype, h->d_eos_param, h->d_hyd_ptr, h->d_hyd_idx,
                   h->d_ale_ptr, h->d_ale_idx, h->d_el_ptr, h->d_el_idx, h->d_part,
@@ -610,9 +619,6 @@ int bl_destroy(bl_handle_t* h) {
   return BL_SUCCESS;
 }
 
-static void invalidate_graph(bl_handle_t* h) {
-  if (h->graph_exec) { hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
-}
 
 int bl_set_config(bl_handle_t* h, const bl_config_t* cfg) {
   if (!h || !cfg) { set_error("bl_set_config: null argument"); return BL_FAILURE; }
@@ -917,7 +923,9 @@ int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) 
             h->R(BL_RSCRATCH12));
   BL_LAUNCH(h, "getdt_c", k_getdt_c, nb, kBlock, nel, h->I(BL_IELREG), h->d_zdtnotreg,
             h->R(BL_ELVISC), h->R(BL_ELCS2), h->R(BL_RSCRATCH12), h->R(BL_RSCRATCH11),
-            h->d_part, h->d_parti, h->d_ticket, (int)nb, h->d_dtres);
+            h->d_part, h->d_parti, (int)nb);
+  BL_LAUNCH(h, "dt_final", k_dt_final, 1, kBlock, (int)nb, (int)nb, h->d_part, h->d_parti,
+            h->d_dtres);
   BL_HIP(hipMemcpyAsync(h->h_dtres, h->d_dtres, sizeof(DtResult), hipMemcpyDeviceToHost, h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));
   const DtResult& r = *h->h_dtres;

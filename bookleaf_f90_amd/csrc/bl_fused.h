@@ -1,0 +1,80 @@
+// bl_fused.h -- kernel arguments of the device-resident step (bl_fused.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bl_internal.h"
+
+namespace bl {
+
+struct FusedNodes {
+  const double* x;
+  const double* y;
+  const double* u;
+  const double* v;
+};
+struct FusedNodesOut {
+  double* x;
+  double* y;
+  double* u;
+  double* v;
+};
+
+// everything a step kernel needs, passed by value (kernel argument space)
+struct FusedArgs {
+  int nel, nnd;
+  int reg_tables, reg_tables_force, zsp, zhg, zexist, write_geometry;
+  double zcut, zerocut, dencut, accut2, cvisc1, cvisc2, cfl_sf, div_sf, ccut, pcut;
+  double dt_g, dt_min, dt_max, dt_initial, mintime, maxtime;
+  // connectivity
+  const int32_t* ielnd;
+  const int32_t* ielel;
+  const int32_t* ielfc;
+  const int32_t* ielreg;
+  const int32_t* ielmat;
+  const int32_t* indtype;
+  const int32_t* hyd_ptr;
+  const int32_t* hyd_idx;
+  // tables
+  const int32_t* zdtnotreg;
+  const int32_t* zmidlength;
+  const double* kappareg;
+  const double* pmeritreg;
+  const int32_t* eos_itype;
+  const double* eos_param;
+  // element fields
+  double* rho;
+  double* en;
+  double* pre;
+  double* cs2;
+  double* vol;
+  const double* mass;
+  const double* cnmass;
+  const double* spmass;
+  double* cnwt;
+  double* cnx;
+  double* cny;
+  double* a1;
+  double* a2;
+  double* a3;
+  double* b1;
+  double* b2;
+  double* b3;
+  double* cnviscx;
+  double* cnviscy;
+  double* elvisc;
+  double* cnfx;  // rscratch23 (lagstep cnfx)
+  double* cnfy;  // rscratch24
+  double* ubar;  // rscratch14 (lagstep ndubar)
+  double* vbar;  // rscratch15
+  // reductions and device timestep
+  double* part;
+  int32_t* parti;
+  uint32_t* ticket;
+  DtResult* dtres;
+  DevTimestep* ts;
+  double* hist;
+  int hist_cap;
+};
+
+}  // namespace bl

@@ -1,0 +1,628 @@
+// bl_fused.hip -- the device-resident whole step (bl_run_graph).
+//
+// One Lagrangian step (inf_solver_getdt + inf_solver_lagstep) in four
+// full-grid kernels plus a one-workgroup step prologue, with dt, time and the
+// error state kept on the device (time_dr_calc/time_dr_end run in the last
+// workgroup of F1), so a step can be captured as a hipGraph and replayed
+// with no host synchronisation:
+//
+//   F0 begin   : time_end / error gate for the step, per-step counters
+//   F1 getdt   : corner gather, edge viscosity with the Christensen limiter
+//                (neighbour edges recomputed from node data), cnviscx/y,
+//                elvisc, CFL and divergence candidates -> dt on the device
+//   F2 force   : predictor force, half-step geometry, energy and EOS in
+//                registers, then the corrector's corner forces P+Q(+SP)(+HG)
+//                and half-step cnwt, rho (the corrector force of an element
+//                only needs that element's predictor results)
+//   F3 accel   : node gather through the (corner, element) CSR map in the
+//                reference's summation order: mass, area, force ->
+//                acceleration, BCs, u^{n+1}, ubar, x^{n+1} into the other
+//                node buffer (ping-pong: no node is read after it is written)
+//   F4 state   : corrector geometry, density, energy with ubar, EOS
+//
+// Every FP64 expression is the same function of the same inputs as in the
+// per-driver path (bl_math.h), so the results are bit-identical to the
+// reference drivers; the intermediate half-step pressure/energy never reach
+// HBM.  Fields that hold only phase scratch in the reference (rscratch*) are
+// not written except ubar (rscratch14/15, which the ALE step reads) and the
+// corrector corner forces (rscratch23/24).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+
+#include "bl_fused.h"
+#include "bl_internal.h"
+#include "bl_kernels.h"
+#include "bl_math.h"
+
+namespace bl {
+
+// ---------------------------------------------------------------- F0
+__global__ void k_f0_begin(DevTimestep* __restrict__ ts, int stop_at_time_end,
+                           double time_end, DtResult* __restrict__ res) {
+  if (threadIdx.x != 0) return;
+  int skip = ts->done;
+  if (!skip && stop_at_time_end && ts->time >= time_end) {
+    ts->done = 1;
+    skip = 1;
+  }
+  ts->skip = skip;
+  res->negvol = 0;
+  res->negvol2 = 0;
+}
+
+__device__ __forceinline__ void load4(const double* __restrict__ a, const int nd[4],
+                                      double v[4]) {
+  v[0] = a[nd[0]]; v[1] = a[nd[1]]; v[2] = a[nd[2]]; v[3] = a[nd[3]];
+}
+__device__ __forceinline__ void load_nd(const int32_t* __restrict__ ielnd, int e, int nd[4]) {
+  int4 q = reinterpret_cast<const int4*>(ielnd)[e];
+  nd[0] = q.x - 1; nd[1] = q.y - 1; nd[2] = q.z - 1; nd[3] = q.w - 1;
+}
+__device__ __forceinline__ void st4(double* __restrict__ a, int e, const double v[4]) {
+  reinterpret_cast<double4*>(a)[e] = make_double4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void ld4(const double* __restrict__ a, int e, double v[4]) {
+  double4 q = reinterpret_cast<const double4*>(a)[e];
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+
+// edge f (corner f -> f+1) differences of element `el` from node data
+__device__ __forceinline__ void edge_of(const int32_t* __restrict__ ielnd,
+                                        const FusedNodes& N, int el, int f, double& du,
+                                        double& dv, double& dx, double& dy) {
+  const int32_t* q = ielnd + 4 * el;
+  int a = q[f] - 1, b = q[(f + 1) & 3] - 1;
+  du = N.u[b] - N.u[a];
+  dv = N.v[b] - N.v[a];
+  dx = N.x[b] - N.x[a];
+  dy = N.y[b] - N.y[a];
+}
+
+// ---------------------------------------------------------------- F1
+__global__ void __launch_bounds__(kBlock)
+k_f1_getdt(FusedArgs A, FusedNodes N) {
+  DevTimestep* ts = A.ts;
+  if (ts->skip) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double cflv = DBL_MAX, divv = -1.0;
+  if (e < A.nel) {
+    int nd[4];
+    load_nd(A.ielnd, e, nd);
+    double x[4], y[4], u[4], v[4];
+    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+    const int4 nb4 = reinterpret_cast<const int4*>(A.ielel)[e];
+    const int4 fc4 = reinterpret_cast<const int4*>(A.ielfc)[e];
+    const int nb[4] = {nb4.x - 1, nb4.y - 1, nb4.z - 1, nb4.w - 1};
+    const int fc[4] = {fc4.x - 1, fc4.y - 1, fc4.z - 1, fc4.w - 1};
+    double ou[4], ov[4], ox[4], oy[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      int g = (f + 1) & 3;
+      ou[f] = u[g] - u[f]; ov[f] = v[g] - v[f]; ox[f] = x[g] - x[f]; oy[f] = y[g] - y[f];
+    }
+    const double cs2e = A.cs2[e], rhoe = A.rho[e];
+    double ev = 0.0, ed[4];
+#pragma unroll
+    for (int iside = 0; iside < 2; ++iside) {
+      const int is1 = (iside + 3) & 3, is2 = iside + 1;
+      const int in1 = nb[iside], in2 = nb[iside + 2];
+      double s1, s2, s3, s4;
+      double au, av, ax, ay, bu, bv, bx, by;
+      edge_of(A.ielnd, N, in1, (fc[iside] + 1) & 3, au, av, ax, ay);
+      edge_of(A.ielnd, N, in2, (fc[iside + 2] + 3) & 3, bu, bv, bx, by);
+      limiter_ratios(ou[is1], ov[is1], ox[is1], oy[is1], au, av, ax, ay, bu, bv, bx, by,
+                     A.zerocut, s1, s2);
+      edge_of(A.ielnd, N, in1, (fc[iside] + 3) & 3, au, av, ax, ay);
+      edge_of(A.ielnd, N, in2, (fc[iside + 2] + 1) & 3, bu, bv, bx, by);
+      limiter_ratios(ou[is2], ov[is2], ox[is2], oy[is2], au, av, ax, ay, bu, bv, bx, by,
+                     A.zerocut, s3, s4);
+      if (in1 == e) {
+        int ic1 = nd[iside], ic2 = nd[(iside + 1) & 3];
+        if (A.indtype[ic1] < 0 && A.indtype[ic2] < 0 && in2 != e) { s1 = 1.0; s3 = 1.0; }
+        else { s1 = 0.0; s3 = 0.0; }
+      }
+      if (in2 == e) {
+        int ic1 = nd[iside + 2], ic2 = nd[(iside + 3) & 3];
+        if (A.indtype[ic1] < 0 && A.indtype[ic2] < 0 && in1 != e) { s2 = 1.0; s4 = 1.0; }
+        else { s2 = 0.0; s4 = 0.0; }
+      }
+      int ins = nb[is1];
+      ed[is1] = limiter_edge(s1, s2, ou[is1], ov[is1], cs2e, A.cs2[ins], rhoe, A.rho[ins],
+                             A.cvisc1, A.cvisc2, A.zerocut, ev);
+      ins = nb[is2];
+      ed[is2] = limiter_edge(s3, s4, ou[is2], ov[is2], cs2e, A.cs2[ins], rhoe, A.rho[ins],
+                             A.cvisc1, A.cvisc2, A.zerocut, ev);
+    }
+    double qx[4], qy[4];
+    geteviscosity(x, y, u, v, ed, ed, A.zerocut, qx, qy);
+    st4(A.cnviscx, e, qx);
+    st4(A.cnviscy, e, qy);
+    A.elvisc[e] = ev;
+    // CFL (hydro_kn_getdtcfl) with cs2 + elvisc (hydro_dr_getcs2)
+    const double c2 = cs2e + ev;
+    int dtnot = 0, mid = 0;
+    if (A.reg_tables) {
+      int ireg = A.ielreg[e] - 1;
+      dtnot = A.zdtnotreg[ireg];
+      mid = A.zmidlength[ireg];
+    }
+    cflv = dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
+    // divergence (hydro_kn_getdtdiv) from the geometry at x^n
+    Iso g;
+    getiso(x, y, g);
+    divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
+  }
+  const int nb = gridDim.x;
+  block_argmax_first(divv, e + 1, A.part, A.parti);
+  __syncthreads();
+  block_argmin_first(cflv, e + 1, A.part + 2 * nb, A.parti + 2 * nb);
+}
+
+// F1b: one workgroup -- combine the partials of F1, then the hydro_dr_getdt
+// candidates and time_dr_calc / time_dr_end (time_dr_advance.f90:38-132)
+__global__ void __launch_bounds__(kBlock) k_f1b_select(FusedArgs A, int nb) {
+  DevTimestep* ts = A.ts;
+  if (ts->skip) return;
+  final_dt_reduce(nb, nb, A.part, A.parti, A.dtres);
+  if (threadIdx.x != 0) return;
+  const DtResult& r = *A.dtres;
+  if (r.cfl_min < 0.0) {  // hydro_kn_getdt.f90:75-78
+    ts->err = 2; ts->done = 1; ts->skip = 1; ts->err_step = ts->nstep + 1;
+    return;
+  }
+  double best = A.dt_max;
+  int bidx = -1, bsdt = BL_DT_MAXIMUM;
+  double c = A.dt_g * ts->dt;
+  if (c < best) { best = c; bidx = -1; bsdt = BL_DT_GROWTH; }
+  if (ts->nstep == 0 && A.dt_initial < best) { best = A.dt_initial; bidx = -1; bsdt = BL_DT_INITIAL; }
+  c = A.cfl_sf * sqrt(r.cfl_min);
+  if (c < best) { best = c; bidx = r.cfl_idx; bsdt = BL_DT_CFL; }
+  c = A.div_sf / r.div_max;
+  if (c < best) { best = c; bidx = r.div_idx; bsdt = BL_DT_DIV; }
+  ts->nstep = ts->nstep + 1;
+  ts->dt = best;
+  ts->idtel = bidx;
+  ts->sdt = bsdt;
+  ts->time = ts->time + best;
+  ts->dts = best;
+  ts->ale_on = A.zexist && (ts->time >= A.mintime) && (ts->time <= A.maxtime);
+  if (A.hist && ts->hist_n < A.hist_cap) {
+    int k = ts->hist_n++;
+    A.hist[3 * k] = best;
+    A.hist[3 * k + 1] = (double)bidx;
+    A.hist[3 * k + 2] = (double)bsdt;
+  }
+  if (best < A.dt_min) {  // time_dr_advance.f90:122-127
+    ts->err = 3; ts->done = 1; ts->skip = 1; ts->err_step = ts->nstep;
+  }
+}
+
+// ---------------------------------------------------------------- F2
+__global__ void __launch_bounds__(kBlock)
+k_f2_force(FusedArgs A, FusedNodes N) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.nel) return;
+  const double dt = ts->dt;
+  const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
+  int nd[4];
+  load_nd(A.ielnd, e, nd);
+  double x[4], y[4], u[4], v[4];
+  load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+  double qx[4], qy[4];
+  ld4(A.cnviscx, e, qx);
+  ld4(A.cnviscy, e, qy);
+  int ireg = 0;
+  if (A.reg_tables_force) ireg = A.ielreg[e] - 1;
+  // --- predictor: hydro_dr_getforce at x^n with p^n
+  Iso g;
+  getiso(x, y, g);
+  double fx[4], fy[4];
+  forceP(A.pre[e], g.a1, g.a3, g.b1, g.b3, fx, fy);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) { fx[c] = fx[c] + qx[c]; fy[c] = fy[c] + qy[c]; }
+  if (A.zsp) {
+    double spm[4];
+    ld4(A.spmass, e, spm);
+    // elcs2 at this point holds cs2 + elvisc (hydro_dr_getcs2 in getdt)
+    forceSP(A.pmeritreg[ireg], A.rho[e], A.cs2[e] + A.elvisc[e], x, y, spm, fx, fy);
+  }
+  // --- predictor getstate: vertex, geometry, density; energy; EOS
+  double xh[4], yh[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) { xh[c] = x[c] + h * u[c]; yh[c] = y[c] + h * v[c]; }
+  Iso gh;
+  getiso(xh, yh, gh);
+  if (gh.vol < 0.0) atomicMax(&A.dtres->negvol, e + 1);
+  const double m = A.mass[e];
+  const double rhoh = m / gh.vol;
+  const double eh = energy_update(A.en[e], m, fx, fy, u, v, h, A.zerocut);
+  int mat = A.ielmat[e];
+  mat = mat > 0 ? mat : 0;
+  const int et = A.eos_itype[mat];
+  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
+  const double ph = eos_pressure(et, ep, rhoh, eh, A.pcut);
+  // --- corrector force at the half step (zcorrector: HG on)
+  forceP(ph, gh.a1, gh.a3, gh.b1, gh.b3, fx, fy);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) { fx[c] = fx[c] + qx[c]; fy[c] = fy[c] + qy[c]; }
+  if (A.zsp) {
+    double spm[4];
+    ld4(A.spmass, e, spm);
+    const double cs2h = eos_cs2(et, ep, rhoh, eh, A.pcut, A.ccut);
+    forceSP(A.pmeritreg[ireg], rhoh, cs2h, xh, yh, spm, fx, fy);
+  }
+  if (A.zhg) forceHG(A.kappareg[ireg], rhoh, gh.vol, 1.0 / dt, u, v, fx, fy);
+  st4(A.cnfx, e, fx);
+  st4(A.cnfy, e, fy);
+  st4(A.cnwt, e, gh.wt);
+  A.rho[e] = rhoh;
+}
+
+// ---------------------------------------------------------------- F3
+__global__ void __launch_bounds__(kBlock)
+k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || A.dtres->negvol) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= A.nnd) return;
+  const double dt = ts->dt;
+  const double h = 0.5 * dt;
+  double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
+  const int b = A.hyd_ptr[n], end = A.hyd_ptr[n + 1];
+  for (int k = b; k < end; ++k) {
+    const int c = A.hyd_idx[k];
+    const int el = c >> 2, j = c & 3;
+    const double cm = A.cnmass[c];
+    if (cm > A.zerocut) {
+      m = m + cm;
+    } else {
+      const double cp = A.cnmass[4 * el + ((j + 3) & 3)];
+      if (cp > A.zerocut) m = m + cp;
+      else m = m + A.rho[el] * A.cnwt[c];
+    }
+    a = a + A.cnwt[c];
+    fu = fu + A.cnfx[c];
+    fv = fv + A.cnfy[c];
+  }
+  // hydro_kn_getacceleration + utils_kn_bc
+  const double w1 = A.dencut * a;
+  if (m > w1) { fu = fu / m; fv = fv / m; }
+  else { fu = 0.0; fv = 0.0; }
+  const int t = A.indtype[n];
+  if (t == -1) fu = 0.0;
+  else if (t == -2) fv = 0.0;
+  else if (t == -3) { fu = 0.0; fv = 0.0; }
+  if (fu * fu + fv * fv < A.accut2) { fu = 0.0; fv = 0.0; }
+  // hydro_kn_applyacceleration, then the corrector's geometry_kn_getvertex
+  const double u0 = N.u[n], v0 = N.v[n];
+  const double u1 = u0 + dt * fu, v1 = v0 + dt * fv;
+  const double dt05 = 0.5 * dt;
+  A.ubar[n] = u0 + dt05 * fu;
+  A.vbar[n] = v0 + dt05 * fv;
+  O.u[n] = u1;
+  O.v[n] = v1;
+  const double xh = N.x[n] + h * u0, yh = N.y[n] + h * v0;
+  O.x[n] = xh + h * u1;
+  O.y[n] = yh + h * v1;
+}
+
+// ---------------------------------------------------------------- F4
+__global__ void __launch_bounds__(kBlock)
+k_f4_state(FusedArgs A, FusedNodes O) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || A.dtres->negvol) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.nel) return;
+  const double dt = ts->dt;
+  int nd[4];
+  load_nd(A.ielnd, e, nd);
+  double x[4], y[4], ub[4], vb[4];
+  load4(O.x, nd, x); load4(O.y, nd, y); load4(A.ubar, nd, ub); load4(A.vbar, nd, vb);
+  Iso g;
+  getiso(x, y, g);
+  if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
+  const double m = A.mass[e];
+  const double rho = m / g.vol;
+  double fx[4], fy[4];
+  ld4(A.cnfx, e, fx);
+  ld4(A.cnfy, e, fy);
+  const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
+  int mat = A.ielmat[e];
+  mat = mat > 0 ? mat : 0;
+  const int et = A.eos_itype[mat];
+  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
+  A.en[e] = en;
+  A.rho[e] = rho;
+  A.pre[e] = eos_pressure(et, ep, rho, en, A.pcut);
+  A.cs2[e] = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
+  if (A.write_geometry) {
+    st4(A.cnx, e, x);
+    st4(A.cny, e, y);
+    st4(A.cnwt, e, g.wt);
+    A.vol[e] = g.vol;
+    A.a1[e] = g.a1; A.a2[e] = g.a2; A.a3[e] = g.a3;
+    A.b1[e] = g.b1; A.b2[e] = g.b2; A.b3[e] = g.b3;
+  }
+}
+
+// negative volume found by F2 (F3/F4 did not run) or F4 (F3 did): stop the
+// batch at this step; the ALE kernels that follow are gated by skip
+__global__ void k_f5_end(DevTimestep* __restrict__ ts, const DtResult* __restrict__ res) {
+  if (threadIdx.x != 0 || ts->skip) return;
+  if (res->negvol) {
+    ts->err = 1; ts->err_idx = -res->negvol; ts->done = 1; ts->skip = 1;
+    ts->err_step = ts->nstep;
+    return;
+  }
+  ts->flips = ts->flips + 1;
+  if (res->negvol2) {
+    ts->err = 1; ts->err_idx = -res->negvol2; ts->done = 1; ts->skip = 1;
+    ts->err_step = ts->nstep;
+  }
+}
+
+// negative volume in the ALE hydro update (ale_dr_update.f90:44)
+__global__ void k_f7_aleend(DevTimestep* __restrict__ ts, const DtResult* __restrict__ res) {
+  if (threadIdx.x != 0 || ts->skip || !ts->ale_on) return;
+  if (res->negvol) {
+    ts->err = 1; ts->err_idx = -res->negvol; ts->done = 1; ts->skip = 1;
+    ts->err_step = ts->nstep;
+  }
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+namespace {
+
+FusedArgs make_args(bl_handle_t* h) {
+  FusedArgs A{};
+  const bl_config_t& c = h->cfg;
+  A.nel = h->sz.nel;
+  A.nnd = h->sz.nnd;
+  bool dtnot = false, mid = false;
+  for (int r = 0; r < h->nreg; ++r) {
+    dtnot = dtnot || h->zdtnotreg[r];
+    mid = mid || h->zmidlength[r];
+  }
+  A.reg_tables = (dtnot || mid) ? 1 : 0;
+  A.zsp = c.zsp;
+  A.zhg = c.zhg;
+  A.reg_tables_force = (c.zsp || c.zhg) ? 1 : 0;
+  A.zexist = c.zexist;
+  A.write_geometry = c.zexist ? 1 : 0;
+  A.zcut = c.zcut; A.zerocut = c.zerocut; A.dencut = c.dencut; A.accut2 = c.accut * c.accut;
+  A.cvisc1 = c.cvisc1; A.cvisc2 = c.cvisc2; A.cfl_sf = c.cfl_sf; A.div_sf = c.div_sf;
+  A.ccut = c.ccut; A.pcut = c.pcut;
+  A.dt_g = c.dt_g; A.dt_min = c.dt_min; A.dt_max = c.dt_max; A.dt_initial = c.dt_initial;
+  A.mintime = c.mintime; A.maxtime = c.maxtime;
+  A.ielnd = h->I(BL_IELND); A.ielel = h->I(BL_IELEL); A.ielfc = h->I(BL_IELFC);
+  A.ielreg = h->I(BL_IELREG); A.ielmat = h->I(BL_IELMAT); A.indtype = h->I(BL_INDTYPE);
+  A.hyd_ptr = h->d_hyd_ptr; A.hyd_idx = h->d_hyd_idx;
+  A.zdtnotreg = h->d_zdtnotreg; A.zmidlength = h->d_zmidlength;
+  A.kappareg = h->d_kappareg; A.pmeritreg = h->d_pmeritreg;
+  A.eos_itype = h->d_eos_itype; A.eos_param = h->d_eos_param;
+  A.rho = h->R(BL_ELDENSITY); A.en = h->R(BL_ELENERGY); A.pre = h->R(BL_ELPRESSURE);
+  A.cs2 = h->R(BL_ELCS2); A.vol = h->R(BL_ELVOLUME); A.mass = h->R(BL_ELMASS);
+  A.cnmass = h->R(BL_CNMASS); A.spmass = h->R(BL_SPMASS); A.cnwt = h->R(BL_CNWT);
+  A.cnx = h->R(BL_CNX); A.cny = h->R(BL_CNY);
+  A.a1 = h->R(BL_A1); A.a2 = h->R(BL_A2); A.a3 = h->R(BL_A3);
+  A.b1 = h->R(BL_B1); A.b2 = h->R(BL_B2); A.b3 = h->R(BL_B3);
+  A.cnviscx = h->R(BL_CNVISCX); A.cnviscy = h->R(BL_CNVISCY); A.elvisc = h->R(BL_ELVISC);
+  A.cnfx = h->R(BL_RSCRATCH23); A.cnfy = h->R(BL_RSCRATCH24);
+  A.ubar = h->R(BL_RSCRATCH14); A.vbar = h->R(BL_RSCRATCH15);
+  A.part = h->d_part; A.parti = h->d_parti; A.ticket = h->d_ticket; A.dtres = h->d_dtres;
+  A.ts = h->d_ts;
+  A.hist = h->d_hist;
+  A.hist_cap = (int)h->hist_cap;
+  return A;
+}
+
+// one step (parity p: node set p -> node set 1-p), launched on h->stream
+int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
+  FusedArgs A = make_args(h);
+  const int nel = h->sz.nel, nnd = h->sz.nnd;
+  const unsigned nbe = nblocks(nel), nbn = nblocks(nnd);
+  FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
+  FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
+                  h->nodeset[1 - p][3]};
+  FusedNodes Oc{O.x, O.y, O.u, O.v};
+  BL_LAUNCH(h, "f0_begin", k_f0_begin, 1, 64, h->d_ts, stop_at_time_end, h->cfg.time_end,
+            h->d_dtres);
+  BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
+  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kBlock, A, (int)nbe);
+  BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
+  BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
+  BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
+  BL_LAUNCH(h, "f5_end", k_f5_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  if (h->cfg.zexist) {
+    // the ALE kernels address the registry's node fields: point them at the
+    // set this step wrote
+    double* saved[4] = {h->R(BL_NDX), h->R(BL_NDY), h->R(BL_NDU), h->R(BL_NDV)};
+    h->f[BL_NDX] = O.x; h->f[BL_NDY] = O.y; h->f[BL_NDU] = O.u; h->f[BL_NDV] = O.v;
+    int rc = ale_step_device(h, h->d_ts);
+    h->f[BL_NDX] = saved[0]; h->f[BL_NDY] = saved[1];
+    h->f[BL_NDU] = saved[2]; h->f[BL_NDV] = saved[3];
+    if (rc) return BL_FAILURE;
+    BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  }
+  return BL_SUCCESS;
+}
+
+int ensure_nodesets(bl_handle_t* h) {
+  if (h->nodeset[1][0]) return BL_SUCCESS;
+  const int ids[4] = {BL_NDX, BL_NDY, BL_NDU, BL_NDV};
+  for (int k = 0; k < 4; ++k) {
+    h->nodeset[0][k] = h->R(ids[k]);
+    BL_HIP(hipMalloc(&h->nodeset[1][k], sizeof(double) * std::max(h->sz.nnd2, 1)));
+  }
+  h->cur_set = 0;
+  return BL_SUCCESS;
+}
+
+void point_fields_at(bl_handle_t* h, int s) {
+  h->f[BL_NDX] = h->nodeset[s][0];
+  h->f[BL_NDY] = h->nodeset[s][1];
+  h->f[BL_NDU] = h->nodeset[s][2];
+  h->f[BL_NDV] = h->nodeset[s][3];
+  h->cur_set = s;
+}
+
+int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
+  invalidate_graph(h);
+  if (h->cfg.zexist && !tmp_buffer(h, 0)) return BL_FAILURE;  // no allocation in a capture
+  bool timing = h->timing;
+  h->timing = false;  // no event records inside a capture
+  for (int p = 0; p < 2; ++p) {
+    hipGraph_t g = nullptr;
+    BL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(h, p, stop_at_time_end);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc != BL_SUCCESS || e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      h->timing = timing;
+      set_error("step graph capture failed");
+      return BL_FAILURE;
+    }
+    e = hipGraphInstantiate(&h->graph_exec[p], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      h->timing = timing;
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      return BL_FAILURE;
+    }
+  }
+  h->timing = timing;
+  h->graph_stop_at_end = stop_at_time_end;
+  return BL_SUCCESS;
+}
+
+// the registry's geometry fields after a Lagrangian batch: F4 keeps them in
+// registers, so they are rebuilt once from the final coordinates
+__global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
+                              const double* __restrict__ ndx, const double* __restrict__ ndy,
+                              double* __restrict__ cnx, double* __restrict__ cny,
+                              double* __restrict__ a1, double* __restrict__ a2,
+                              double* __restrict__ a3, double* __restrict__ b1,
+                              double* __restrict__ b2, double* __restrict__ b3,
+                              double* __restrict__ cnwt, double* __restrict__ vol) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  int nd[4];
+  load_nd(ielnd, e, nd);
+  double x[4], y[4];
+  load4(ndx, nd, x);
+  load4(ndy, nd, y);
+  Iso g;
+  getiso(x, y, g);
+  st4(cnx, e, x);
+  st4(cny, e, y);
+  a1[e] = g.a1; a2[e] = g.a2; a3[e] = g.a3;
+  b1[e] = g.b1; b2[e] = g.b2; b3[e] = g.b3;
+  st4(cnwt, e, g.wt);
+  vol[e] = g.vol;
+}
+
+void fill_err(bl_error_t* err, int ierr, int iout, const char* msg) {
+  if (!err) return;
+  err->ierr = ierr;
+  err->iout = iout;
+  std::memset(err->serr, ' ', BL_SLEN);
+  size_t n = std::strlen(msg);
+  std::memcpy(err->serr, msg, n < (size_t)BL_SLEN ? n : (size_t)BL_SLEN);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep_t* ts,
+                 double* dt_hist, int32_t* idt_hist, int32_t* ctl_hist, int32_t* steps_taken,
+                 bl_error_t* err) {
+  if (!h || !ts) return BL_FAILURE;
+  if (!h->maps_built || !h->d_eos_itype || !h->d_zdtnotreg) {
+    set_error("bl_run_graph: mesh maps, regions or EOS not set");
+    return BL_FAILURE;
+  }
+  if (h->cfg.zexist && h->cfg.adv_type != 1)  // split advection: host-driven
+    return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
+                  err);
+  fill_err(err, BL_SUCCESS, 0, "");
+  if (ensure_nodesets(h)) return BL_FAILURE;
+  if (h->hist_cap < nsteps) {
+    if (h->d_hist) hipFree(h->d_hist);
+    h->d_hist = nullptr;
+    BL_HIP(hipMalloc(&h->d_hist, sizeof(double) * 3 * std::max(nsteps, 1)));
+    h->hist_cap = nsteps;
+    invalidate_graph(h);
+  }
+  // the graphs bake in the node set pointers, which are fixed; capture once
+  if (!h->graph_exec[0] || h->graph_stop_at_end != stop_at_time_end)
+    if (capture_graphs(h, stop_at_time_end)) return BL_FAILURE;
+  DevTimestep* d = h->h_ts;
+  std::memset(d, 0, sizeof(DevTimestep));
+  d->time = ts->time; d->dt = ts->dt; d->dts = ts->dts;
+  d->nstep = ts->nstep; d->idtel = ts->idtel; d->sdt = ts->sdt;
+  BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
+  int p = h->cur_set;
+  for (int k = 0; k < nsteps; ++k) {
+    BL_HIP(hipGraphLaunch(h->graph_exec[p], h->stream));
+    p ^= 1;
+  }
+  BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
+  BL_HIP(hipStreamSynchronize(h->stream));
+  // steps that selected a dt (a failing step included)
+  const int taken = d->hist_n;
+  // the node set holding the latest state: every step whose F3 ran wrote the
+  // other set
+  point_fields_at(h, (h->cur_set + d->flips) & 1);
+  if (!h->cfg.zexist) {
+    int nel = h->sz.nel;
+    BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
+              h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
+              h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
+              h->R(BL_ELVOLUME));
+  }
+  if (taken > 0) {
+    std::vector<double> hist(3 * (size_t)taken);
+    BL_HIP(hipMemcpyAsync(hist.data(), h->d_hist, sizeof(double) * 3 * taken,
+                          hipMemcpyDeviceToHost, h->stream));
+    BL_HIP(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < taken; ++k) {
+      if (dt_hist) dt_hist[k] = hist[3 * k];
+      if (idt_hist) idt_hist[k] = (int32_t)hist[3 * k + 1];
+      if (ctl_hist) ctl_hist[k] = (int32_t)hist[3 * k + 2];
+    }
+  }
+  BL_HIP(hipStreamSynchronize(h->stream));
+  ts->time = d->time; ts->dt = d->dt; ts->dts = d->dts;
+  ts->nstep = d->nstep; ts->idtel = d->idtel; ts->sdt = d->sdt;
+  ts->zcorrector = 1;
+  if (steps_taken) *steps_taken = taken;
+  switch (d->err) {
+    case 1:
+      fill_err(err, d->err_idx, BL_HALT_SINGLE,
+               "ERROR: negative volume in geometry_dr_getgeometry");
+      set_error("negative volume in geometry_dr_getgeometry");
+      return BL_FAILURE;
+    case 2:
+      fill_err(err, BL_FAILURE, BL_HALT_SINGLE,
+               "ERROR: negative CFL condition in hydro_dr_getdt");
+      set_error("negative CFL condition in hydro_dr_getdt");
+      return BL_FAILURE;
+    case 3:
+      fill_err(err, BL_FAILURE, BL_HALT_ALL, "ERROR: dt < dt_min");
+      set_error("dt < dt_min");
+      return BL_FAILURE;
+    default:
+      return BL_SUCCESS;
+  }
+}
+
+}  // extern "C"

@@ -34,14 +34,22 @@ struct DtResult {
   int32_t cfl_idx;  // 1-based, first index attaining the min
   int32_t div_idx;  // 1-based, 0 if none exceeds TINY
   int32_t negvol;   // largest 1-based index with V < 0 (0 if none)
-  int32_t pad;
+  int32_t negvol2;  // the same for the corrector geometry of the fused step
 };
 
-// device-resident timestep state of the fused step (bl_run / bl_run_graph)
+// device-resident timestep state of the fused step (bl_run_graph): the
+// fields of timestep_t (dataAPI_types.f90:46-52) plus the batch control
 struct DevTimestep {
-  double time, dt, dts, dt_prev;
-  int32_t nstep, idtel, sdt, done;   // done: time_end reached or error
-  int32_t ierr, ierr_step, err_kind, pad;  // err_kind: 1 negvol, 2 cfl, 3 dtmin
+  double time, dt, dts, pad_d;
+  int32_t nstep, idtel, sdt, ale_on;
+  int32_t done;      // time_end reached or an error: later steps do nothing
+  int32_t skip;      // this step does nothing (set by the step prologue)
+  int32_t err;       // 0 none, 1 negative volume, 2 negative CFL, 3 dt < dt_min
+  int32_t err_step;  // nstep of the failing step
+  int32_t err_idx;   // -iel of the last negative-volume element
+  int32_t hist_n;    // steps recorded in the history buffer
+  int32_t flips;     // steps whose node update wrote the other node set
+  int32_t pad_i;
 };
 
 // per-launch kernel timing with HIP events on the handle's stream
@@ -93,8 +101,12 @@ struct Handle {
   double* d_hist = nullptr;     // dt history (3 values per step)
   int64_t hist_cap = 0;
   double* d_tmp[8] = {};        // fused-step temporaries (nel or nnd sized)
-  hipGraphExec_t graph_exec = nullptr;
-  bool graph_ale = false;
+  // node ping-pong of the fused step: set[0] is the registry's own ndx..ndv,
+  // set[1] the alternate; cur_set names the one h->f[ndx..ndv] points at
+  double* nodeset[2][4] = {};
+  int cur_set = 0;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
+  int graph_stop_at_end = -1;
   // timing
   bool timing = false;
   std::vector<TimingClass> tclass;
@@ -110,6 +122,13 @@ struct Handle {
 // timing helpers (bl_api.hip)
 int timing_begin(Handle* h, const char* cls);
 void timing_end(Handle* h, int slot);
+// lazily allocated nsz-sized temporaries (bl_ale.hip)
+double* tmp_buffer(Handle* h, int k);
+// ALE step launches; with d_ts the kernels read dt, the step gate and
+// ale_on from the device timestep (bl_ale.hip)
+int ale_step_device(Handle* h, const DevTimestep* d_ts);
+// drop the captured step graphs (config or maps changed)
+void invalidate_graph(Handle* h);
 
 }  // namespace bl
 

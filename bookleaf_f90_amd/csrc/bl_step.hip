@@ -160,10 +160,4 @@ int bl_run(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep
   return rc;
 }
 
-int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep_t* ts,
-                 double* dt_hist, int32_t* idt_hist, int32_t* ctl_hist, int32_t* steps_taken,
-                 bl_error_t* err) {
-  return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken, err);
-}
-
 }  // extern "C"
