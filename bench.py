@@ -1,51 +1,126 @@
 #!/usr/bin/env python3
 """Element-steps/s of BookLeaf's FP64 Lagrangian step on one MI355X.
 
-Workload: Sedov blast, 1024x1024 quads on [0,1.2]^2 with interior nodes
-jittered by U(-0.1,0.1) cell widths (seed 12345), gamma=1.4, rho=1,
-E=0.25 in the origin corner element, reflective walls (a 500-step run;
---steps K times K of them after --warmup W untimed ones).  Inputs are
-resident in HBM before the timed region; dt never leaves the device inside
-it (bl_run_graph: one captured hipGraph per step).
+Workload: Sedov blast, 1024x1024 quads on [0,1.2]^2, interior nodes jittered
+by U(-0.1,0.1) cell widths (seed 12345), gamma=1.4, rho=1, E=0.25 in the
+origin corner element, reflective walls, anti-hourglass coefficient 0.01
+(kappaall of the reference's own Sedov deck, input/sedov_lag_small; without
+it the deposit element is crushed by hourglass modes and dt falls below
+dt_min before step 500), a 500-step run (W untimed warm-up steps, then K
+timed ones).  Inputs are resident in HBM before the timed
+region and dt never leaves the device inside it (bl_run_graph: one captured
+hipGraph per step, dt selected on the device).
 
-Prints one JSON line (see the keys below).  One GPU only: --gpus N>1 exits.
+Reported next to the throughput:
+  roofline      the dominant kernel's algorithmic HBM bytes per launch (the
+                distinct dataAPI array elements it must read or write, see
+                KERNEL_BYTES) over its average duration, timed with HIP events
+                on the library's stream in a profile pass of the same run;
+                traffic is the rocprofv3 FETCH_SIZE+WRITE_SIZE of that kernel
+                from profiles/traffic_sedov1024.json when present
+  cpu_baseline  the reference's own Fortran (oracle/_ref, flang -O2, serial)
+                timed on this host, one core, on the first --cpu-steps steps of
+                the same workload ("port": the C oracle when _ref is absent)
+  driver_boundary_GBps  the throughput expressed in the per-driver
+                algorithmic bytes of the reference's call graph
+                (1664 B/element + 272 B/node per Lagrangian step, +84 B/element
+                with the anti-hourglass force)
+
+One GPU only: --gpus N with N > 1 exits.
 """
 import argparse
 import json
 import os
+import re
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+KAPPA = 0.01       # anti-hourglass coefficient (input/sedov_lag_small kappaall)
+# algorithmic bytes per launch of each fused kernel: (bytes/element, bytes/node)
+KERNEL_BYTES = {
+    # ielnd, ielel 32, limiter stencil 32; rho, cs2 16; writes cnviscx/y 64,
+    # elvisc 8 | node x,y,u,v 32, indtype 4
+    "f1_getdt": (152, 36),
+    # ielnd 16, ielmat 4, ielreg 4 (hourglass kappa), p, m, e 24, cnviscx/y
+    # 64, cnmass 32; writes corner force x/y, weight, mass term 128 |
+    # node x,y,u,v 32
+    "f2_force": (272, 32),
+    # CSR corner ids 16, corner force x/y, weight, mass term 128 | ptr 4,
+    # indtype 4, x,y,u,v 32, writes x,y,u,v 32, ubar,vbar 16
+    "f3_accel": (144, 88),
+    # ielnd 16, ielmat 4, m, e 16, corner force x/y 64; writes e, rho, p,
+    # cs2 32 | x,y 16, ubar,vbar 16
+    "f4_state": (132, 32),
+}
+# per Lagrangian step at the driver boundary: 1664 B/element + 272 B/node,
+# +84 B/element with the anti-hourglass force on
+DRIVER_BOUNDARY = (1664 + 84, 272)
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=490)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--runner", choices=["run", "run_graph"], default="run_graph")
     ap.add_argument("--n", type=int, default=1024, help="mesh is n x n")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="launch the plain nel-double copy kernel (hydro_setpredictor) this "
+                         "many times, to scale profiler byte counters")
     return ap.parse_args()
+
+
+def cpu_baseline(n, steps):
+    from bookleaf_f90_amd import problem as P
+    prob = P.sedov(n=n, nsteps=steps, kappa=KAPPA)
+    from oracle import refio
+    if refio.available():
+        with tempfile.TemporaryDirectory() as d:
+            fin, fout = os.path.join(d, "p.bin"), os.path.join(d, "o.bin")
+            refio.write_problem(prob, fin, nsteps=steps)
+            r = subprocess.run(["bash", "-c", 'ulimit -s unlimited; exec "$0" "$@"',
+                                refio.REF_EXE, fin, fout, "2"], capture_output=True,
+                               text=True, timeout=900)
+            m = re.search(r"steps=\s*(\d+) seconds=\s*(\S+) elsteps_per_s=\s*(\S+)", r.stdout)
+            if r.returncode == 0 and m:
+                return {"value": float(m.group(3)), "unit": "element-steps/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": "oracle/_ref (the reference's Fortran, flang -O2 "
+                                  "-ffp-contract=off, serial) on steps 1-%d of the same "
+                                  "%dx%d Sedov run, wall clock of the step loop "
+                                  "(%.2f s)" % (int(m.group(1)), n, n, float(m.group(2)))}
+    from oracle.pyoracle import Oracle
+    o = Oracle(prob)
+    t0 = time.perf_counter()
+    o.run(steps)
+    sec = time.perf_counter() - t0
+    return {"value": prob.nel * steps / sec, "unit": "element-steps/s", "cores": 1,
+            "kind": "port", "sample": "C oracle, steps 1-%d of the same run (%.2f s)"
+            % (steps, sec)}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the path is single-partition "
-              "(see DESIGN.md, 'How the path would shard'); --gpus %d refused" % args.gpus)
+        print("bench.py runs on one GPU only: the path is a single mesh partition per "
+              "GPU (see DESIGN.md); --gpus %d refused" % args.gpus)
         sys.exit(2)
     from bookleaf_f90_amd import problem as P
     from bookleaf_f90_amd.device import BookLeaf
 
-    prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps)
+    prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps, kappa=KAPPA)
+    nel, nnd = prob.nel, prob.nnd
     g = BookLeaf(prob)
-    nel = prob.nel
-    print("# workload sedov %dx%d nel=%d nnd=%d" % (args.n, args.n, nel, prob.nnd),
+    print("# workload sedov %dx%d nel=%d nnd=%d" % (args.n, args.n, nel, nnd),
           file=sys.stderr, flush=True)
     if args.warmup:
         g.run(args.warmup, runner=args.runner)
@@ -53,25 +128,48 @@ def main():
     t0 = time.perf_counter()
     g.run(args.steps, runner=args.runner)
     g.synchronize()
-    t1 = time.perf_counter()
-    sec = t1 - t0
+    sec = time.perf_counter() - t0
     value = nel * args.steps / sec
+    print("# timed %d steps in %.4f s" % (args.steps, sec), file=sys.stderr, flush=True)
 
-    # per-kernel durations: HIP events on the library's stream around every
-    # launch of a further profile region of the same run
+    # per-kernel durations: HIP events around every launch on the library's
+    # stream, over a further profile pass of the same run and kernels
     g.timing(True)
     g.timing_reset()
-    g.run(args.profile_steps, runner="run")
+    g.run(args.profile_steps, runner=args.runner)
     kern = {}
     for name in g.timing_names():
         ms, n = g.timing_query(name)
         if n:
-            kern[name] = dict(avg_ms=ms, launches=n, total_ms=ms * n)
+            kern[name] = {"avg_ms": ms, "launches": n, "total_ms": ms * n}
     g.timing(False)
-    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
-
+    dom = max((k for k in kern if k in KERNEL_BYTES), key=lambda k: kern[k]["total_ms"],
+              default=None)
+    roof = None
+    if dom:
+        be, bn = KERNEL_BYTES[dom]
+        alg = be * nel + bn * nnd
+        avg_s = kern[dom]["avg_ms"] * 1e-3
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic_sedov1024.json")
+        if os.path.exists(tf) and args.n == 1024:
+            traffic = json.load(open(tf)).get(dom)
+        roof = {"kernel": dom, "bound": "hbm", "achieved": alg / avg_s / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg / avg_s / HBM_PEAK,
+                "traffic": traffic, "algorithmic_bytes": alg, "avg_ms": kern[dom]["avg_ms"]}
+        for k in kern:
+            if k in KERNEL_BYTES:
+                be, bn = KERNEL_BYTES[k]
+                kern[k]["alg_GBps"] = (be * nel + bn * nnd) / (kern[k]["avg_ms"] * 1e-3) / 1e9
+    for _ in range(args.calibrate_copy):
+        g.hydro_setpredictor()  # k_copy_r: reads and writes 8*nel bytes
+    g.synchronize()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.n, args.cpu_steps)
+    step_bytes = DRIVER_BOUNDARY[0] * nel + DRIVER_BOUNDARY[1] * nnd
+    fused_bytes = sum(be * nel + bn * nnd for be, bn in KERNEL_BYTES.values())
     out = {
-        "metric": "element-steps/s FP64 at 1M cells, %HBM peak; learned vs plain ALE at equal L1(rho)",
+        "metric": "element-steps/s FP64 at 1M cells, %HBM peak; learned vs plain ALE at "
+                  "equal L1(rho)",
         "value": value,
         "unit": "element-steps/s",
         "n_gpus": 1,
@@ -83,10 +181,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "sedov_lagrangian_1024x1024_jitter0.1_seed12345_500steps",
-                   "nel": nel, "nnd": prob.nnd, "runner": args.runner},
+        "config": {"workload": "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_500steps"
+                               % (args.n, args.n),
+                   "nel": nel, "nnd": nnd, "runner": args.runner},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "step_hbm_frac": fused_bytes / (sec / args.steps) / HBM_PEAK,
+        "driver_boundary_GBps": step_bytes / (sec / args.steps) / 1e9,
+        "driver_boundary_hbm_frac": step_bytes / (sec / args.steps) / HBM_PEAK,
         "kernels": kern,
-        "dominant_kernel": dom,
     }
     print(json.dumps(out))
 

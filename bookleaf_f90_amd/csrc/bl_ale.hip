@@ -27,7 +27,7 @@ namespace bl {
 // ale_kn_getmeshstatus (Eulerian: indstatus = 2)
 __global__ void k_meshstatus(int nnd, int32_t* __restrict__ st, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i < nnd) st[i] = 2;
 }
 
@@ -39,7 +39,7 @@ __global__ void k_fluxvol_nodes(int nnd, int zeul, double dt, double* __restrict
                                 double* __restrict__ ndy, const DevTimestep* ts) {
   ALE_GATE(ts);
   if (ts) dt = ts->dt;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= nnd) return;
   double u = st4[i], v = st5[i];
   if (zeul) { u = -u; v = -v; st4[i] = u; st5[i] = v; }
@@ -56,7 +56,7 @@ __global__ void k_fluxvol_faces(int nel, double cut, const int32_t* __restrict__
                                 const double* __restrict__ x1, const double* __restrict__ y1,
                                 double* __restrict__ fcdv, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   int4 nd4 = reinterpret_cast<const int4*>(ielnd)[e];
   int nd[4] = {nd4.x - 1, nd4.y - 1, nd4.z - 1, nd4.w - 1};
@@ -131,7 +131,7 @@ __global__ void k_el1(int nel, int id1, int id2, const int32_t* __restrict__ iel
                       double* __restrict__ rwork1, double* __restrict__ fcdm,
                       const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
   rwork1[e] = sumflux_el(e, id1, id2, c.nb, c.fc, fcdv);
@@ -153,7 +153,7 @@ __global__ void k_el2(int nel, int id1, int id2, double zerocut, double dencut,
                       double* __restrict__ rho, double* __restrict__ flux,
                       const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
   double totm = sumflux_el(e, id1, id2, c.nb, c.fc, fcdm);
@@ -179,7 +179,7 @@ __global__ void k_el3(int nel, int id1, int id2, const int32_t* __restrict__ iel
                       const double* __restrict__ cutm, double* __restrict__ rwork1,
                       double* __restrict__ en, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
   double s = sumflux_el(e, id1, id2, c.nb, c.fc, flux);
@@ -199,7 +199,7 @@ __global__ void k_nd1(int nel, int id1, int id2, const int32_t* __restrict__ iel
                       double* __restrict__ cndm, double* __restrict__ cnflux,
                       double* __restrict__ cnmass, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
   reinterpret_cast<double4*>(cnu)[e] =
@@ -247,7 +247,7 @@ __global__ void k_nd2(int nnd, const int32_t* __restrict__ ptr, const int32_t* _
                       double* __restrict__ ndm1, double* __restrict__ cutv,
                       double* __restrict__ cutm, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = xcd_index();
   if (n >= nnd) return;
   double v0 = 0.0, v1 = 0.0, m0 = 0.0;
   int b = ptr[n], end = ptr[n + 1];
@@ -270,7 +270,7 @@ __global__ void k_fluxndvl(int nel, const int32_t* __restrict__ ielel,
                            const double* __restrict__ cdb, const double* __restrict__ var,
                            double* __restrict__ cnflux, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   NbFc c = load_nbfc(e, ielel, ielfc);
   double out[4] = {0.0, 0.0, 0.0, 0.0};
@@ -341,7 +341,7 @@ __global__ void k_updatend(int nnd, const int32_t* __restrict__ ptr,
                            const double* __restrict__ cnflux, double* __restrict__ ndflux,
                            double* __restrict__ var, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = xcd_index();
   if (n >= nnd) return;
   double s = 0.0;
   for (int k = ptr[n]; k < ptr[n + 1]; ++k) s = s + cnflux[idx[k]];
@@ -370,7 +370,7 @@ __global__ void k_ale_update(int nel, const int32_t* __restrict__ ielnd,
                              const double* __restrict__ param, double pcut, double ccut,
                              DtResult* __restrict__ res, const DevTimestep* ts) {
   ALE_GATE(ts);
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
   double x[4] = {ndx[nd.x - 1], ndx[nd.y - 1], ndx[nd.z - 1], ndx[nd.w - 1]};

@@ -138,7 +138,7 @@ __global__ void k_fill_z(uint8_t* __restrict__ p, int64_t n, uint8_t v) {
   if (i < n) p[i] = v;
 }
 __global__ void k_copy_r(const double* __restrict__ a, double* __restrict__ b, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i < n) b[i] = a[i];
 }
 
@@ -152,7 +152,7 @@ __global__ void k_getgeometry(int nel, const int32_t* __restrict__ ielnd,
                               double* __restrict__ b2, double* __restrict__ b3,
                               double* __restrict__ cnwt, double* __restrict__ vol,
                               DtResult* __restrict__ res) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
   double x[4] = {ndx[nd.x - 1], ndx[nd.y - 1], ndx[nd.z - 1], ndx[nd.w - 1]};
@@ -171,7 +171,7 @@ __global__ void k_getgeometry(int nel, const int32_t* __restrict__ ielnd,
 // utils_kn_divide (utils_kn_math.f90:28-41)
 __global__ void k_divide(int n, const double* __restrict__ num, const double* __restrict__ den,
                          double* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i < n) out[i] = num[i] / den[i];
 }
 
@@ -180,7 +180,7 @@ __global__ void k_geteos(int n, const int32_t* __restrict__ imat, const double* 
                          const double* __restrict__ en, double* __restrict__ pre,
                          double* __restrict__ cs2, const int32_t* __restrict__ itype,
                          const double* __restrict__ param, double pcut, double ccut) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= n) return;
   int m = imat[i];
   m = m > 0 ? m : 0;
@@ -195,7 +195,7 @@ __global__ void k_geteos(int n, const int32_t* __restrict__ imat, const double* 
 __global__ void k_elmass(int nel, const double* __restrict__ rho, const double* __restrict__ vol,
                          const double* __restrict__ cnwt, double* __restrict__ m,
                          double* __restrict__ cm) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   double r = rho[e];
   m[e] = r * vol[e];
@@ -207,7 +207,7 @@ __global__ void k_elmass(int nel, const double* __restrict__ rho, const double* 
 __global__ void k_spmassinit(int nel, const double* __restrict__ rho,
                              const double* __restrict__ cnx, const double* __restrict__ cny,
                              double* __restrict__ spm) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   double4 X = reinterpret_cast<const double4*>(cnx)[e];
   double4 Y = reinterpret_cast<const double4*>(cny)[e];
@@ -220,7 +220,7 @@ __global__ void k_spmassinit(int nel, const double* __restrict__ rho,
 __global__ void k_getvertex(int nnd, double dt, double* __restrict__ ndx,
                             double* __restrict__ ndy, const double* __restrict__ ndu,
                             const double* __restrict__ ndv) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= nnd) return;
   ndx[i] = ndx[i] + dt * ndu[i];
   ndy[i] = ndy[i] + dt * ndv[i];
@@ -231,7 +231,7 @@ __global__ void k_getenergy(int nel, double dt, double zerocut, double* __restri
                             const double* __restrict__ m, const double* __restrict__ cnfx,
                             const double* __restrict__ cnfy, const double* __restrict__ cnu,
                             const double* __restrict__ cnv) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   double4 FX = reinterpret_cast<const double4*>(cnfx)[e];
   double4 FY = reinterpret_cast<const double4*>(cnfy)[e];
@@ -254,7 +254,7 @@ __global__ void k_getforce(int nel, const double* __restrict__ p, const double* 
                            const double* __restrict__ vol, const double* __restrict__ cnu,
                            const double* __restrict__ cnv, const double* __restrict__ kappareg,
                            const double* __restrict__ pmeritreg) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   double fx[4], fy[4];
   forceP(p[e], a1[e], a3[e], b1[e], b3[e], fx, fy);
@@ -292,7 +292,7 @@ __global__ void k_scatteracc(int nnd, const int32_t* __restrict__ ptr,
                              const double* __restrict__ cnfy, double* __restrict__ ndarea,
                              double* __restrict__ ndmass, double* __restrict__ ndudot,
                              double* __restrict__ ndvdot) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = xcd_index();
   if (n >= nnd) return;
   double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
   int b = ptr[n], end = ptr[n + 1];
@@ -323,7 +323,7 @@ __global__ void k_getacc(int nnd, double dencut, double zerocut, double accut2,
                          const double* __restrict__ ndarea, double* __restrict__ ndmass,
                          double* __restrict__ ndu, double* __restrict__ ndv,
                          const int32_t* __restrict__ indtype) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = xcd_index();
   if (n >= nnd) return;
   double w1 = dencut * ndarea[n];
   double m = ndmass[n], u = ndu[n], v = ndv[n];
@@ -349,7 +349,7 @@ __global__ void k_getacc(int nnd, double dencut, double zerocut, double accut2,
 __global__ void k_applyacc(int nnd, double dt, double* __restrict__ ndubar,
                            double* __restrict__ ndvbar, double* __restrict__ ndu,
                            double* __restrict__ ndv) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = xcd_index();
   if (n >= nnd) return;
   double dt05 = 0.5 * dt;
   double w1 = ndu[n], w2 = ndv[n], au = ndubar[n], av = ndvbar[n];
@@ -363,7 +363,7 @@ __global__ void k_applyacc(int nnd, double dt, double* __restrict__ ndubar,
 __global__ void k_cngather2(int nel, const int32_t* __restrict__ ielnd,
                             const double* __restrict__ a, const double* __restrict__ b,
                             double* __restrict__ ca, double* __restrict__ cb) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
   reinterpret_cast<double4*>(ca)[e] =
@@ -385,7 +385,7 @@ __global__ void k_getdt_a(int nel, const int32_t* __restrict__ ielnd,
                           double* __restrict__ dv, double* __restrict__ dx,
                           double* __restrict__ dy, double* __restrict__ part,
                           int32_t* __restrict__ parti) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   double dval = -1.0;  // below TINY: never selected
   if (e < nel) {
     int4 nd = reinterpret_cast<const int4*>(ielnd)[e];
@@ -425,7 +425,7 @@ __global__ void k_getdt_b(int nel, double zerocut, double cvisc1, double cvisc2,
                           const double* __restrict__ cnv, double* __restrict__ scratch,
                           double* __restrict__ cnviscx, double* __restrict__ cnviscy,
                           double* __restrict__ elvisc, double* __restrict__ ellength) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   if (e >= nel) return;
   const int4 nb4 = reinterpret_cast<const int4*>(ielel)[e];
   const int4 fc4 = reinterpret_cast<const int4*>(ielfc)[e];
@@ -500,7 +500,7 @@ __global__ void k_getdt_c(int nel, const int32_t* __restrict__ ielreg,
                           const double* __restrict__ elvisc, double* __restrict__ cs2,
                           const double* __restrict__ ellength, double* __restrict__ rs11,
                           double* __restrict__ part, int32_t* __restrict__ parti, int nblk_a) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_index();
   double r = DBL_MAX;
   if (e < nel) {
     double c = cs2[e] + elvisc[e];
@@ -608,9 +608,10 @@ int bl_destroy(bl_handle_t* h) {
   void* bufs[] = {h->d_kappareg, h->d_pmeritreg, h->d_zdtnotreg, h->d_zmidlength,
                   h->d_eos_itype, h->d_eos_param, h->d_hyd_ptr, h->d_hyd_idx,
                   h->d_ale_ptr, h->d_ale_idx, h->d_el_ptr, h->d_el_idx, h->d_part,
-                  h->d_parti, h->d_ticket, h->d_dtres, h->d_ts, h->d_hist};
+                  h->d_parti, h->d_ticket, h->d_dtres, h->d_ts, h->d_hist, h->d_outer};
   for (void* p : bufs) if (p) hipFree(p);
   for (double* p : h->d_tmp) if (p) hipFree(p);
+  for (double* p : h->cterm) if (p) hipFree(p);
   if (h->h_dtres) hipHostFree(h->h_dtres);
   if (h->h_ts) hipHostFree(h->h_ts);
   for (hipEvent_t ev : h->ev_pool) hipEventDestroy(ev);
@@ -727,6 +728,57 @@ static int build_csr(bl_handle_t* h, const std::vector<int32_t>& nodes,
   return BL_SUCCESS;
 }
 
+// The limiter of hydro_kn_getviscosity (hydro_kn_edgeviscosity.f90:120-186)
+// reads, for direction iside (0,1) of element e with neighbours in1 =
+// ielel(iside) and in2 = ielel(iside+2) and faces f1 = ielfc(iside),
+// f2 = ielfc(iside+2), four edges of the neighbours:
+//   in1 edge f1+1 = (in1 node f1+1, in1 node f1+2)
+//   in1 edge f1+3 = (in1 node f1+3, in1 node f1)
+//   in2 edge f2+3 = (in2 node f2+3, in2 node f2)
+//   in2 edge f2+1 = (in2 node f2+1, in2 node f2+2)
+// When the neighbour traverses the shared face in the opposite sense, one end
+// of each is a node of e (nd(iside), nd(iside+1), nd(iside+3), nd(iside+2)
+// in that order), and the other, stored here as outer(e, 4*iside + k), is a
+// node of the neighbour only.  A boundary side (in == e) has its ratios
+// replaced, so any valid node will do there.  Elements that break the
+// pattern leave d_outer unset and the fused step uses the general gather.
+static int build_outer(bl_handle_t* h, const std::vector<int32_t>& ielnd) {
+  if (h->d_outer) hipFree(h->d_outer);
+  h->d_outer = nullptr;
+  const int nel = h->sz.nel;
+  std::vector<int32_t> ielel(4 * (size_t)nel), ielfc(4 * (size_t)nel);
+  if (bl_download(h, BL_IELEL, ielel.data(), 4 * (int64_t)nel) ||
+      bl_download(h, BL_IELFC, ielfc.data(), 4 * (int64_t)nel))
+    return BL_FAILURE;
+  std::vector<int32_t> outer(8 * (size_t)nel);
+  auto node = [&](int el, int c) { return ielnd[4 * (size_t)el + (c & 3)]; };
+  for (int e = 0; e < nel; ++e) {
+    for (int is = 0; is < 2; ++is) {
+      int32_t* o = &outer[8 * (size_t)e + 4 * is];
+      const int in1 = ielel[4 * (size_t)e + is] - 1, in2 = ielel[4 * (size_t)e + is + 2] - 1;
+      const int f1 = ielfc[4 * (size_t)e + is] - 1, f2 = ielfc[4 * (size_t)e + is + 2] - 1;
+      if (in1 < 0 || in1 >= nel || in2 < 0 || in2 >= nel) return BL_SUCCESS;
+      if (in1 == e) {
+        o[0] = node(e, is); o[1] = node(e, is + 1);
+      } else {
+        if (f1 < 0 || f1 > 3 || node(in1, f1 + 1) != node(e, is) ||
+            node(in1, f1) != node(e, is + 1))
+          return BL_SUCCESS;
+        o[0] = node(in1, f1 + 2); o[1] = node(in1, f1 + 3);
+      }
+      if (in2 == e) {
+        o[2] = node(e, is + 3); o[3] = node(e, is + 2);
+      } else {
+        if (f2 < 0 || f2 > 3 || node(in2, f2) != node(e, is + 3) ||
+            node(in2, f2 + 1) != node(e, is + 2))
+          return BL_SUCCESS;
+        o[2] = node(in2, f2 + 3); o[3] = node(in2, f2 + 2);
+      }
+    }
+  }
+  return upload_table(h, &h->d_outer, outer);
+}
+
 int bl_build_maps(bl_handle_t* h) {
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
@@ -761,6 +813,7 @@ int bl_build_maps(bl_handle_t* h) {
   for (int iel = 0; iel < nel; ++iel)
     for (int j = 0; j < 4; ++j) { nodes.push_back(ielnd[4 * (size_t)iel + j]); corners.push_back(4 * iel + j); }
   if (build_csr(h, nodes, corners, &h->d_el_ptr, &h->d_el_idx)) return BL_FAILURE;
+  if (build_outer(h, ielnd)) return BL_FAILURE;
   h->maps_built = true;
   invalidate_graph(h);
   return BL_SUCCESS;
@@ -924,7 +977,7 @@ int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) 
   BL_LAUNCH(h, "getdt_c", k_getdt_c, nb, kBlock, nel, h->I(BL_IELREG), h->d_zdtnotreg,
             h->R(BL_ELVISC), h->R(BL_ELCS2), h->R(BL_RSCRATCH12), h->R(BL_RSCRATCH11),
             h->d_part, h->d_parti, (int)nb);
-  BL_LAUNCH(h, "dt_final", k_dt_final, 1, kBlock, (int)nb, (int)nb, h->d_part, h->d_parti,
+  BL_LAUNCH(h, "dt_final", k_dt_final, 1, kFinalBlock, (int)nb, (int)nb, h->d_part, h->d_parti,
             h->d_dtres);
   BL_HIP(hipMemcpyAsync(h->h_dtres, h->d_dtres, sizeof(DtResult), hipMemcpyDeviceToHost, h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));

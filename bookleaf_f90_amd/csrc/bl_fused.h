@@ -29,12 +29,12 @@ struct FusedArgs {
   // connectivity
   const int32_t* ielnd;
   const int32_t* ielel;
-  const int32_t* ielfc;
   const int32_t* ielreg;
   const int32_t* ielmat;
   const int32_t* indtype;
   const int32_t* hyd_ptr;
   const int32_t* hyd_idx;
+  const int32_t* outer;  // limiter stencil (8 far nodes per element) or null
   // tables
   const int32_t* zdtnotreg;
   const int32_t* zmidlength;
@@ -65,6 +65,13 @@ struct FusedArgs {
   double* elvisc;
   double* cnfx;  // rscratch23 (lagstep cnfx)
   double* cnfy;  // rscratch24
+  // corrector corner terms in corner-major layout X[j*nel + e] (j = corner):
+  // force x/y, half-step corner weight and the nodal-mass term with the
+  // fallbacks of hydro_kn_scatteracceleration already resolved
+  double* fxT;
+  double* fyT;
+  double* wtT;
+  double* mT;
   double* ubar;  // rscratch14 (lagstep ndubar)
   double* vbar;  // rscratch15
   // reductions and device timestep

@@ -1,19 +1,22 @@
 // bl_fused.hip -- the device-resident whole step (bl_run_graph).
 //
 // One Lagrangian step (inf_solver_getdt + inf_solver_lagstep) in four
-// full-grid kernels plus a one-workgroup step prologue, with dt, time and the
-// error state kept on the device (time_dr_calc/time_dr_end run in the last
-// workgroup of F1), so a step can be captured as a hipGraph and replayed
-// with no host synchronisation:
+// full-grid kernels plus one-workgroup kernels for the step gate and the dt
+// selection, with dt, time and the error state kept on the device, so a step
+// can be captured as a hipGraph and replayed with no host synchronisation:
 //
 //   F0 begin   : time_end / error gate for the step, per-step counters
 //   F1 getdt   : corner gather, edge viscosity with the Christensen limiter
 //                (neighbour edges recomputed from node data), cnviscx/y,
-//                elvisc, CFL and divergence candidates -> dt on the device
+//                elvisc, per-workgroup CFL and divergence candidates
+//   F1b select : one workgroup: first-index min/max over the candidates and
+//                time_dr_calc / time_dr_end -> dt, time, nstep on the device
 //   F2 force   : predictor force, half-step geometry, energy and EOS in
-//                registers, then the corrector's corner forces P+Q(+SP)(+HG)
-//                and half-step cnwt, rho (the corrector force of an element
-//                only needs that element's predictor results)
+//                registers, then the corrector's corner forces P+Q(+SP)(+HG),
+//                half-step corner weights and nodal-mass terms, stored
+//                corner-major (X[j*nel+e]) so the node gather of F3 reads
+//                each of them once, coalesced (the corrector force of an
+//                element only needs that element's predictor results)
 //   F3 accel   : node gather through the (corner, element) CSR map in the
 //                reference's summation order: mass, area, force ->
 //                acceleration, BCs, u^{n+1}, ubar, x^{n+1} into the other
@@ -25,7 +28,8 @@
 // reference drivers; the intermediate half-step pressure/energy never reach
 // HBM.  Fields that hold only phase scratch in the reference (rscratch*) are
 // not written except ubar (rscratch14/15, which the ALE step reads) and the
-// corrector corner forces (rscratch23/24).
+// corrector corner forces (rscratch23/24: by F4 when the remap follows, else
+// once at the end of a batch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,12 +73,9 @@ __device__ __forceinline__ void ld4(const double* __restrict__ a, int e, double 
   v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
 }
 
-// edge f (corner f -> f+1) differences of element `el` from node data
-__device__ __forceinline__ void edge_of(const int32_t* __restrict__ ielnd,
-                                        const FusedNodes& N, int el, int f, double& du,
-                                        double& dv, double& dx, double& dy) {
-  const int32_t* q = ielnd + 4 * el;
-  int a = q[f] - 1, b = q[(f + 1) & 3] - 1;
+// differences (b - a) of node data along the edge a -> b
+__device__ __forceinline__ void edge_nodes(const FusedNodes& N, int a, int b, double& du,
+                                           double& dv, double& dx, double& dy) {
   du = N.u[b] - N.u[a];
   dv = N.v[b] - N.v[a];
   dx = N.x[b] - N.x[a];
@@ -82,11 +83,14 @@ __device__ __forceinline__ void edge_of(const int32_t* __restrict__ ielnd,
 }
 
 // ---------------------------------------------------------------- F1
+// The four neighbour edges of each limiter direction come from the outer-node
+// table: one end is a node of e, already in registers, the other is the
+// neighbour's far node (a mesh without that structure runs host-driven)
 __global__ void __launch_bounds__(kBlock)
 k_f1_getdt(FusedArgs A, FusedNodes N) {
   DevTimestep* ts = A.ts;
   if (ts->skip) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index();
   double cflv = DBL_MAX, divv = -1.0;
   if (e < A.nel) {
     int nd[4];
@@ -94,9 +98,11 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
     double x[4], y[4], u[4], v[4];
     load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
     const int4 nb4 = reinterpret_cast<const int4*>(A.ielel)[e];
-    const int4 fc4 = reinterpret_cast<const int4*>(A.ielfc)[e];
     const int nb[4] = {nb4.x - 1, nb4.y - 1, nb4.z - 1, nb4.w - 1};
-    const int fc[4] = {fc4.x - 1, fc4.y - 1, fc4.z - 1, fc4.w - 1};
+    const int4 o0 = reinterpret_cast<const int4*>(A.outer)[2 * e];
+    const int4 o1 = reinterpret_cast<const int4*>(A.outer)[2 * e + 1];
+    const int on[8] = {o0.x - 1, o0.y - 1, o0.z - 1, o0.w - 1,
+                       o1.x - 1, o1.y - 1, o1.z - 1, o1.w - 1};
     double ou[4], ov[4], ox[4], oy[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -111,12 +117,15 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
       const int in1 = nb[iside], in2 = nb[iside + 2];
       double s1, s2, s3, s4;
       double au, av, ax, ay, bu, bv, bx, by;
-      edge_of(A.ielnd, N, in1, (fc[iside] + 1) & 3, au, av, ax, ay);
-      edge_of(A.ielnd, N, in2, (fc[iside + 2] + 3) & 3, bu, bv, bx, by);
+      const int* o = on + 4 * iside;
+      // in1 edge f1+1, in2 edge f2+3
+      edge_nodes(N, nd[iside], o[0], au, av, ax, ay);
+      edge_nodes(N, o[2], nd[(iside + 3) & 3], bu, bv, bx, by);
       limiter_ratios(ou[is1], ov[is1], ox[is1], oy[is1], au, av, ax, ay, bu, bv, bx, by,
                      A.zerocut, s1, s2);
-      edge_of(A.ielnd, N, in1, (fc[iside] + 3) & 3, au, av, ax, ay);
-      edge_of(A.ielnd, N, in2, (fc[iside + 2] + 1) & 3, bu, bv, bx, by);
+      // in1 edge f1+3, in2 edge f2+1
+      edge_nodes(N, o[1], nd[iside + 1], au, av, ax, ay);
+      edge_nodes(N, nd[iside + 2], o[3], bu, bv, bx, by);
       limiter_ratios(ou[is2], ov[is2], ox[is2], oy[is2], au, av, ax, ay, bu, bv, bx, by,
                      A.zerocut, s3, s4);
       if (in1 == e) {
@@ -163,7 +172,7 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
 
 // F1b: one workgroup -- combine the partials of F1, then the hydro_dr_getdt
 // candidates and time_dr_calc / time_dr_end (time_dr_advance.f90:38-132)
-__global__ void __launch_bounds__(kBlock) k_f1b_select(FusedArgs A, int nb) {
+__global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb) {
   DevTimestep* ts = A.ts;
   if (ts->skip) return;
   final_dt_reduce(nb, nb, A.part, A.parti, A.dtres);
@@ -205,7 +214,7 @@ __global__ void __launch_bounds__(kBlock)
 k_f2_force(FusedArgs A, FusedNodes N) {
   const DevTimestep* ts = A.ts;
   if (ts->skip) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index();
   if (e >= A.nel) return;
   const double dt = ts->dt;
   const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
@@ -257,10 +266,23 @@ k_f2_force(FusedArgs A, FusedNodes N) {
     forceSP(A.pmeritreg[ireg], rhoh, cs2h, xh, yh, spm, fx, fy);
   }
   if (A.zhg) forceHG(A.kappareg[ireg], rhoh, gh.vol, 1.0 / dt, u, v, fx, fy);
-  st4(A.cnfx, e, fx);
-  st4(A.cnfy, e, fy);
-  st4(A.cnwt, e, gh.wt);
-  A.rho[e] = rhoh;
+  // nodal-mass term of each corner as hydro_kn_scatteracceleration picks it
+  // (hydro_kn_getacceleration.f90:58-67): cnmass, else the previous corner's
+  // cnmass, else rho*cnwt at the half step
+  double cm[4];
+  ld4(A.cnmass, e, cm);
+  const int nel = A.nel;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double mt;
+    if (cm[j] > A.zerocut) mt = cm[j];
+    else if (cm[(j + 3) & 3] > A.zerocut) mt = cm[(j + 3) & 3];
+    else mt = rhoh * gh.wt[j];
+    A.fxT[j * nel + e] = fx[j];
+    A.fyT[j * nel + e] = fy[j];
+    A.wtT[j * nel + e] = gh.wt[j];
+    A.mT[j * nel + e] = mt;
+  }
 }
 
 // ---------------------------------------------------------------- F3
@@ -268,26 +290,22 @@ __global__ void __launch_bounds__(kBlock)
 k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   const DevTimestep* ts = A.ts;
   if (ts->skip || A.dtres->negvol) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = xcd_index();
   if (n >= A.nnd) return;
   const double dt = ts->dt;
   const double h = 0.5 * dt;
   double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
   const int b = A.hyd_ptr[n], end = A.hyd_ptr[n + 1];
+  const int nel = A.nel;
+  // incidences in (corner j, element) order; corner-major terms make the
+  // loads of a wave for one corner slot contiguous
   for (int k = b; k < end; ++k) {
     const int c = A.hyd_idx[k];
-    const int el = c >> 2, j = c & 3;
-    const double cm = A.cnmass[c];
-    if (cm > A.zerocut) {
-      m = m + cm;
-    } else {
-      const double cp = A.cnmass[4 * el + ((j + 3) & 3)];
-      if (cp > A.zerocut) m = m + cp;
-      else m = m + A.rho[el] * A.cnwt[c];
-    }
-    a = a + A.cnwt[c];
-    fu = fu + A.cnfx[c];
-    fv = fv + A.cnfy[c];
+    const int off = (c & 3) * nel + (c >> 2);
+    m = m + A.mT[off];
+    a = a + A.wtT[off];
+    fu = fu + A.fxT[off];
+    fv = fv + A.fyT[off];
   }
   // hydro_kn_getacceleration + utils_kn_bc
   const double w1 = A.dencut * a;
@@ -316,7 +334,7 @@ __global__ void __launch_bounds__(kBlock)
 k_f4_state(FusedArgs A, FusedNodes O) {
   const DevTimestep* ts = A.ts;
   if (ts->skip || A.dtres->negvol) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index();
   if (e >= A.nel) return;
   const double dt = ts->dt;
   int nd[4];
@@ -329,8 +347,12 @@ k_f4_state(FusedArgs A, FusedNodes O) {
   const double m = A.mass[e];
   const double rho = m / g.vol;
   double fx[4], fy[4];
-  ld4(A.cnfx, e, fx);
-  ld4(A.cnfy, e, fy);
+  const int nel = A.nel;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    fx[j] = A.fxT[j * nel + e];
+    fy[j] = A.fyT[j * nel + e];
+  }
   const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
   int mat = A.ielmat[e];
   mat = mat > 0 ? mat : 0;
@@ -341,6 +363,10 @@ k_f4_state(FusedArgs A, FusedNodes O) {
   A.pre[e] = eos_pressure(et, ep, rho, en, A.pcut);
   A.cs2[e] = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
   if (A.write_geometry) {
+    // the remap that follows reuses rscratch23/24 only in part: it must see
+    // the corrector forces there, as the reference leaves them
+    st4(A.cnfx, e, fx);
+    st4(A.cnfy, e, fy);
     st4(A.cnx, e, x);
     st4(A.cny, e, y);
     st4(A.cnwt, e, g.wt);
@@ -402,9 +428,9 @@ FusedArgs make_args(bl_handle_t* h) {
   A.ccut = c.ccut; A.pcut = c.pcut;
   A.dt_g = c.dt_g; A.dt_min = c.dt_min; A.dt_max = c.dt_max; A.dt_initial = c.dt_initial;
   A.mintime = c.mintime; A.maxtime = c.maxtime;
-  A.ielnd = h->I(BL_IELND); A.ielel = h->I(BL_IELEL); A.ielfc = h->I(BL_IELFC);
+  A.ielnd = h->I(BL_IELND); A.ielel = h->I(BL_IELEL);
   A.ielreg = h->I(BL_IELREG); A.ielmat = h->I(BL_IELMAT); A.indtype = h->I(BL_INDTYPE);
-  A.hyd_ptr = h->d_hyd_ptr; A.hyd_idx = h->d_hyd_idx;
+  A.hyd_ptr = h->d_hyd_ptr; A.hyd_idx = h->d_hyd_idx; A.outer = h->d_outer;
   A.zdtnotreg = h->d_zdtnotreg; A.zmidlength = h->d_zmidlength;
   A.kappareg = h->d_kappareg; A.pmeritreg = h->d_pmeritreg;
   A.eos_itype = h->d_eos_itype; A.eos_param = h->d_eos_param;
@@ -416,6 +442,7 @@ FusedArgs make_args(bl_handle_t* h) {
   A.b1 = h->R(BL_B1); A.b2 = h->R(BL_B2); A.b3 = h->R(BL_B3);
   A.cnviscx = h->R(BL_CNVISCX); A.cnviscy = h->R(BL_CNVISCY); A.elvisc = h->R(BL_ELVISC);
   A.cnfx = h->R(BL_RSCRATCH23); A.cnfy = h->R(BL_RSCRATCH24);
+  A.fxT = h->cterm[0]; A.fyT = h->cterm[1]; A.wtT = h->cterm[2]; A.mT = h->cterm[3];
   A.ubar = h->R(BL_RSCRATCH14); A.vbar = h->R(BL_RSCRATCH15);
   A.part = h->d_part; A.parti = h->d_parti; A.ticket = h->d_ticket; A.dtres = h->d_dtres;
   A.ts = h->d_ts;
@@ -436,7 +463,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   BL_LAUNCH(h, "f0_begin", k_f0_begin, 1, 64, h->d_ts, stop_at_time_end, h->cfg.time_end,
             h->d_dtres);
   BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
-  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kBlock, A, (int)nbe);
+  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe);
   BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
   BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
@@ -456,6 +483,9 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
 }
 
 int ensure_nodesets(bl_handle_t* h) {
+  if (!h->cterm[0])
+    for (int k = 0; k < 4; ++k)
+      BL_HIP(hipMalloc(&h->cterm[k], sizeof(double) * 4 * std::max(h->sz.nel, 1)));
   if (h->nodeset[1][0]) return BL_SUCCESS;
   const int ids[4] = {BL_NDX, BL_NDY, BL_NDU, BL_NDV};
   for (int k = 0; k < 4; ++k) {
@@ -504,16 +534,27 @@ int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
 }
 
 // the registry's geometry fields after a Lagrangian batch: F4 keeps them in
-// registers, so they are rebuilt once from the final coordinates
+// registers, so they are rebuilt once from the final coordinates; the last
+// step's corrector corner forces go back to the element-major rscratch23/24
 __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
                               const double* __restrict__ ndx, const double* __restrict__ ndy,
                               double* __restrict__ cnx, double* __restrict__ cny,
                               double* __restrict__ a1, double* __restrict__ a2,
                               double* __restrict__ a3, double* __restrict__ b1,
                               double* __restrict__ b2, double* __restrict__ b3,
-                              double* __restrict__ cnwt, double* __restrict__ vol) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+                              double* __restrict__ cnwt, double* __restrict__ vol,
+                              const double* __restrict__ fxT, const double* __restrict__ fyT,
+                              double* __restrict__ cnfx, double* __restrict__ cnfy) {
+  int e = xcd_index();
   if (e >= nel) return;
+  double fx[4], fy[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    fx[j] = fxT[j * nel + e];
+    fy[j] = fyT[j * nel + e];
+  }
+  st4(cnfx, e, fx);
+  st4(cnfy, e, fy);
   int nd[4];
   load_nd(ielnd, e, nd);
   double x[4], y[4];
@@ -550,7 +591,8 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     set_error("bl_run_graph: mesh maps, regions or EOS not set");
     return BL_FAILURE;
   }
-  if (h->cfg.zexist && h->cfg.adv_type != 1)  // split advection: host-driven
+  // split advection, or a mesh without the limiter stencil table: host-driven
+  if ((h->cfg.zexist && h->cfg.adv_type != 1) || !h->d_outer)
     return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
                   err);
   fill_err(err, BL_SUCCESS, 0, "");
@@ -572,7 +614,13 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
   int p = h->cur_set;
   for (int k = 0; k < nsteps; ++k) {
-    BL_HIP(hipGraphLaunch(h->graph_exec[p], h->stream));
+    // with kernel timing on, the same step kernels are launched directly so
+    // each launch can be bracketed by HIP events on this stream
+    if (h->timing) {
+      if (launch_step(h, p, stop_at_time_end)) return BL_FAILURE;
+    } else {
+      BL_HIP(hipGraphLaunch(h->graph_exec[p], h->stream));
+    }
     p ^= 1;
   }
   BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
@@ -587,7 +635,8 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
               h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
               h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
-              h->R(BL_ELVOLUME));
+              h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
+              h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
   }
   if (taken > 0) {
     std::vector<double> hist(3 * (size_t)taken);

@@ -87,6 +87,10 @@ struct Handle {
   int32_t* d_ale_idx = nullptr;
   int32_t* d_el_ptr = nullptr;   // order (element 1..nel, corner)
   int32_t* d_el_idx = nullptr;
+  // limiter stencil of a consistently oriented conforming quad mesh: per
+  // element, the far node of each of the 8 neighbour edges the Christensen
+  // limiter reads (null when some element breaks that pattern)
+  int32_t* d_outer = nullptr;
   bool maps_built = false;
   // reductions
   double* d_part = nullptr;  // 4 doubles per workgroup
@@ -105,6 +109,9 @@ struct Handle {
   // set[1] the alternate; cur_set names the one h->f[ndx..ndv] points at
   double* nodeset[2][4] = {};
   int cur_set = 0;
+  // corrector corner terms of the fused step, corner-major (4*nel each):
+  // force x, force y, half-step corner weight, nodal-mass term
+  double* cterm[4] = {};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
   int graph_stop_at_end = -1;
   // timing

@@ -63,38 +63,40 @@ __device__ __forceinline__ void block_argmax_first(double v, int i, double* part
   block_reduce_store<false>(v, i, part, parti);
 }
 
-// run by the whole last workgroup: DIV partials in part[0,na), CFL partials in
-// part[2na, 2na+nc)
+// run by one workgroup of up to kFinalBlock threads: DIV partials in
+// part[0,na), CFL partials in part[2na, 2na+nc); both in one pass so the
+// loads of a thread are independent
+constexpr int kFinalBlock = 1024;
 __device__ __forceinline__ void final_dt_reduce(int na, int nc, const double* part,
                                                 const int32_t* parti, DtResult* res) {
-  __shared__ double sv[kBlock / 64];
-  __shared__ int si[kBlock / 64];
-  // DIV (max)
-  double v = -1.0;
-  int i = 0x7fffffff;
-  for (int k = threadIdx.x; k < na; k += blockDim.x) better_max(v, i, part[k], parti[k]);
-  wave_reduce<false>(v, i);
+  __shared__ double sa[kFinalBlock / 64], sc[kFinalBlock / 64];
+  __shared__ int ia[kFinalBlock / 64], ic[kFinalBlock / 64];
+  double va = -1.0, vc = DBL_MAX;
+  int xa = 0x7fffffff, xc = 0x7fffffff;
+  const double* pc = part + 2 * na;
+  const int32_t* qc = parti + 2 * na;
+  const int nmax = na > nc ? na : nc;
+#pragma unroll 4
+  for (int k = threadIdx.x; k < nmax; k += blockDim.x) {
+    if (k < na) better_max(va, xa, part[k], parti[k]);
+    if (k < nc) better_min(vc, xc, pc[k], qc[k]);
+  }
+  wave_reduce<false>(va, xa);
+  wave_reduce<true>(vc, xc);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) { sv[wid] = v; si[wid] = i; }
+  if (lane == 0) { sa[wid] = va; ia[wid] = xa; sc[wid] = vc; ic[wid] = xc; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better_max(sv[0], si[0], sv[w], si[w]);
-    if (sv[0] > DBL_MIN) { res->div_max = sv[0]; res->div_idx = si[0]; }
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      better_max(sa[0], ia[0], sa[w], ia[w]);
+      better_min(sc[0], ic[0], sc[w], ic[w]);
+    }
+    if (sa[0] > DBL_MIN) { res->div_max = sa[0]; res->div_idx = ia[0]; }
     else { res->div_max = DBL_MIN; res->div_idx = 0; }
+    res->cfl_min = sc[0];
+    res->cfl_idx = ic[0];
   }
   __syncthreads();
-  // CFL (min)
-  v = DBL_MAX;
-  i = 0x7fffffff;
-  for (int k = threadIdx.x; k < nc; k += blockDim.x) better_min(v, i, part[2 * na + k], parti[2 * na + k]);
-  wave_reduce<true>(v, i);
-  if (lane == 0) { sv[wid] = v; si[wid] = i; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) better_min(sv[0], si[0], sv[w], si[w]);
-    res->cfl_min = sv[0];
-    res->cfl_idx = si[0];
-  }
 }
 
 }  // namespace bl

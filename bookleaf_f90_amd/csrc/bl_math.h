@@ -12,6 +12,21 @@
 
 namespace bl {
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
+// (workgroup b runs on XCD b mod 8) and each XCD has its own L2.  Remap so
+// XCD x processes one contiguous 1/8 of the index range: mesh neighbours
+// (the element rows shared by adjacent node rows, the neighbour elements of
+// the limiter and the remap) then hit the same L2 instead of being fetched
+// from HBM once per XCD.  A bijection on [0, gridDim.x).
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, n = gridDim.x;
+  const int x = b % kNumXcd, local = b / kNumXcd;
+  const int q = n / kNumXcd, r = n % kNumXcd;
+  return x * q + (x < r ? x : r) + local;
+}
+__device__ __forceinline__ int xcd_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
 // Fortran SIGN(a,b)

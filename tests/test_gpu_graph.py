@@ -104,3 +104,21 @@ def test_graph_sedov_1024(oracle_lib):
     ho = o.run(4)
     assert all(np.array_equal(a, b) for a, b in zip(hg, ho))
     assert diff_fields(g.state(), o.state(), compared(p)) == []
+
+
+def test_graph_without_stencil_runs_host_driven(oracle_lib):
+    """A side connectivity that breaks the conforming-quad pattern leaves no
+    limiter stencil table: bl_run_graph then runs the per-driver kernels and
+    still matches the oracle fed the same ielfc."""
+    p = P.sedov(n=24, nsteps=12, kappa=0.01)
+    g, o = _pair(p)
+    fc = g.get("ielfc")
+    e = 12 * 24 + 12  # interior element
+    fc[4 * e] = fc[4 * e] % 4 + 1
+    g.set("ielfc", fc)
+    o.set("ielfc", fc)
+    assert g.lib.bl_build_maps(g.h) == 0
+    hg = g.run(12, runner="run_graph")
+    ho = o.run(12)
+    assert all(np.array_equal(a, b) for a, b in zip(hg, ho))
+    assert diff_fields(g.state(), o.state()) == []
