@@ -61,6 +61,9 @@ KERNEL_BYTES = {
 # per Lagrangian step at the driver boundary: 1664 B/element + 272 B/node,
 # +84 B/element with the anti-hourglass force on
 DRIVER_BOUNDARY = (1664 + 84, 272)
+# Eulerian: the Lagrangian step without the hourglass force plus the remap's
+# 504 B/element + 108 B/node
+DRIVER_BOUNDARY_EUL = (1664 + 504, 272 + 108)
 
 
 def parse():
@@ -69,7 +72,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=490)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--runner", choices=["run", "run_graph"], default="run_graph")
-    ap.add_argument("--n", type=int, default=1024, help="mesh is n x n")
+    ap.add_argument("--n", type=int, default=1024, help="mesh is n x n (sedov)")
+    ap.add_argument("--workload", choices=["sedov", "sod2mat"], default="sedov",
+                    help="sedov: Lagrangian 1024^2 Sedov (the headline); sod2mat: Eulerian "
+                         "(ALE remap every step) two-material Sod on 2048x256, pure cells")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -79,9 +85,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(n, steps):
-    from bookleaf_f90_amd import problem as P
-    prob = P.sedov(n=n, nsteps=steps, kappa=KAPPA)
+def cpu_baseline(prob, steps):
     from oracle import refio
     if refio.available():
         with tempfile.TemporaryDirectory() as d:
@@ -96,8 +100,8 @@ def cpu_baseline(n, steps):
                         "kind": "reference",
                         "sample": "oracle/_ref (the reference's Fortran, flang -O2 "
                                   "-ffp-contract=off, serial) on steps 1-%d of the same "
-                                  "%dx%d Sedov run, wall clock of the step loop "
-                                  "(%.2f s)" % (int(m.group(1)), n, n, float(m.group(2)))}
+                                  "%s run, wall clock of the step loop "
+                                  "(%.2f s)" % (int(m.group(1)), prob.name, float(m.group(2)))}
     from oracle.pyoracle import Oracle
     o = Oracle(prob)
     t0 = time.perf_counter()
@@ -117,11 +121,17 @@ def main():
     from bookleaf_f90_amd import problem as P
     from bookleaf_f90_amd.device import BookLeaf
 
-    prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps, kappa=KAPPA)
+    if args.workload == "sedov":
+        prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps, kappa=KAPPA)
+        wname = "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_%dsteps" % (
+            args.n, args.n, args.warmup + args.steps)
+    else:
+        prob = P.sod2mat(nx=2048, ny=256, nsteps=args.warmup + args.steps)
+        wname = "sod2mat_eulerian_2048x256_pure_cells_seed2024_%dsteps" % (
+            args.warmup + args.steps)
     nel, nnd = prob.nel, prob.nnd
     g = BookLeaf(prob)
-    print("# workload sedov %dx%d nel=%d nnd=%d" % (args.n, args.n, nel, nnd),
-          file=sys.stderr, flush=True)
+    print("# workload %s nel=%d nnd=%d" % (wname, nel, nnd), file=sys.stderr, flush=True)
     if args.warmup:
         g.run(args.warmup, runner=args.runner)
     g.synchronize()
@@ -152,7 +162,7 @@ def main():
         avg_s = kern[dom]["avg_ms"] * 1e-3
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic_sedov1024.json")
-        if os.path.exists(tf) and args.n == 1024:
+        if os.path.exists(tf) and args.workload == "sedov" and args.n == 1024:
             traffic = json.load(open(tf)).get(dom)
         roof = {"kernel": dom, "bound": "hbm", "achieved": alg / avg_s / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg / avg_s / HBM_PEAK,
@@ -164,8 +174,9 @@ def main():
     for _ in range(args.calibrate_copy):
         g.hydro_setpredictor()  # k_copy_r: reads and writes 8*nel bytes
     g.synchronize()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.n, args.cpu_steps)
-    step_bytes = DRIVER_BOUNDARY[0] * nel + DRIVER_BOUNDARY[1] * nnd
+    cpu = None if args.no_cpu_baseline else cpu_baseline(prob, args.cpu_steps)
+    dbe, dbn = DRIVER_BOUNDARY if args.workload == "sedov" else DRIVER_BOUNDARY_EUL
+    step_bytes = dbe * nel + dbn * nnd
     fused_bytes = sum(be * nel + bn * nnd for be, bn in KERNEL_BYTES.values())
     out = {
         "metric": "element-steps/s FP64 at 1M cells, %HBM peak; learned vs plain ALE at "
@@ -181,9 +192,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_500steps"
-                               % (args.n, args.n),
-                   "nel": nel, "nnd": nnd, "runner": args.runner},
+        "config": {"workload": wname, "nel": nel, "nnd": nnd, "runner": args.runner},
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,

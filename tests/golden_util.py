@@ -1,0 +1,51 @@
+"""Load the golden fixtures of tests/golden/ (written from the reference's own
+build by tests/golden/make_golden.py)."""
+import glob
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from bookleaf_f90_amd import abi
+from bookleaf_f90_amd.problem import Problem
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INPUTS = ("ielnd", "ndx", "ndy", "indtype", "ielreg", "ielmat", "density", "energy", "ndu",
+          "ndv", "eos_itype", "eos_param", "kappareg", "pmeritreg", "zdtnotreg", "zmidlength")
+
+
+def names():
+    return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))  # plain arrays, no pickle
+    meta = json.loads(bytes(z["meta"]).decode())
+    kw = {k: z["in_" + k].copy() for k in INPUTS}
+    p = Problem(name=name, nmat=meta["nmat"], nreg=meta["nreg"], config=dict(meta["config"]),
+                nsteps=meta["nsteps"], stop_at_time_end=meta["stop_at_time_end"], **kw)
+    return p, meta, z
+
+
+def canonical(a):
+    a = np.asarray(a)
+    if a.dtype.kind == "f":
+        a = np.where(a == 0.0, 0.0, a)
+        a = np.where(np.isnan(a), np.nan, a)
+    return np.ascontiguousarray(a)
+
+
+def digest(a):
+    return hashlib.sha256(canonical(a).tobytes()).hexdigest()[:32]
+
+
+def digest_mismatches(state, record, skip=()):
+    """Fields of state whose first len entries do not match the recorded digest."""
+    bad = []
+    for n, (count, d) in record.items():
+        if n in skip or n not in state:
+            continue
+        if digest(np.asarray(state[n])[:count]) != d:
+            bad.append(n)
+    return bad

@@ -1,0 +1,29 @@
+"""The device-resident runner (bl_run_graph) against the reference's own
+recorded results (tests/golden/, written from oracle/_ref): dt sequence and
+fields after many steps, bit for bit, with no oracle in between."""
+import numpy as np
+import pytest
+
+from bookleaf_f90_amd import abi
+from tests import golden_util as G
+from tests.test_gpu_graph import LAG_SCRATCH
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", G.names())
+def test_graph_runner_matches_reference_record(name):
+    from bookleaf_f90_amd.device import BookLeaf
+    p, meta, z = G.load(name)
+    print("golden case", name, "nel", p.nel, flush=True)
+    g = BookLeaf(p)
+    dt, idt, ctl = g.run(meta["nsteps"], stop_at_time_end=bool(p.stop_at_time_end),
+                         runner="run_graph")
+    assert np.array_equal(dt, z["ref_dt"])
+    assert np.array_equal(idt, z["ref_idt"])
+    assert [abi.DT_NAMES[c].strip() for c in ctl] == meta["dt_names"]
+    s = g.state()
+    for k in ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv"):
+        assert np.array_equal(s[k][:len(z["final_" + k])], z["final_" + k]), k
+    skip = () if p.config.get("zexist") else LAG_SCRATCH
+    assert G.digest_mismatches(s, meta["final"], skip=skip) == []

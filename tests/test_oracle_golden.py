@@ -1,0 +1,44 @@
+"""The CPU oracle against the committed golden fixtures (tests/golden/), which
+hold the reference's own results (oracle/_ref, flang build of the reference
+sources) for small problems: every field after every driver call of step 1,
+the dt sequence of many steps and every field after the last step.  These pin
+oracle/bl_oracle.c where the reference sources are not present."""
+import numpy as np
+import pytest
+
+from bookleaf_f90_amd import abi
+from tests import golden_util as G
+from tests.helpers import granular_sequence
+
+NAMES = G.names()
+
+
+def test_fixtures_present():
+    assert len(NAMES) >= 9
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_every_driver_matches_reference_record(name, oracle_lib):
+    from oracle.pyoracle import Oracle
+    p, meta, _ = G.load(name)
+    o = Oracle(p)
+    gran = meta["granular"]
+    assert G.digest_mismatches(o.state(), gran["0"]) == []
+    for lab, fn in granular_sequence(o):
+        fn()
+        assert G.digest_mismatches(o.state(), gran[str(lab)]) == [], "after driver %d" % lab
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_steps_match_reference_record(name, oracle_lib):
+    from oracle.pyoracle import Oracle
+    p, meta, z = G.load(name)
+    o = Oracle(p)
+    dt, idt, ctl = o.run(meta["nsteps"], stop_at_time_end=bool(p.stop_at_time_end))
+    assert np.array_equal(dt, z["ref_dt"])
+    assert np.array_equal(idt, z["ref_idt"])
+    assert [abi.DT_NAMES[c].strip() for c in ctl] == meta["dt_names"]
+    s = o.state()
+    for k in ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv"):
+        assert np.array_equal(s[k][:len(z["final_" + k])], z["final_" + k]), k
+    assert G.digest_mismatches(s, meta["final"]) == []

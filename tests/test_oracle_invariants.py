@@ -1,0 +1,254 @@
+"""Closed forms, invariants and known answers that pin the CPU oracle
+independently of the reference build: each would fail under a plausible
+restatement mistake (a dropped term, a sign, a wrong index or order).
+
+  - corner forces of an element sum to zero (pressure, viscosity, hourglass
+    and sub-zonal parts; hydro_kn_getforce.f90)
+  - corner weights sum to the volume; rectangles are exact
+    (geometry_kn_volume.f90:30-99)
+  - ideal-gas EOS p=(g-1) rho e, cs2=g(g-1)e (eos_kn_forms.f90)
+  - uniform mesh and state: CFL dt = cfl_sf*h/c (hydro_kn_getdt.f90:30-85)
+  - a uniform state at rest stays at rest (Lagrangian and Eulerian)
+  - Lagrangian mass is never written
+  - rigid translation leaves rho, e, p unchanged and moves nodes by t*u
+  - the node scatter equals a brute-force sum in the reference's (corner,
+    element) order, bit for bit (hydro_kn_getacceleration.f90:30-74)
+  - the Sod L1(rho) of the reference's sod_sm_{lag,eul}_small decks at 384
+    steps against the reference's analytic solution
+    (check_kn_tests.f90:33-111, check_dr_tests.f90:56-69), to the printed
+    digits of the reference's own run
+"""
+import numpy as np
+import pytest
+
+from bookleaf_f90_amd import abi
+from bookleaf_f90_amd import problem as P
+from tests.helpers import granular_sequence
+
+
+def _oracle(p):
+    from oracle.pyoracle import Oracle
+    return Oracle(p)
+
+
+def _uniform(nx, ny, h, rho=1.0, e=2.5, u=0.0, v=0.0, bcs=("SLIPY", "SLIPX", "SLIPY", "SLIPX"),
+             eulerian=False, **cfg):
+    ielnd, ndx, ndy, indtype = P.structured_mesh(nx, ny, 0.0, nx * h, 0.0, ny * h, bcs=bcs)
+    nel, nnd = nx * ny, len(ndx)
+    itype, param = P.ideal_eos([1.4])
+    config = dict(P.DEFAULT_CONFIG, time_end=1.0e30, **cfg)
+    if eulerian:
+        config["zeul"] = 1
+    return P.Problem(name="uniform", ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+                     ielreg=np.ones(nel, np.int32), ielmat=np.ones(nel, np.int32),
+                     density=np.full(nel, rho), energy=np.full(nel, e),
+                     ndu=np.full(nnd, u), ndv=np.full(nnd, v), nmat=1, nreg=1,
+                     eos_itype=itype, eos_param=param, config=config, nsteps=10)
+
+
+def _random_state(n=10, seed=7, kappa=0.05, pmerit=0.1):
+    """Jittered mesh with random density, energy and velocity, hourglass and
+    sub-zonal pressure on: every force term is non-zero."""
+    p = P.sedov(n=n, nsteps=5, kappa=kappa, jitter=0.15, seed=seed)
+    rng = np.random.default_rng(seed)
+    p.density = rng.uniform(0.5, 2.0, p.nel)
+    p.energy = rng.uniform(1.0, 3.0, p.nel)
+    p.ndu = rng.uniform(-0.3, 0.3, p.nnd)
+    p.ndv = rng.uniform(-0.3, 0.3, p.nnd)
+    p.pmeritreg[:] = pmerit
+    p.__post_init__()
+    return p
+
+
+def _run_until(o, label):
+    for lab, fn in granular_sequence(o):
+        fn()
+        if lab == label:
+            return
+    raise AssertionError(label)
+
+
+def test_corner_forces_sum_to_zero(oracle_lib):
+    p = _random_state()
+    o = _oracle(p)
+    nel = p.nel
+    checked = 0
+    for label, fn in granular_sequence(o):
+        fn()
+        if label not in (201, 301):  # predictor (P+Q), corrector (P+Q+SP+HG)
+            continue
+        checked += 1
+        fx = o.get(abi.LAGSTEP["cnfx"])[:4 * nel].reshape(nel, 4)
+        fy = o.get(abi.LAGSTEP["cnfy"])[:4 * nel].reshape(nel, 4)
+        scale = np.abs(fx).sum(axis=1) + np.abs(fy).sum(axis=1)
+        assert np.all(scale > 0.0)
+        assert np.all(np.abs(fx.sum(axis=1)) <= 1e-13 * scale), label
+        assert np.all(np.abs(fy.sum(axis=1)) <= 1e-13 * scale), label
+    assert checked == 2
+
+
+def test_corner_weights_sum_to_volume(oracle_lib):
+    p = _random_state()
+    o = _oracle(p)
+    nel = p.nel
+    wt = o.get("cnwt")[:4 * nel].reshape(nel, 4)
+    vol = o.get("elvolume")[:nel]
+    assert np.all(vol > 0.0)
+    np.testing.assert_allclose(wt.sum(axis=1), vol, rtol=1e-14)
+    # rectangles: V = hx*hy exactly (to rounding), each corner V/4
+    q = _uniform(6, 3, 0.25)
+    o = _oracle(q)
+    vol = o.get("elvolume")[:q.nel]
+    wt = o.get("cnwt")[:4 * q.nel].reshape(q.nel, 4)
+    np.testing.assert_allclose(vol, 0.0625, rtol=1e-15)
+    np.testing.assert_allclose(wt, 0.0625 / 4.0, rtol=1e-15)
+
+
+def test_ideal_gas_eos(oracle_lib):
+    p = _random_state()
+    o = _oracle(p)
+    o.hydro_geteos()
+    n = p.nel
+    rho, e = o.get("eldensity")[:n], o.get("elenergy")[:n]
+    pre, cs2 = o.get("elpressure")[:n], o.get("elcs2")[:n]
+    g = 1.4
+    np.testing.assert_allclose(pre, (g - 1.0) * rho * e, rtol=1e-15)
+    np.testing.assert_allclose(cs2, g * (g - 1.0) * e, rtol=1e-15)
+
+
+def test_uniform_mesh_cfl_dt(oracle_lib):
+    h, e, g = 0.01, 2.5, 1.4
+    p = _uniform(8, 5, h, e=e)
+    o = _oracle(p)
+    cfl, div = o.hydro_getdt()
+    c = np.sqrt(g * (g - 1.0) * e)
+    assert cfl.rdt == pytest.approx(p.config["cfl_sf"] * h / c, rel=1e-14)
+    assert 1 <= cfl.idt <= p.nel and cfl.sdt == abi.DT_CFL
+    # at rest there is no divergence: the DIV control never wins
+    assert div.rdt > 1e100
+
+
+@pytest.mark.parametrize("eulerian", [False, True])
+def test_uniform_state_at_rest_is_steady(eulerian, oracle_lib):
+    p = _uniform(10, 6, 0.1, eulerian=eulerian)
+    o = _oracle(p)
+    o.run(10)
+    n = p.nel
+    assert np.max(np.abs(o.get("ndu")[:p.nnd])) <= 1e-13
+    assert np.max(np.abs(o.get("ndv")[:p.nnd])) <= 1e-13
+    np.testing.assert_allclose(o.get("eldensity")[:n], 1.0, rtol=1e-13)
+    np.testing.assert_allclose(o.get("elenergy")[:n], 2.5, rtol=1e-13)
+    assert np.all(o.get("elvisc")[:n] == 0.0)
+
+
+def test_lagrangian_mass_is_never_written(oracle_lib):
+    p = _random_state(kappa=0.01, pmerit=0.0)
+    o = _oracle(p)
+    m0, cm0 = o.get("elmass").copy(), o.get("cnmass").copy()
+    o.run(5)
+    assert np.array_equal(o.get("elmass"), m0)
+    assert np.array_equal(o.get("cnmass"), cm0)
+
+
+def test_rigid_translation(oracle_lib):
+    """Free boundaries and no pressure (e = 0): nothing but the velocity acts."""
+    u, v = 0.3, -0.2
+    p = _uniform(8, 6, 0.05, e=0.0, u=u, v=v, bcs=("FREE",) * 4)
+    o = _oracle(p)
+    x0, y0 = o.get("ndx").copy(), o.get("ndy").copy()
+    o.run(10)
+    n, t = p.nel, o.ts.time
+    assert t > 0.0
+    np.testing.assert_allclose(o.get("eldensity")[:n], 1.0, rtol=1e-12)
+    assert np.all(o.get("elenergy")[:n] == 0.0)
+    assert np.all(o.get("elpressure")[:n] == 0.0)
+    np.testing.assert_allclose(o.get("ndu")[:p.nnd], u, rtol=1e-12)
+    np.testing.assert_allclose(o.get("ndx")[:p.nnd], x0[:p.nnd] + u * t, atol=1e-12)
+    np.testing.assert_allclose(o.get("ndy")[:p.nnd], y0[:p.nnd] + v * t, atol=1e-12)
+    assert np.all(o.get("elvisc")[:n] == 0.0)
+
+
+def test_node_scatter_equals_brute_force_sum(oracle_lib):
+    p = _random_state(n=6)
+    o = _oracle(p)
+    _run_until(o, 302)  # initacceleration + scatteracceleration
+    nel, nnd = p.nel, p.nnd
+    zerocut = p.config["zerocut"]
+    ielnd = o.get("ielnd")[:4 * nel]
+    srt = o.get("ielsort1")[:nel]
+    rho, wt = o.get("eldensity"), o.get("cnwt")
+    cm = o.get("cnmass")
+    fx, fy = o.get(abi.LAGSTEP["cnfx"]), o.get(abi.LAGSTEP["cnfy"])
+    mass, area, fu, fv = [0.0] * nnd, [0.0] * nnd, [0.0] * nnd, [0.0] * nnd
+    for jj in range(4):
+        for kk in range(nel):
+            e = int(srt[kk]) - 1
+            c = 4 * e + jj
+            n = int(ielnd[c]) - 1
+            if cm[c] > zerocut:
+                mass[n] = mass[n] + float(cm[c])
+            elif cm[4 * e + (jj + 3) % 4] > zerocut:
+                mass[n] = mass[n] + float(cm[4 * e + (jj + 3) % 4])
+            else:
+                mass[n] = mass[n] + float(rho[e]) * float(wt[c])
+            area[n] = area[n] + float(wt[c])
+            fu[n] = fu[n] + float(fx[c])
+            fv[n] = fv[n] + float(fy[c])
+    L = abi.LAGSTEP
+    assert np.array_equal(o.get(L["ndmass"])[:nnd], np.array(mass))
+    assert np.array_equal(o.get(L["ndarea"])[:nnd], np.array(area))
+    assert np.array_equal(o.get(L["ndubar"])[:nnd], np.array(fu))
+    assert np.array_equal(o.get(L["ndvbar"])[:nnd], np.array(fv))
+
+
+def _sod_l1(o, p):
+    """check_kn_sod + check_dr_tests: volume-weighted L1 of density against
+    sod(x_c - 50, t) over element centroids (geometry_kn_getcentroid)."""
+    from oracle.pyoracle import sod_exact
+    nel = p.nel
+    cx = o.get("cnx")[:4 * nel].reshape(nel, 4)
+    cy = o.get("cny")[:4 * nel].reshape(nel, 4)
+    vol, rho = o.get("elvolume")[:nel], o.get("eldensity")[:nel]
+    t = o.ts.time
+    basis, l1 = 0.0, 0.0
+    for i in range(nel):
+        c, rv = 0.0, 0.0
+        for i1 in range(4):
+            i2 = (i1 + 1) % 4
+            rr = cx[i, i1] * cy[i, i2] - cx[i, i2] * cy[i, i1]
+            rv = rv + rr
+            c = c + rr * (cx[i, i1] + cx[i, i2])
+        rv = 0.5 * rv
+        xc = c / (6.0 * rv) if rv > 0.0 else 0.0
+        r, _ = sod_exact(xc - 50.0, t)
+        basis = basis + vol[i]
+        l1 = abs(rho[i] - r) * vol[i] + l1
+    return l1 / basis
+
+
+@pytest.mark.parametrize("eulerian,expected", [(False, "5.19756949E-03"),
+                                               (True, "6.74455780E-03")])
+def test_sod_deck_l1_matches_reference_run(eulerian, expected, oracle_lib):
+    p = P.sod_deck_small(eulerian=eulerian)
+    o = _oracle(p)
+    dt, _, _ = o.run(5000, stop_at_time_end=True)
+    assert len(dt) == 384
+    assert "%.8E" % _sod_l1(o, p) == expected
+
+
+def test_sod_exact_matches_closed_form_states(oracle_lib):
+    """sod() of check_kn_tests.f90: the four constant states and the
+    rarefaction fan's self-similar form."""
+    from oracle.pyoracle import sod_exact
+    t = 10.0
+    assert sod_exact(-50.0, t) == (1.0, 2.5)
+    assert sod_exact(0.5 * t, t) == (0.42631942817849519385, 1.77760006942335264781)
+    assert sod_exact(1.5 * t, t) == (0.26557371170530706471, 2.85354088799095973738)
+    assert sod_exact(40.0, t) == (0.125, 2.0)
+    # inside the fan: rho = t**2.5 with t = ((u - s)^2)/gamma, u = 5/6 (s + sqrt(g))
+    s = -0.5
+    u = (5.0 / 6.0) * (s + np.sqrt(1.4))
+    tt = ((u - s) ** 2) / 1.4
+    r, e = sod_exact(s * t, t)
+    assert r == pytest.approx(tt ** 2.5, rel=1e-15)
+    assert e == pytest.approx(tt / 0.4, rel=1e-15)
