@@ -53,5 +53,38 @@ def build(force=True, verbose=False):
     return LIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN, "_build")
+
+
+def build_fortran(programs=()):
+    """Compile the Fortran binding bookleaf_hip_mod (fortran/bookleaf_hip_mod.f90)
+    into fortran/_build and link each program source given against it and
+    libbookleaf_hip.so (rpath relative to the executable).  Returns the list of
+    executables, or None when flang is not installed."""
+    if not os.path.exists(FLANG):
+        return None
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    mod_o = os.path.join(FORTRAN_BUILD, "bookleaf_hip_mod.o")
+    cmds = [[FLANG, "-O2", "-J", FORTRAN_BUILD, "-c",
+             os.path.join(FORTRAN, "bookleaf_hip_mod.f90"), "-o", mod_o]]
+    exes = []
+    rpath = "-Wl,-rpath,$ORIGIN/" + os.path.relpath(HERE, FORTRAN_BUILD)
+    for src in programs:
+        src = os.path.abspath(src)
+        exe = os.path.join(FORTRAN_BUILD, os.path.splitext(os.path.basename(src))[0])
+        cmds.append([FLANG, "-O2", "-I", FORTRAN_BUILD, src, mod_o, "-L", HERE,
+                     "-lbookleaf_hip", rpath, "-o", exe])
+        exes.append(exe)
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=FORTRAN_BUILD)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("flang failed: " + " ".join(cmd))
+    return exes
+
+
 if __name__ == "__main__":
     print(build(force=True, verbose=True))
+    print(build_fortran())

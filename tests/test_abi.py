@@ -62,3 +62,21 @@ def test_missing_library_fails_loudly(tmp_path):
             device.load_library(str(tmp_path / "absent.so"))
     finally:
         device._LIB = saved
+
+
+def test_fortran_binding_declares_every_entry_point():
+    """bookleaf_f90_amd/fortran/bookleaf_hip_mod.f90 binds every C entry point
+    under its own name."""
+    src = open(os.path.join(ROOT, "bookleaf_f90_amd", "fortran", "bookleaf_hip_mod.f90")).read()
+    bound = set(re.findall(r'NAME="(bl_[a-z0-9_]+)"', src))
+    assert sorted(set(declared_functions()) - bound) == []
+
+
+def test_fortran_binding_and_host_build(libpath):
+    from bookleaf_f90_amd import build
+    if not os.path.exists(build.FLANG):
+        pytest.skip("flang not installed")
+    exes = build.build_fortran([os.path.join(ROOT, "tests", "fortran", "fortran_host.f90")])
+    assert all(os.access(e, os.X_OK) for e in exes)
+    out = subprocess.run(["ldd", exes[0]], capture_output=True, text=True).stdout
+    assert "libbookleaf_hip.so" in out and "not found" not in out
