@@ -75,7 +75,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1024, help="mesh is n x n (sedov)")
     ap.add_argument("--workload", choices=["sedov", "sod2mat"], default="sedov",
                     help="sedov: Lagrangian 1024^2 Sedov (the headline); sod2mat: Eulerian "
-                         "(ALE remap every step) two-material Sod on 2048x256, pure cells")
+                         "(ALE remap every step) two-material Sod on 2048x256 with mixed "
+                         "interface cells")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -126,8 +127,8 @@ def main():
         wname = "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_%dsteps" % (
             args.n, args.n, args.warmup + args.steps)
     else:
-        prob = P.sod2mat(nx=2048, ny=256, nsteps=args.warmup + args.steps)
-        wname = "sod2mat_eulerian_2048x256_pure_cells_seed2024_%dsteps" % (
+        prob = P.sod2mat_mixed(nx=2048, ny=256, nsteps=args.warmup + args.steps)
+        wname = "sod2mat_eulerian_2048x256_mixed_interface_seed2024_%dsteps" % (
             args.warmup + args.steps)
     nel, nnd = prob.nel, prob.nnd
     g = BookLeaf(prob)

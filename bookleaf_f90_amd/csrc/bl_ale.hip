@@ -15,6 +15,7 @@
 
 #include "bl_internal.h"
 #include "bl_math.h"
+#include "bl_mix.h"
 
 namespace bl {
 
@@ -522,6 +523,8 @@ int ale_step_device(Handle* hb, const DevTimestep* d_ts) {
             h->R(BL_ELVOLUME), h->R(BL_ELMASS), h->R(BL_ELDENSITY), h->R(BL_ELENERGY),
             h->R(BL_ELPRESSURE), h->R(BL_ELCS2), h->I(BL_IELMAT), h->d_eos_itype,
             h->d_eos_param, h->cfg.pcut, h->cfg.ccut, h->d_dtres, d_ts);
+  // mixed cells: component geometry, density and EOS of the hydro update
+  if (mix_geometry(h, d_ts) || mix_density(h, d_ts) || mix_eos(h, d_ts)) return BL_FAILURE;
   return BL_SUCCESS;
 }
 

@@ -18,6 +18,7 @@
 
 #include "bl_internal.h"
 #include "bl_kernels.h"
+#include "bl_mix.h"
 #include "bl_math.h"
 
 namespace bl {
@@ -499,11 +500,14 @@ __global__ void k_getdt_c(int nel, const int32_t* __restrict__ ielreg,
                           const int32_t* __restrict__ zdtnotreg,
                           const double* __restrict__ elvisc, double* __restrict__ cs2,
                           const double* __restrict__ ellength, double* __restrict__ rs11,
-                          double* __restrict__ part, int32_t* __restrict__ parti, int nblk_a) {
+                          double* __restrict__ part, int32_t* __restrict__ parti, int nblk_a,
+                          const int32_t* __restrict__ ielmat, MixArgs M) {
   int e = xcd_index();
   double r = DBL_MAX;
   if (e < nel) {
     double c = cs2[e] + elvisc[e];
+    // mixed cell: the frmass average of the components (hydro_dr_getcs2.f90:40-42)
+    if (M.ncp > 0 && ielmat[e] < 0) c = mix_cs2(M, -ielmat[e] - 1);
     cs2[e] = c;
     r = zdtnotreg[ielreg[e] - 1] ? DBL_MAX : ellength[e] / c;
     rs11[e] = r;
@@ -899,6 +903,7 @@ int bl_geometry_getgeometry(bl_handle_t* h, bl_error_t* err) {
             h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1), h->R(BL_A2),
             h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
             h->R(BL_ELVOLUME), h->d_dtres);
+  if (mix_geometry(h, nullptr)) return BL_FAILURE;
   BL_HIP(hipMemcpyAsync(&h->h_dtres->negvol, &h->d_dtres->negvol, sizeof(int32_t),
                         hipMemcpyDeviceToHost, h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));
@@ -923,7 +928,7 @@ int bl_density_getdensity(bl_handle_t* h) {
   int nel = h->sz.nel;
   BL_LAUNCH(h, "getdensity", k_divide, nblocks(nel), kBlock, nel, h->R(BL_ELMASS),
             h->R(BL_ELVOLUME), h->R(BL_ELDENSITY));
-  return BL_SUCCESS;
+  return mix_density(h, nullptr);
 }
 
 int bl_hydro_geteos(bl_handle_t* h) {
@@ -932,7 +937,7 @@ int bl_hydro_geteos(bl_handle_t* h) {
   BL_LAUNCH(h, "geteos", k_geteos, nblocks(nel), kBlock, nel, h->I(BL_IELMAT),
             h->R(BL_ELDENSITY), h->R(BL_ELENERGY), h->R(BL_ELPRESSURE), h->R(BL_ELCS2),
             h->d_eos_itype, h->d_eos_param, h->cfg.pcut, h->cfg.ccut);
-  return BL_SUCCESS;
+  return mix_eos(h, nullptr);
 }
 
 int bl_init_state(bl_handle_t* h) {
@@ -940,6 +945,7 @@ int bl_init_state(bl_handle_t* h) {
   int nel = h->sz.nel;
   BL_LAUNCH(h, "elmass", k_elmass, nblocks(nel), kBlock, nel, h->R(BL_ELDENSITY),
             h->R(BL_ELVOLUME), h->R(BL_CNWT), h->R(BL_ELMASS), h->R(BL_CNMASS));
+  if (mix_initmass(h)) return BL_FAILURE;
   return bl_hydro_geteos(h);
 }
 
@@ -952,6 +958,7 @@ int bl_hydro_init(bl_handle_t* h) {
   if (h->cfg.zsp)
     BL_LAUNCH(h, "spmassinit", k_spmassinit, nblocks(h->sz.nel), kBlock, h->sz.nel,
               h->R(BL_ELDENSITY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_SPMASS));
+  if (mix_viscinit(h)) return BL_FAILURE;
   BL_HIP(hipStreamSynchronize(h->stream));
   return BL_SUCCESS;
 }
@@ -976,7 +983,7 @@ int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) 
             h->R(BL_RSCRATCH12));
   BL_LAUNCH(h, "getdt_c", k_getdt_c, nb, kBlock, nel, h->I(BL_IELREG), h->d_zdtnotreg,
             h->R(BL_ELVISC), h->R(BL_ELCS2), h->R(BL_RSCRATCH12), h->R(BL_RSCRATCH11),
-            h->d_part, h->d_parti, (int)nb);
+            h->d_part, h->d_parti, (int)nb, (const int32_t*)h->I(BL_IELMAT), make_mix(h));
   BL_LAUNCH(h, "dt_final", k_dt_final, 1, kFinalBlock, (int)nb, (int)nb, h->d_part, h->d_parti,
             h->d_dtres);
   BL_HIP(hipMemcpyAsync(h->h_dtres, h->d_dtres, sizeof(DtResult), hipMemcpyDeviceToHost, h->stream));
@@ -1022,7 +1029,7 @@ int bl_hydro_getforce(bl_handle_t* h, const bl_timestep_t* ts) {
             h->I(BL_IELREG), h->R(BL_ELDENSITY), h->R(BL_ELCS2), h->R(BL_CNX), h->R(BL_CNY),
             h->R(BL_SPMASS), h->R(BL_ELVOLUME), h->R(BL_RSCRATCH21), h->R(BL_RSCRATCH22),
             h->d_kappareg, h->d_pmeritreg);
-  return BL_SUCCESS;
+  return mix_force(h);
 }
 
 int bl_hydro_getstate(bl_handle_t* h, const bl_timestep_t* ts, bl_error_t* err) {
@@ -1040,12 +1047,14 @@ int bl_hydro_getenergy(bl_handle_t* h, const bl_timestep_t* ts) {
   BL_LAUNCH(h, "getenergy", k_getenergy, nblocks(nel), kBlock, nel, ts->dts, h->cfg.zerocut,
             h->R(BL_ELENERGY), h->R(BL_ELMASS), h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24),
             h->R(BL_RSCRATCH21), h->R(BL_RSCRATCH22));
-  return BL_SUCCESS;
+  return mix_energy(h, ts->dts);
 }
 
 int bl_hydro_initacceleration(bl_handle_t* h) {
-  // single material and nproc=1: no component force average, no exchange
-  return h ? BL_SUCCESS : BL_FAILURE;
+  // mixed cells: element corner forces from the components
+  // (hydro_dr_getacceleration.f90:69-72); nproc=1: no exchange
+  if (!h) return BL_FAILURE;
+  return mix_average_force(h);
 }
 
 int bl_hydro_scatteracceleration(bl_handle_t* h) {

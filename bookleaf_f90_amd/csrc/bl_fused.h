@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "bl_internal.h"
+#include "bl_mix.h"
 
 namespace bl {
 
@@ -82,6 +83,7 @@ struct FusedArgs {
   DevTimestep* ts;
   double* hist;
   int hist_cap;
+  MixArgs mx;  // mixed cells (mx.ncp == 0: none)
 };
 
 }  // namespace bl

@@ -22,6 +22,10 @@
 //                acceleration, BCs, u^{n+1}, ubar, x^{n+1} into the other
 //                node buffer (ping-pong: no node is read after it is written)
 //   F4 state   : corrector geometry, density, energy with ubar, EOS
+//   F2m, F4m   : mixed cells only (ncp > 0), one thread per mixed cell after
+//                F2 / F4: the component terms of the hydro drivers, and the
+//                cell's corrector corner force replaced by the frvolume
+//                average of its component forces
 //
 // Every FP64 expression is the same function of the same inputs as in the
 // per-driver path (bl_math.h), so the results are bit-identical to the
@@ -150,8 +154,13 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
     st4(A.cnviscx, e, qx);
     st4(A.cnviscy, e, qy);
     A.elvisc[e] = ev;
-    // CFL (hydro_kn_getdtcfl) with cs2 + elvisc (hydro_dr_getcs2)
-    const double c2 = cs2e + ev;
+    // CFL (hydro_kn_getdtcfl) with cs2 + elvisc (hydro_dr_getcs2); a mixed
+    // cell takes the frmass average of its components instead
+    double c2 = cs2e + ev;
+    if (A.mx.ncp > 0) {
+      const int im = A.ielmat[e];
+      if (im < 0) c2 = mix_cs2(A.mx, -im - 1);
+    }
     int dtnot = 0, mid = 0;
     if (A.reg_tables) {
       int ireg = A.ielreg[e] - 1;
@@ -207,6 +216,122 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb)
   if (best < A.dt_min) {  // time_dr_advance.f90:122-127
     ts->err = 3; ts->done = 1; ts->skip = 1; ts->err_step = ts->nstep;
   }
+}
+
+// component terms of a mixed cell in F2 (hydro_dr_getforce.f90:52-66,
+// hydro_dr_getenergy.f90:47-62, geometry_dr_volume.f90:66-79,
+// density_dr_getdensity.f90:40-43, eos_dr_geteos.f90:50-55,
+// hydro_dr_getacceleration.f90:69-72): predictor component force at x^n,
+// predictor component energy (cpu = frvolume*u^n corners, dts = dt/2),
+// half-step component volume, density and pressure, corrector component
+// force (kept in rcpscratch21/22 for F4), and the cell's corrector corner
+// force replaced by the frvolume average of the component forces.
+// Component energies are not saved and restored between predictor and
+// corrector (hydro_dr_set.f90 copies elenergy only).
+__device__ __forceinline__ void mix_f2(const FusedArgs& A, int imx, const Iso& g,
+                                      const Iso& gh, const double u[4], const double v[4],
+                                      double h, double fx[4], double fy[4]) {
+  const MixArgs& M = A.mx;
+  int icp = M.imxfcp[imx] - 1;
+  const int n = M.imxncp[imx];
+  double w1[4] = {0.0, 0.0, 0.0, 0.0}, w2[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int ii = 0; ii < n; ++ii, ++icp) {
+    const double f = M.frvolume[icp], cm = M.cpmass[icp];
+    double cfx[4], cfy[4], cu[4], cv[4];
+    mix_force(M, icp, M.cppressure[icp], g.a1, g.a3, g.b1, g.b3, cfx, cfy);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { cu[j] = f * u[j]; cv[j] = f * v[j]; }
+    const double ce = energy_update(M.cpenergy[icp], cm, cfx, cfy, cu, cv, h, A.zerocut);
+    M.cpenergy[icp] = ce;
+    const double crho = cm / (gh.vol * f);
+    int cmat = M.icpmat[icp];
+    cmat = cmat > 0 ? cmat : 0;
+    const double cph = eos_pressure(A.eos_itype[cmat], A.eos_param + BL_NEOSPARAM * cmat, crho,
+                                    ce, A.pcut);
+    mix_force(M, icp, cph, gh.a1, gh.a3, gh.b1, gh.b3, cfx, cfy);
+    st4c(M.cpfx, icp, cfx);
+    st4c(M.cpfy, icp, cfy);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { w1[j] = w1[j] + f * cfx[j]; w2[j] = w2[j] + f * cfy[j]; }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) { fx[j] = w1[j]; fy[j] = w2[j]; }
+}
+
+// component terms of a mixed cell in F4: corrector geometry (the cell's iso
+// terms at x^{n+1}), volume, density, energy with cpu = frvolume*ubar
+// corners, EOS
+__device__ __forceinline__ void mix_f4(const FusedArgs& A, int imx, const Iso& g,
+                                      const double ub[4], const double vb[4], double dt) {
+  const MixArgs& M = A.mx;
+  int icp = M.imxfcp[imx] - 1;
+  const int n = M.imxncp[imx];
+  for (int ii = 0; ii < n; ++ii, ++icp) {
+    const double f = M.frvolume[icp], cm = M.cpmass[icp];
+    M.cpa1[icp] = g.a1; M.cpa3[icp] = g.a3; M.cpb1[icp] = g.b1; M.cpb3[icp] = g.b3;
+    const double cvol = g.vol * f;
+    M.cpvolume[icp] = cvol;
+    const double crho = cm / cvol;
+    M.cpdensity[icp] = crho;
+    double cfx[4], cfy[4], cu[4], cv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { cu[j] = f * ub[j]; cv[j] = f * vb[j]; }
+    ld4c(M.cpfx, icp, cfx);
+    ld4c(M.cpfy, icp, cfy);
+    const double ce = energy_update(M.cpenergy[icp], cm, cfx, cfy, cu, cv, dt, A.zerocut);
+    M.cpenergy[icp] = ce;
+    int cmat = M.icpmat[icp];
+    cmat = cmat > 0 ? cmat : 0;
+    const int et = A.eos_itype[cmat];
+    const double* ep = A.eos_param + BL_NEOSPARAM * cmat;
+    M.cppressure[icp] = eos_pressure(et, ep, crho, ce, A.pcut);
+    M.cpcs2[icp] = eos_cs2(et, ep, crho, ce, A.pcut, A.ccut);
+  }
+}
+
+// F2m: one thread per mixed cell, after F2 (whose corrector corner force of
+// the cell it replaces): recomputes the cell's geometry at x^n and at the
+// half step exactly as F2 does, then the component terms
+__global__ void __launch_bounds__(kBlock) k_f2_mix(FusedArgs A, FusedNodes N) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip) return;
+  const int imx = xcd_index();
+  if (imx >= A.mx.nmx) return;
+  const int e = A.mx.imxel[imx] - 1;
+  const double h = 0.5 * ts->dt;
+  int nd[4];
+  load_nd(A.ielnd, e, nd);
+  double x[4], y[4], u[4], v[4], xh[4], yh[4];
+  load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) { xh[c] = x[c] + h * u[c]; yh[c] = y[c] + h * v[c]; }
+  Iso g, gh;
+  getiso(x, y, g);
+  getiso(xh, yh, gh);
+  double fx[4], fy[4];
+  mix_f2(A, imx, g, gh, u, v, h, fx, fy);
+  const int nel = A.nel;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    A.fxT[j * nel + e] = fx[j];
+    A.fyT[j * nel + e] = fy[j];
+  }
+}
+
+// F4m: one thread per mixed cell, after F4: the components' corrector state
+__global__ void __launch_bounds__(kBlock) k_f4_mix(FusedArgs A, FusedNodes O) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || A.dtres->negvol) return;
+  const int imx = xcd_index();
+  if (imx >= A.mx.nmx) return;
+  const int e = A.mx.imxel[imx] - 1;
+  int nd[4];
+  load_nd(A.ielnd, e, nd);
+  double x[4], y[4], ub[4], vb[4];
+  load4(O.x, nd, x); load4(O.y, nd, y); load4(A.ubar, nd, ub); load4(A.vbar, nd, vb);
+  Iso g;
+  getiso(x, y, g);
+  mix_f4(A, imx, g, ub, vb, ts->dt);
 }
 
 // ---------------------------------------------------------------- F2
@@ -448,6 +573,7 @@ FusedArgs make_args(bl_handle_t* h) {
   A.ts = h->d_ts;
   A.hist = h->d_hist;
   A.hist_cap = (int)h->hist_cap;
+  A.mx = make_mix(h);
   return A;
 }
 
@@ -465,9 +591,13 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
   BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe);
   BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
+  if (A.mx.ncp > 0) BL_LAUNCH(h, "f2_mix", k_f2_mix, nblocks(A.mx.nmx), kBlock, A, N);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
   BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
+  if (A.mx.ncp > 0) BL_LAUNCH(h, "f4_mix", k_f4_mix, nblocks(A.mx.nmx), kBlock, A, Oc);
   BL_LAUNCH(h, "f5_end", k_f5_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  // mixed cells before the remap: the corrector's cpu/cpv in rscratch23/24
+  if (h->cfg.zexist && mix_cpu_overlay(h, h->d_ts)) return BL_FAILURE;
   if (h->cfg.zexist) {
     // the ALE kernels address the registry's node fields: point them at the
     // set this step wrote
@@ -637,6 +767,7 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
               h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
               h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
               h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
+    if (mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
   }
   if (taken > 0) {
     std::vector<double> hist(3 * (size_t)taken);

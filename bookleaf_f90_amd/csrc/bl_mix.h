@@ -1,0 +1,92 @@
+// bl_mix.h -- mixed cells (ncp > 0): component fields and the component
+// terms of the hydro drivers.
+//
+// A mixed element iel has ielmat(iel) = -imx; its components are the
+// contiguous range imxfcp(imx) .. imxfcp(imx)+imxncp(imx)-1 of the flattened
+// lists (mix_dr_flatten, mix_dr_list.f90).  Every sum over a cell's
+// components runs in that order, as utils_kn_average* do
+// (utils_kn_average.f90:29-107), so the element averages are bit-identical.
+// Component corner arrays are (4, mcp), component j of component icp at
+// 4*icp + j.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bl_internal.h"
+#include "bl_math.h"
+
+namespace bl {
+
+struct MixArgs {
+  int ncp, nmx;
+  const int32_t* imxel;   // 1-based element of each mixed cell
+  const int32_t* imxfcp;  // 1-based first component
+  const int32_t* imxncp;
+  const int32_t* icpmat;
+  const double* frvolume;
+  double* frmass;
+  double* cpdensity;
+  double* cpenergy;
+  double* cppressure;
+  double* cpcs2;
+  double* cpvolume;
+  double* cpmass;
+  const double* cpvisc;
+  const double* cpviscx;  // (4, mcp)
+  const double* cpviscy;
+  double* cpa1;
+  double* cpa3;
+  double* cpb1;
+  double* cpb3;
+  double* cpfx;  // rcpscratch21 (lagstep cpfx), (4, mcp)
+  double* cpfy;  // rcpscratch22
+};
+
+MixArgs make_mix(Handle* h);
+
+// component forces of hydro_dr_getforce (hydro_dr_getforce.f90:52-66):
+// pressure force with the component pressure and iso terms, plus cpvisc
+__device__ __forceinline__ void mix_force(const MixArgs& M, int icp, double p, double a1,
+                                          double a3, double b1, double b3, double fx[4],
+                                          double fy[4]) {
+  forceP(p, a1, a3, b1, b3, fx, fy);
+  const double4 qx = reinterpret_cast<const double4*>(M.cpviscx)[icp];
+  const double4 qy = reinterpret_cast<const double4*>(M.cpviscy)[icp];
+  fx[0] = fx[0] + qx.x; fx[1] = fx[1] + qx.y; fx[2] = fx[2] + qx.z; fx[3] = fx[3] + qx.w;
+  fy[0] = fy[0] + qy.x; fy[1] = fy[1] + qy.y; fy[2] = fy[2] + qy.z; fy[3] = fy[3] + qy.w;
+}
+
+// hydro_dr_getcs2 for a mixed element (hydro_dr_getcs2.f90:40-42,
+// utils_kn_average12): sum over components of frmass*(cpcs2 + cpvisc)
+__device__ __forceinline__ double mix_cs2(const MixArgs& M, int imx) {
+  int icp = M.imxfcp[imx] - 1;
+  const int n = M.imxncp[imx];
+  double w1 = 0.0;
+  for (int ii = 0; ii < n; ++ii, ++icp) w1 = w1 + M.frmass[icp] * (M.cpcs2[icp] + M.cpvisc[icp]);
+  return w1;
+}
+
+__device__ __forceinline__ void st4c(double* a, int icp, const double v[4]) {
+  reinterpret_cast<double4*>(a)[icp] = make_double4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void ld4c(const double* a, int icp, double v[4]) {
+  const double4 q = reinterpret_cast<const double4*>(a)[icp];
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+
+// launches of the per-driver component terms (bl_mix.hip); with ts non-null
+// the kernels are gated like the ALE kernels of a captured step
+int mix_geometry(Handle* h, const DevTimestep* ts);
+int mix_density(Handle* h, const DevTimestep* ts);
+int mix_eos(Handle* h, const DevTimestep* ts);
+int mix_initmass(Handle* h);
+int mix_viscinit(Handle* h);
+int mix_force(Handle* h);
+int mix_energy(Handle* h, double dts);
+int mix_average_force(Handle* h);
+// the component corner velocities cpu/cpv of the last corrector getenergy
+// into rscratch23/24 (the reference's lagstep aliases cpu => rscratch23,
+// cpv => rscratch24, dataAPI_id.f90:94-96), from ubar in rscratch14/15
+int mix_cpu_overlay(Handle* h, const DevTimestep* ts);
+
+}  // namespace bl

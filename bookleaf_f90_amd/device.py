@@ -97,6 +97,10 @@ class DriverState:
             self.set(name, getattr(p, name))
         self.set("eldensity", p.density)
         self.set("elenergy", p.energy)
+        if p.ncp:  # mixed cells: the flattened component lists and states
+            for name in ("imxel", "imxfcp", "imxncp", "icpmat", "icpnext", "icpprev",
+                         "frvolume", "cpdensity", "cpenergy"):
+                self.set(name, getattr(p, name))
         err = abi.Error()
         self._check(self._f("geometry_getgeometry")(self.h, C.byref(err)), "getgeometry", err)
         self._check(self._f("init_state")(self.h), "init_state")

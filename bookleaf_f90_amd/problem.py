@@ -60,6 +60,17 @@ class Problem:
     nsteps: int = 1
     stop_at_time_end: int = 0
     dims: tuple = None
+    # mixed cells (ncp > 0): the flattened component lists of mix_dr_flatten
+    # (mix_dr_list.f90), 1-based contents; ielmat(iel) = -imx for mixed cells
+    imxel: np.ndarray = None       # (nmx) element of each mixed cell
+    imxfcp: np.ndarray = None      # (nmx) first component
+    imxncp: np.ndarray = None      # (nmx) number of components
+    icpmat: np.ndarray = None      # (ncp) material of each component
+    icpnext: np.ndarray = None     # (ncp) next component (0: last)
+    icpprev: np.ndarray = None     # (ncp) previous component (-imx: first)
+    frvolume: np.ndarray = None    # (ncp) volume fraction
+    cpdensity: np.ndarray = None   # (ncp)
+    cpenergy: np.ndarray = None    # (ncp)
 
     def __post_init__(self):
         if self.kappareg is None:
@@ -87,14 +98,61 @@ class Problem:
     def nnd(self):
         return len(self.ndx)
 
+    @property
+    def ncp(self):
+        return 0 if self.icpmat is None else len(self.icpmat)
+
+    @property
+    def nmx(self):
+        return 0 if self.imxel is None else len(self.imxel)
+
     def sizes(self):
         s = abi.Sizes()
         s.nel = s.nel1 = s.nel2 = self.nel
         s.nnd = s.nnd1 = s.nnd2 = self.nnd
         s.nsz = max(self.nel, self.nnd)
         s.nmat, s.nreg = self.nmat, self.nreg
-        s.ncp = s.nmx = s.mcp = s.mmx = 0
+        s.ncp, s.nmx = self.ncp, self.nmx
+        # component capacities as mix_dr_addcp / mix_dr_addel first size them
+        # (INCR = 5%, mix_dr_list.f90:53-57,110-114), and at least 4*ncp: the
+        # reference registers cpviscx/cpviscy with one entry per component
+        # (data.f90:360-361) but hydro_kn_viscinit and hydro_kn_getforceq
+        # address four per component
+        s.mcp = max(4 * self.ncp, int(0.05 * self.nmat * self.nel)) if self.ncp else 0
+        s.mmx = max(self.nmx, int(0.05 * self.nel)) if self.nmx else 0
         return s
+
+    def set_mixed(self, cells):
+        """Make cells mixed.  cells: {element (0-based): [(material, frvolume,
+        density, energy), ...]}.  The lists are laid out as mix_dr_flatten
+        leaves them (mix_kn_list.f90:230-330): mixed cells in ascending element
+        order, each cell's components contiguous and in ascending material
+        order, icpnext/icpprev chaining them, ielmat(iel) = -imx."""
+        imxel, imxfcp, imxncp, icpmat, frv, rho, en = [], [], [], [], [], [], []
+        for e in sorted(cells):
+            comps = sorted(cells[e], key=lambda c: c[0])
+            imxel.append(e + 1)
+            imxfcp.append(len(icpmat) + 1)
+            imxncp.append(len(comps))
+            for m, f, r, u in comps:
+                icpmat.append(m); frv.append(f); rho.append(r); en.append(u)
+            self.ielmat[e] = -len(imxel)
+        ncp = len(icpmat)
+        nxt = np.zeros(ncp, np.int32)
+        prv = np.zeros(ncp, np.int32)
+        for imx, (f, n) in enumerate(zip(imxfcp, imxncp), start=1):
+            for k in range(n):
+                icp = f + k
+                nxt[icp - 1] = icp + 1 if k < n - 1 else 0
+                prv[icp - 1] = icp - 1 if k > 0 else -imx
+        self.imxel = np.array(imxel, np.int32)
+        self.imxfcp = np.array(imxfcp, np.int32)
+        self.imxncp = np.array(imxncp, np.int32)
+        self.icpmat = np.array(icpmat, np.int32)
+        self.icpnext, self.icpprev = nxt, prv
+        self.frvolume = np.array(frv, np.float64)
+        self.cpdensity = np.array(rho, np.float64)
+        self.cpenergy = np.array(en, np.float64)
 
     def cconfig(self):
         c = abi.Config()
@@ -257,6 +315,79 @@ def sod2mat(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps=300,
         ielreg=mat.copy(), ielmat=mat, density=density, energy=energy,
         ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)), nmat=2, nreg=2, eos_itype=itype,
         eos_param=param, config=config, nsteps=nsteps, dims=(nx, ny))
+
+
+def sod_mm_deck_small():
+    """The reference deck input/sod_mm_lag_small: Sod on [0,100]x[0,25],
+    100x25, air (gamma 1.4) everywhere with rho=1, e=2.5 from the region IC,
+    helium (gamma 1.2, rho=0.125, e=2.0) in the rectangle x > 50.5.  The
+    shape test of setup_IC.f90:1585-1634 (intersect: 0.5 when some but not
+    all corners are inside; shape_param(5) = 0.5) makes the column
+    x in [50,51] mixed with 50/50 volume fractions; its air component takes
+    the element's region values (setup_thermodynamic_rationalise,
+    setup_IC.f90:740-790), its helium component the material IC."""
+    nx, ny = 100, 25
+    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, 100.0, 0.0, 25.0)
+    nel = nx * ny
+    i = np.arange(nel) % nx
+    itype, param = ideal_eos([1.4, 1.2])
+    mat = np.where(i >= 51, 2, 1).astype(np.int32)
+    density = np.where(mat == 2, 0.125, 1.0)
+    energy = np.where(mat == 2, 2.0, 2.5)
+    config = dict(DEFAULT_CONFIG, time_end=20.0, dt_initial=1.0e-3)
+    p = Problem(name="sod_mm_lag_small", ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+                ielreg=np.ones(nel, np.int32), ielmat=mat, density=density, energy=energy,
+                ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)), nmat=2, nreg=1,
+                eos_itype=itype, eos_param=param, config=config, nsteps=100000,
+                stop_at_time_end=1, dims=(nx, ny))
+    p.set_mixed({int(e): [(1, 0.5, 1.0, 2.5), (2, 0.5, 0.125, 2.0)]
+                 for e in np.nonzero(i == 50)[0]})
+    return p
+
+
+def sod2mat_mixed(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps=300,
+                  eulerian=True, sub=8, **cfg):
+    """sod2mat with mixed interface cells: the volume fraction of material 2
+    in each element is the fraction of its sub x sub sample points (bilinear
+    in the element) right of the interface; elements with both materials
+    present are mixed with components (1, 1-vf) and (2, vf) at the material
+    states, and element density and energy are the volume- and
+    mass-weighted means."""
+    p = sod2mat(nx=nx, ny=ny, length=length, amplitude=amplitude, seed=seed, nsteps=nsteps,
+                eulerian=eulerian, **cfg)
+    height = length * ny / nx
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(1, 4))
+    phi = float(rng.uniform(0.0, 2.0 * math.pi))
+    n = p.ielnd.reshape(-1, 4) - 1
+    X, Y = p.ndx[n], p.ndy[n]
+    t = (np.arange(sub) + 0.5) / sub
+    s_, t_ = np.meshgrid(t, t)
+    s_, t_ = s_.ravel(), t_.ravel()
+    w = np.stack([(1 - s_) * (1 - t_), s_ * (1 - t_), s_ * t_, (1 - s_) * t_], axis=1)
+    px, py = X @ w.T, Y @ w.T
+    right = px >= 0.5 * length + amplitude * np.sin(2.0 * math.pi * k * py / height + phi)
+    vf = right.mean(axis=1)
+    g1, g2 = 1.4, 1.67
+    st = {1: (1.0, 1.0 / ((g1 - 1.0) * 1.0)), 2: (0.125, 0.1 / ((g2 - 1.0) * 0.125))}
+    mixed = np.nonzero((vf > 0.0) & (vf < 1.0))[0]
+    cells = {}
+    for e in mixed:
+        f2 = float(vf[e])
+        (r1, e1), (r2, e2) = st[1], st[2]
+        cells[int(e)] = [(1, 1.0 - f2, r1, e1), (2, f2, r2, e2)]
+        rho = (1.0 - f2) * r1 + f2 * r2
+        p.density[e] = rho
+        p.energy[e] = ((1.0 - f2) * r1 * e1 + f2 * r2 * e2) / rho
+    pure2 = vf >= 1.0
+    p.ielmat[:] = np.where(pure2, 2, 1)
+    p.ielreg[:] = p.ielmat
+    p.density[pure2], p.energy[pure2] = st[2]
+    pure1 = vf <= 0.0
+    p.density[pure1], p.energy[pure1] = st[1]
+    p.set_mixed(cells)
+    p.name = "sod2mat_mixed_%dx%d" % (nx, ny)
+    return p
 
 
 def noh(n=30, length=1.0, gamma=5.0 / 3.0, nsteps=100, **cfg):

@@ -1147,6 +1147,82 @@ static void k_activend(int nnd, int ibc, const int32_t* indstatus,
 /* ===================================================================== */
 
 /* geometry_dr_getgeometry, geometry_dr_volume.f90:40-96 (single material) */
+/* ---- mixed cells: component gathers and averages ------------------------
+ * The flattened lists (mix_dr_flatten) keep each mixed cell's components
+ * contiguous from imxfcp(imx), imxncp(imx) of them; every loop below walks
+ * them in that order as the reference does. */
+
+/* utils_kn_mxgather1/4, utils_kn_gather.f90:53-106: element value to each of
+ * the cell's components */
+static void k_mxgather(int nmx, const int32_t* imxel, const int32_t* imxfcp,
+                       const int32_t* imxncp, int narr, const double** el,
+                       double** mx) {
+  for (int imx = 1; imx <= nmx; ++imx) {
+    int iel = EL(imxel, imx), icp = EL(imxfcp, imx);
+    for (int ii = 1; ii <= EL(imxncp, imx); ++ii) {
+      for (int k = 0; k < narr; ++k) EL(mx[k], icp) = EL(el[k], iel);
+      icp = icp + 1;
+    }
+  }
+}
+
+/* utils_kn_mxgathercnav, utils_kn_gather.f90:130-152:
+ * mx(:,icp) = frac(icp)*el(:,iel) */
+static void k_mxgathercnav(int nmx, const int32_t* imxel,
+                           const int32_t* imxfcp, const int32_t* imxncp,
+                           const double* el, const double* frac, double* mx) {
+  for (int imx = 1; imx <= nmx; ++imx) {
+    int iel = EL(imxel, imx), icp = EL(imxfcp, imx);
+    double w1[4];
+    for (int j = 1; j <= 4; ++j) w1[j - 1] = CN(el, j, iel);
+    for (int ii = 1; ii <= EL(imxncp, imx); ++ii) {
+      for (int j = 1; j <= 4; ++j) CN(mx, j, icp) = EL(frac, icp) * w1[j - 1];
+      icp = icp + 1;
+    }
+  }
+}
+
+/* utils_kn_average12, utils_kn_average.f90:84-105:
+ * el(iel) = sum frac*(mx1+mx2) over the cell's components */
+static void k_average12(int nmx, const int32_t* imxel, const int32_t* imxfcp,
+                        const int32_t* imxncp, const double* frac,
+                        const double* mx1, const double* mx2, double* el) {
+  for (int imx = 1; imx <= nmx; ++imx) {
+    double w1 = 0.0;
+    int icp = EL(imxfcp, imx);
+    for (int ii = 1; ii <= EL(imxncp, imx); ++ii) {
+      w1 = w1 + EL(frac, icp) * (EL(mx1, icp) + EL(mx2, icp));
+      icp = icp + 1;
+    }
+    EL(el, EL(imxel, imx)) = w1;
+  }
+}
+
+/* utils_kn_average22, utils_kn_average.f90:53-82: corner arrays */
+static void k_average22(int nmx, const int32_t* imxel, const int32_t* imxfcp,
+                        const int32_t* imxncp, const double* frac,
+                        const double* mx1, const double* mx2, double* el1,
+                        double* el2) {
+  for (int imx = 1; imx <= nmx; ++imx) {
+    double w1[4] = {0.0, 0.0, 0.0, 0.0}, w2[4] = {0.0, 0.0, 0.0, 0.0};
+    int icp = EL(imxfcp, imx);
+    for (int ii = 1; ii <= EL(imxncp, imx); ++ii) {
+      for (int j = 1; j <= 4; ++j) {
+        w1[j - 1] = w1[j - 1] + EL(frac, icp) * CN(mx1, j, icp);
+        w2[j - 1] = w2[j - 1] + EL(frac, icp) * CN(mx2, j, icp);
+      }
+      icp = icp + 1;
+    }
+    int iel = EL(imxel, imx);
+    for (int j = 1; j <= 4; ++j) {
+      CN(el1, j, iel) = w1[j - 1];
+      CN(el2, j, iel) = w2[j - 1];
+    }
+  }
+}
+
+#define MXLISTS I(BL_IMXEL), I(BL_IMXFCP), I(BL_IMXNCP)
+
 int orc_geometry_getgeometry(orc_state_t* s, bl_error_t* err) {
   int nel = s->sz.nel;
   k_cngather(nel, I(BL_IELND), R(BL_NDX), R(BL_CNX));
@@ -1154,6 +1230,16 @@ int orc_geometry_getgeometry(orc_state_t* s, bl_error_t* err) {
   k_getiso(nel, R(BL_CNX), R(BL_CNY), R(BL_A1), R(BL_A2), R(BL_A3), R(BL_B1),
            R(BL_B2), R(BL_B3), R(BL_CNWT));
   k_getvolume(nel, R(BL_A1), R(BL_A3), R(BL_B1), R(BL_B3), R(BL_ELVOLUME));
+  int ncp = s->sz.ncp;
+  if (ncp > 0) { /* geometry_dr_volume.f90:66-79 */
+    const double* el[4] = {R(BL_A1), R(BL_A3), R(BL_B1), R(BL_B3)};
+    double* mx[4] = {R(BL_CPA1), R(BL_CPA3), R(BL_CPB1), R(BL_CPB3)};
+    k_mxgather(s->sz.nmx, MXLISTS, 4, el, mx);
+    k_getvolume(ncp, R(BL_CPA1), R(BL_CPA3), R(BL_CPB1), R(BL_CPB3),
+                R(BL_CPVOLUME));
+    for (int icp = 1; icp <= ncp; ++icp) /* utils_kn_multiply */
+      EL(R(BL_CPVOLUME), icp) = EL(R(BL_CPVOLUME), icp) * EL(R(BL_FRVOLUME), icp);
+  }
   int ierr = k_checkvolume(nel, 0.0, R(BL_ELVOLUME));
   if (ierr != BL_SUCCESS) {
     set_err(err, ierr, BL_HALT_SINGLE,
@@ -1167,6 +1253,8 @@ int orc_geometry_getgeometry(orc_state_t* s, bl_error_t* err) {
 /* density_dr_getdensity, density_dr_getdensity.f90:32-46 */
 int orc_density_getdensity(orc_state_t* s) {
   k_divide(s->sz.nel, R(BL_ELMASS), R(BL_ELVOLUME), R(BL_ELDENSITY));
+  if (s->sz.ncp > 0)
+    k_divide(s->sz.ncp, R(BL_CPMASS), R(BL_CPVOLUME), R(BL_CPDENSITY));
   return BL_SUCCESS;
 }
 
@@ -1257,6 +1345,15 @@ int orc_init_state(orc_state_t* s) {
     EL(m, iel) = EL(den, iel) * EL(vol, iel);
     for (int j = 1; j <= 4; ++j) CN(cm, j, iel) = EL(den, iel) * CN(cnwt, j, iel);
   }
+  int ncp = s->sz.ncp;
+  if (ncp > 0) { /* init_dr_mass, init_dr_utils.f90:121-130 */
+    for (int icp = 1; icp <= ncp; ++icp) /* init_kn_mxmass */
+      EL(R(BL_CPMASS), icp) = EL(R(BL_CPDENSITY), icp) * EL(R(BL_CPVOLUME), icp);
+    const double* el[1] = {m};
+    double* mx[1] = {R(BL_RCPSCRATCH11)};
+    k_mxgather(s->sz.nmx, MXLISTS, 1, el, mx);
+    k_divide(ncp, R(BL_CPMASS), R(BL_RCPSCRATCH11), R(BL_FRMASS));
+  }
   return orc_hydro_geteos(s);
 }
 
@@ -1268,6 +1365,13 @@ int orc_hydro_init(orc_state_t* s) {
     for (int j = 1; j <= 4; ++j) {
       CN(R(BL_CNVISCX), j, iel) = 0.0;
       CN(R(BL_CNVISCY), j, iel) = 0.0;
+    }
+  }
+  for (int icp = 1; icp <= s->sz.ncp; ++icp) { /* hydro_dr_init.f90:48-51 */
+    EL(R(BL_CPVISC), icp) = 0.0;
+    for (int j = 1; j <= 4; ++j) {
+      CN(R(BL_CPVISCX), j, icp) = 0.0;
+      CN(R(BL_CPVISCY), j, icp) = 0.0;
     }
   }
   if (s->cfg.zsp)
@@ -1298,6 +1402,9 @@ int orc_hydro_getdt(orc_state_t* s, bl_dt_t* cfl, bl_dt_t* div,
   k_geteviscosity(nel, c->zerocut, R(BL_CNX), R(BL_CNY), cnu, cnv,
                   R(BL_CNVISCX), R(BL_CNVISCY));
   k_sum(nel, R(BL_ELVISC), R(BL_ELCS2));
+  if (s->sz.ncp > 0) /* hydro_dr_getcs2.f90:40-42 */
+    k_average12(s->sz.nmx, MXLISTS, R(BL_FRMASS), R(BL_CPCS2), R(BL_CPVISC),
+                R(BL_ELCS2));
   double rdt = 0.0;
   int idt = 0;
   int ierr = k_getdtcfl(s, nel, c->zcut, c->cfl_sf, I(BL_IELREG), R(BL_ELCS2),
@@ -1325,13 +1432,20 @@ int orc_hydro_setcorrector(orc_state_t* s) {
   return BL_SUCCESS;
 }
 
-/* hydro_dr_getforce, hydro_dr_getforce.f90:40-91 (single material) */
+/* hydro_dr_getforce, hydro_dr_getforce.f90:40-91; component forces cpfx/cpfy
+ * are rcpscratch21/22 (dataAPI_id.f90:94-96) */
 int orc_hydro_getforce(orc_state_t* s, const bl_timestep_t* ts) {
-  int nel = s->sz.nel;
+  int nel = s->sz.nel, ncp = s->sz.ncp;
   double *cnfx = R(BL_RSCRATCH23), *cnfy = R(BL_RSCRATCH24);
   k_getforceP(nel, R(BL_ELPRESSURE), R(BL_A1), R(BL_A3), R(BL_B1), R(BL_B3),
               cnfx, cnfy);
+  if (ncp > 0)
+    k_getforceP(ncp, R(BL_CPPRESSURE), R(BL_CPA1), R(BL_CPA3), R(BL_CPB1),
+                R(BL_CPB3), R(BL_RCPSCRATCH21), R(BL_RCPSCRATCH22));
   k_getforceQ(nel, R(BL_CNVISCX), R(BL_CNVISCY), cnfx, cnfy);
+  if (ncp > 0)
+    k_getforceQ(ncp, R(BL_CPVISCX), R(BL_CPVISCY), R(BL_RCPSCRATCH21),
+                R(BL_RCPSCRATCH22));
   if (s->cfg.zsp)
     k_getforceSP(s, nel, I(BL_IELREG), R(BL_ELDENSITY), R(BL_ELCS2), R(BL_CNX),
                  R(BL_CNY), R(BL_SPMASS), cnfx, cnfy);
@@ -1356,6 +1470,18 @@ int orc_hydro_getenergy(orc_state_t* s, const bl_timestep_t* ts) {
   k_getenergy(s->sz.nel, ts->dts, s->cfg.zerocut, R(BL_ELENERGY), R(BL_ELMASS),
               R(BL_RSCRATCH23), R(BL_RSCRATCH24), R(BL_RSCRATCH21),
               R(BL_RSCRATCH22));
+  if (s->sz.ncp > 0) {
+    /* hydro_dr_getenergy.f90:47-62: the component corner velocities cpu/cpv
+     * are rscratch23/24, the same memory as the element corner forces
+     * (dataAPI_id.f90:94-96), written after the element update has read them */
+    k_mxgathercnav(s->sz.nmx, MXLISTS, R(BL_RSCRATCH21), R(BL_FRVOLUME),
+                   R(BL_RSCRATCH23));
+    k_mxgathercnav(s->sz.nmx, MXLISTS, R(BL_RSCRATCH22), R(BL_FRVOLUME),
+                   R(BL_RSCRATCH24));
+    k_getenergy(s->sz.ncp, ts->dts, s->cfg.zerocut, R(BL_CPENERGY),
+                R(BL_CPMASS), R(BL_RCPSCRATCH21), R(BL_RCPSCRATCH22),
+                R(BL_RSCRATCH23), R(BL_RSCRATCH24));
+  }
   return BL_SUCCESS;
 }
 
@@ -1363,12 +1489,20 @@ int orc_hydro_getenergy(orc_state_t* s, const bl_timestep_t* ts) {
 int orc_hydro_geteos(orc_state_t* s) {
   k_geteos(s, s->sz.nel, I(BL_IELMAT), R(BL_ELDENSITY), R(BL_ELENERGY),
            R(BL_ELPRESSURE), R(BL_ELCS2));
+  if (s->sz.ncp > 0)
+    k_geteos(s, s->sz.ncp, I(BL_ICPMAT), R(BL_CPDENSITY), R(BL_CPENERGY),
+             R(BL_CPPRESSURE), R(BL_CPCS2));
   return BL_SUCCESS;
 }
 
 /* hydro_dr_getacceleration.f90:44-112 */
 int orc_hydro_initacceleration(orc_state_t* s) {
-  (void)s; /* single material, nproc=1: nothing to do */
+  /* mixed cells: the element corner forces become the volume-fraction
+   * average of the component forces (hydro_dr_getacceleration.f90:69-72);
+   * nproc=1: no exchange */
+  if (s->sz.ncp > 0)
+    k_average22(s->sz.nmx, MXLISTS, R(BL_FRVOLUME), R(BL_RCPSCRATCH21),
+                R(BL_RCPSCRATCH22), R(BL_RSCRATCH23), R(BL_RSCRATCH24));
   return BL_SUCCESS;
 }
 int orc_hydro_scatteracceleration(orc_state_t* s) {

@@ -5,7 +5,10 @@ TEST INFRASTRUCTURE ONLY.  Compiles the serial (no MPI, no OpenMP, local
 mesh set-up) source set of the reference *where it lies* under
 /root/reference/src with AMD flang, in module-dependency order, and links it
 with oracle/ref/ref_driver.f90 into oracle/_ref/bookleaf_ref.  Nothing is
-copied out of the reference tree; outputs go only to oracle/_ref/.
+copied out of the reference tree; outputs go only to oracle/_ref/.  (The
+reference's own deck reader cannot be used as shipped: under flang it aborts
+at setup_IC.f90:1282, where a PACK result is assigned to a longer character
+array; problems therefore come in as binary files.)
 
 The source set follows the reference's object lists for
 ``NO_MPI=1 NO_OMP=1 LSETUP=1`` (src/utilities/makefile.UTILITIES:41-49,
@@ -116,22 +119,26 @@ def main():
     moddir = os.path.join(OUT, "mod")
     os.makedirs(objdir, exist_ok=True)
     os.makedirs(moddir, exist_ok=True)
-    exe = os.path.join(OUT, "bookleaf_ref")
     files = order(reference_sources())
-    driver = os.path.join(HERE, "ref", "ref_driver.f90")
-    objs = []
-    newest = 0.0
-    for f in files + [driver]:
+    hosts = {"bookleaf_ref": os.path.join(HERE, "ref", "ref_driver.f90")}
+
+    def compile_one(f):
         base = os.path.splitext(os.path.basename(f))[0]
         obj = os.path.join(objdir, base + ".o")
-        objs.append(obj)
         if force or not os.path.exists(obj) or os.path.getmtime(obj) < os.path.getmtime(f):
             run([FLANG, "-c"] + FFLAGS + ["-module-dir", moddir, "-I", moddir,
                                          f, "-o", obj])
-        newest = max(newest, os.path.getmtime(obj))
-    if force or not os.path.exists(exe) or os.path.getmtime(exe) < newest:
-        run([FLANG, "-O2"] + objs + ["-o", exe])
-    print("built", exe, "from", len(files), "reference sources")
+        return obj
+
+    objs = [compile_one(f) for f in files]
+    newest = max(os.path.getmtime(o) for o in objs)
+    for name, host in hosts.items():
+        hobj = compile_one(host)
+        exe = os.path.join(OUT, name)
+        if force or not os.path.exists(exe) or os.path.getmtime(exe) < max(
+                newest, os.path.getmtime(hobj)):
+            run([FLANG, "-O2"] + objs + [hobj, "-o", exe])
+        print("built", exe, "from", len(files), "reference sources")
     return 0
 
 

@@ -30,7 +30,9 @@ PROGRAM bl_ref_driver
   USE time_cf_mod,             ONLY: time_cf_rationalise
   USE hydro_cf_mod,            ONLY: hydro_cf_rationalise
   USE ale_cf_mod,              ONLY: ale_cf_rationalise
-  USE data_mod,                ONLY: data_set
+  USE eos_cf_mod,              ONLY: eos_cf_rationalise
+  USE data_mod,                ONLY: data_set,data_reset_cpquant,             &
+&                                    data_reset_mxquant
   USE init_kn_mod,             ONLY: init_kn_serialghosts,                     &
 &                                    init_kn_connectivity,init_kn_nodetype
   USE init_dr_utils_mod,       ONLY: init_dr_utils,init_dr_state
@@ -70,7 +72,10 @@ PROGRAM bl_ref_driver
   TYPE(error_t)                            :: error
   INTEGER(KIND=ink) :: nel,nnd,nmat,nreg,nsteps,dump_every,stop_end,zeul,     &
 &                      adv_type,mode,istep,ii,ndump,iu,io
-  INTEGER(KIND=ink) :: zhgflag,zspflag
+  INTEGER(KIND=ink) :: zhgflag,zspflag,ncp,nmx,mcp,mmx
+  INTEGER(KIND=ink),DIMENSION(:),  ALLOCATABLE :: imxel,imxfcp,imxncp,icpmat,  &
+&                                                 icpnext,icpprev
+  REAL(KIND=rlk),   DIMENSION(:),  ALLOCATABLE :: frvol,cpden,cpene
   REAL(KIND=rlk)    :: cut(4),hyd(4),ecut(2),tim(6)
   REAL(KIND=rlk),   DIMENSION(:),  ALLOCATABLE :: kappareg,pmeritreg
   INTEGER(KIND=ink),DIMENSION(:),  ALLOCATABLE :: zdtnotreg,zmidlength,itype
@@ -103,6 +108,17 @@ PROGRAM bl_ref_driver
   ALLOCATE(ielnd(NCORN*nel),ielreg(nel),ielmat(nel),indtype(nnd))
   ALLOCATE(ndx(nnd),ndy(nnd),ndu(nnd),ndv(nnd),den(nel),ene(nel))
   READ(iu) ielnd,ielreg,ielmat,indtype,ndx,ndy,ndu,ndv,den,ene
+  ! mixed cells: the flattened lists of mix_dr_flatten (mix_dr_list.f90)
+  READ(iu,IOSTAT=io) ncp,nmx,mcp,mmx
+  IF (io.NE.0) THEN
+    ncp=0_ink
+    nmx=0_ink
+  ENDIF
+  IF (ncp.GT.0_ink) THEN
+    ALLOCATE(imxel(nmx),imxfcp(nmx),imxncp(nmx),icpmat(ncp),icpnext(ncp),      &
+&            icpprev(ncp),frvol(ncp),cpden(ncp),cpene(ncp))
+    READ(iu) imxel,imxfcp,imxncp,icpmat,icpnext,icpprev,frvol,cpden,cpene
+  ENDIF
   CLOSE(iu)
 
   ! ---- configuration: main + inf_init_default + inf_init_rationalise -----
@@ -151,6 +167,9 @@ PROGRAM bl_ref_driver
   runtime%sizes%nnd=nnd
   runtime%sizes%nmat=nmat
   runtime%sizes%nreg=nreg
+  ! void material 0 (setup_read.f90:458, eos_config.f90:163-226)
+  CALL eos_cf_rationalise(config%eos,runtime%sizes,error)
+  CALL check(error)
   CALL init_kn_serialghosts(runtime%sizes%nel,runtime%sizes%nel1,              &
 &                           runtime%sizes%nel2,runtime%sizes%nnd,              &
 &                           runtime%sizes%nnd1,runtime%sizes%nnd2)
@@ -189,6 +208,34 @@ PROGRAM bl_ref_driver
 &                       dh(ielndid)%iaddr,dh(indtypeid)%iaddr)
   CALL init_dr_utils(config,runtime%sizes,dh,error)
   CALL check(error)
+
+  ! ---- mixed cells: what material_shape + mix_dr_flatten leave
+  ! (setup_IC.f90:1585-1634, mix_dr_list.f90) -------------------------------
+  IF (ncp.GT.0_ink) THEN
+    ! capacities as mix_dr_addel / mix_dr_addcp size them (mix_dr_list.f90)
+    CALL data_reset_mxquant(mmx,runtime%sizes,dh,io)
+    CALL data_reset_cpquant(mcp,runtime%sizes,dh,io)
+    runtime%sizes%nmx=nmx
+    runtime%sizes%ncp=ncp
+    CALL iview(imxelid,ip)
+    ip(1:nmx)=imxel(:)
+    CALL iview(imxfcpid,ip)
+    ip(1:nmx)=imxfcp(:)
+    CALL iview(imxncpid,ip)
+    ip(1:nmx)=imxncp(:)
+    CALL iview(icpmatid,ip)
+    ip(1:ncp)=icpmat(:)
+    CALL iview(icpnextid,ip)
+    ip(1:ncp)=icpnext(:)
+    CALL iview(icpprevid,ip)
+    ip(1:ncp)=icpprev(:)
+    CALL rview(frvolumeid,rp)
+    rp(1:ncp)=frvol(:)
+    CALL rview(cpdensityid,rp)
+    rp(1:ncp)=cpden(:)
+    CALL rview(cpenergyid,rp)
+    rp(1:ncp)=cpene(:)
+  ENDIF
 
   ! ---- initial conditions (the part setup_IC would do) + init_dr_state ---
   CALL rview(nduid,rp)

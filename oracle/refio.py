@@ -48,6 +48,15 @@ def write_problem(p, path, nsteps=None, dump_every=0):
                      (p.ndu, np.float64), (p.ndv, np.float64), (p.density, np.float64),
                      (p.energy, np.float64)):
             f.write(np.asarray(a, t).tobytes())
+        # mixed cells (flattened component lists)
+        sz = p.sizes()
+        f.write(i4(p.ncp, p.nmx, sz.mcp, sz.mmx))
+        if p.ncp:
+            for a, t in ((p.imxel, np.int32), (p.imxfcp, np.int32), (p.imxncp, np.int32),
+                         (p.icpmat, np.int32), (p.icpnext, np.int32), (p.icpprev, np.int32),
+                         (p.frvolume, np.float64), (p.cpdensity, np.float64),
+                         (p.cpenergy, np.float64)):
+                f.write(np.asarray(a, t).tobytes())
 
 
 def read_dumps(path):

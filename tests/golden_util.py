@@ -13,6 +13,8 @@ from bookleaf_f90_amd.problem import Problem
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 INPUTS = ("ielnd", "ndx", "ndy", "indtype", "ielreg", "ielmat", "density", "energy", "ndu",
           "ndv", "eos_itype", "eos_param", "kappareg", "pmeritreg", "zdtnotreg", "zmidlength")
+MIXED = ("imxel", "imxfcp", "imxncp", "icpmat", "icpnext", "icpprev", "frvolume", "cpdensity",
+         "cpenergy")
 
 
 def names():
@@ -23,6 +25,7 @@ def load(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"))  # plain arrays, no pickle
     meta = json.loads(bytes(z["meta"]).decode())
     kw = {k: z["in_" + k].copy() for k in INPUTS}
+    kw.update({k: z["in_" + k].copy() for k in MIXED if "in_" + k in z})
     p = Problem(name=name, nmat=meta["nmat"], nreg=meta["nreg"], config=dict(meta["config"]),
                 nsteps=meta["nsteps"], stop_at_time_end=meta["stop_at_time_end"], **kw)
     return p, meta, z

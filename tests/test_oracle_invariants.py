@@ -13,10 +13,10 @@ restatement mistake (a dropped term, a sign, a wrong index or order).
   - rigid translation leaves rho, e, p unchanged and moves nodes by t*u
   - the node scatter equals a brute-force sum in the reference's (corner,
     element) order, bit for bit (hydro_kn_getacceleration.f90:30-74)
-  - the Sod L1(rho) of the reference's sod_sm_{lag,eul}_small decks at 384
-    steps against the reference's analytic solution
-    (check_kn_tests.f90:33-111, check_dr_tests.f90:56-69), to the printed
-    digits of the reference's own run
+  - the Sod L1(rho) of the reference's sod_sm_{lag,eul}_small and
+    sod_mm_lag_small decks at 384 steps against the reference's analytic
+    solution (check_kn_tests.f90:33-111, check_dr_tests.f90:56-69), to the
+    printed digits of the reference's own runs
 """
 import numpy as np
 import pytest
@@ -234,6 +234,16 @@ def test_sod_deck_l1_matches_reference_run(eulerian, expected, oracle_lib):
     dt, _, _ = o.run(5000, stop_at_time_end=True)
     assert len(dt) == 384
     assert "%.8E" % _sod_l1(o, p) == expected
+
+
+def test_sod_mm_deck_l1_matches_reference_run(oracle_lib):
+    """The sod_mm_lag_small deck (mixed cells, hydro_dr_* component paths):
+    the reference's recorded L1(rho) at 384 steps."""
+    p = P.sod_mm_deck_small()
+    o = _oracle(p)
+    dt, _, _ = o.run(5000, stop_at_time_end=True)
+    assert len(dt) == 384
+    assert "%.8E" % _sod_l1(o, p) == "4.18911702E-02"
 
 
 def test_sod_exact_matches_closed_form_states(oracle_lib):

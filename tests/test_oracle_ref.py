@@ -32,13 +32,18 @@ def test_every_driver_bitexact(name, refio, oracle_lib):
     dumps, _, _ = refio.run_reference(p, mode=1)
     o = Oracle(p)
     assert dumps[0]["label"] == 0
-    assert diff_fields(o.state(), dumps[0]) == []
+    # the component forces (rcpscratch21/22) are unset in the reference until
+    # the first getforce
+    unset = ("rcpscratch21", "rcpscratch22")
+    assert diff_fields(o.state(), dumps[0], ncp=p.ncp, skip=unset) == []
     by_label = {d["label"]: d for d in dumps}
     seq = granular_sequence(o)
     assert sorted(by_label) == [0] + [lab for lab, _ in seq]
     for lab, fn in seq:
         fn()
-        assert diff_fields(o.state(), by_label[lab]) == [], "after driver %d" % lab
+        skip = unset if lab < 201 else ()
+        assert diff_fields(o.state(), by_label[lab], ncp=p.ncp, skip=skip) == [], \
+            "after driver %d" % lab
         assert o.ts.dt == by_label[lab]["dt"] and o.ts.time == by_label[lab]["time"]
 
 
@@ -57,7 +62,7 @@ def test_multistep_dt_sequence_and_state(name, refio, oracle_lib):
     assert np.array_equal(dt, ref_dt)
     assert np.array_equal(idt, ref_idt)
     assert [abi.DT_NAMES[c] for c in ctl] == ref_ctl
-    assert diff_fields(o.state(), dumps[-1]) == []
+    assert diff_fields(o.state(), dumps[-1], ncp=p.ncp) == []
 
 
 def test_cfl_and_div_controls_are_exercised(oracle_lib):
