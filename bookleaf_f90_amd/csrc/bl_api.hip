@@ -616,6 +616,7 @@ int bl_destroy(bl_handle_t* h) {
   for (void* p : bufs) if (p) hipFree(p);
   for (double* p : h->d_tmp) if (p) hipFree(p);
   for (double* p : h->cterm) if (p) hipFree(p);
+  for (double* p : h->ale_tmp) if (p) hipFree(p);
   if (h->h_dtres) hipHostFree(h->h_dtres);
   if (h->h_ts) hipHostFree(h->h_ts);
   for (hipEvent_t ev : h->ev_pool) hipEventDestroy(ev);

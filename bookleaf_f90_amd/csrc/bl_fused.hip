@@ -41,6 +41,7 @@
 #include <cstring>
 
 #include "bl_fused.h"
+#include "bl_ale.h"
 #include "bl_internal.h"
 #include "bl_kernels.h"
 #include "bl_math.h"
@@ -488,17 +489,227 @@ k_f4_state(FusedArgs A, FusedNodes O) {
   A.pre[e] = eos_pressure(et, ep, rho, en, A.pcut);
   A.cs2[e] = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
   if (A.write_geometry) {
-    // the remap that follows reuses rscratch23/24 only in part: it must see
-    // the corrector forces there, as the reference leaves them
-    st4(A.cnfx, e, fx);
-    st4(A.cnfy, e, fy);
-    st4(A.cnx, e, x);
-    st4(A.cny, e, y);
+    // the remap that follows reads the corner weights and the volume at
+    // x^{n+1}; it rewrites the rest of the geometry on the moved mesh
     st4(A.cnwt, e, g.wt);
     A.vol[e] = g.vol;
-    A.a1[e] = g.a1; A.a2[e] = g.a2; A.a3[e] = g.a3;
-    A.b1[e] = g.b1; A.b2[e] = g.b2; A.b3[e] = g.b3;
   }
+}
+
+// ------------------------------------------------------- remap (Eulerian)
+// The unsplit remap of ale_dr_advect (adv_type = 1) with zeul, in five grid
+// kernels.  Intermediates live in the handle's own buffers (ale_tmp) rather
+// than the alestep scratch ids, which nothing reads after the step.  On a
+// conforming mesh the neighbour's flux volume of a shared face is a function
+// of the two shared nodes only, so each element computes it itself with the
+// neighbour's argument order (bit-identical to the neighbour's own value).
+struct AleArgs {
+  int nel, nnd, zeul;
+  double zerocut, dencut, pcut, ccut;
+  const int32_t* ielnd;
+  const int32_t* ielel;
+  const int32_t* ielfc;
+  const int32_t* indtype;
+  const int32_t* ielmat;
+  const int32_t* eos_itype;
+  const double* eos_param;
+  const int32_t* ale_ptr;  // node -> corners, ALE order (ielsort2, corner)
+  const int32_t* ale_idx;
+  const int32_t* el_ptr;   // node -> corners, element order
+  const int32_t* el_idx;
+  double* x;  // node set: x^{n+1}, u^{n+1} in, remapped out
+  double* y;
+  double* u;
+  double* v;
+  const double* ubar;
+  const double* vbar;
+  double *cnwt, *rho, *vol, *mass, *en, *pre, *cs2, *cnmass;
+  double *cnx, *cny, *a1, *a2, *a3, *b1, *b2, *b3;
+  // temporaries: (4, nel) face / corner arrays and (nel) scalars
+  double *fcdm, *flux, *cnu, *cnv, *cnm0, *cndm, *cnflux, *fu, *fv;
+  double *dv, *v0, *m0;
+  DtResult* dtres;
+  const DevTimestep* ts;
+};
+
+#define REMAP_GATE(B) \
+  if ((B).ts->skip || !(B).ts->ale_on) return
+
+// nodes of e at x^{n+1} and after the mesh motion (ale_kn_getmeshvelocity
+// + geometry_kn_getvertex, ale_dr_getfluxvolume.f90:49-59), and the flux
+// volumes of e's faces (own) and of the neighbours' matching faces (nbv)
+__device__ __forceinline__ void remap_faces(const AleArgs& B, const int nd[4], double dt,
+                                            double ax[4], double ay[4], double bx[4],
+                                            double by[4], double own[4], double nbv[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double wu = B.ubar[nd[j]], wv = B.vbar[nd[j]];
+    if (B.zeul) { wu = -wu; wv = -wv; }
+    ax[j] = B.x[nd[j]];
+    ay[j] = B.y[nd[j]];
+    bx[j] = ax[j] + dt * wu;
+    by[j] = ay[j] + dt * wv;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int jp = (j + 1) & 3;
+    own[j] = fluxvolume(ax[j], ay[j], ax[jp], ay[jp], bx[jp], by[jp], bx[j], by[j], B.zerocut);
+    nbv[j] = fluxvolume(ax[jp], ay[jp], ax[j], ay[j], bx[j], by[j], bx[jp], by[jp], B.zerocut);
+  }
+}
+
+// R1: ale_kn_sumflux(fcdv) -> volume change, ale_kn_fluxelvl(cnwt, fcdv,
+// rho) -> face mass fluxes (ale_dr_advect.f90:175-190)
+__global__ void __launch_bounds__(kBlock) k_r1(AleArgs B) {
+  REMAP_GATE(B);
+  const int e = xcd_index();
+  if (e >= B.nel) return;
+  int nd[4];
+  load_nd(B.ielnd, e, nd);
+  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  double ax[4], ay[4], bx[4], by[4], own[4], nbv[4];
+  remap_faces(B, nd, B.ts->dt, ax, ay, bx, by, own, nbv);
+  B.dv[e] = sumflux_regs(e, 0, 1, c.nb, own, nbv);
+  double f[4];
+  fluxelvl_regs(e, 0, 1, c.nb, c.fc, B.cnwt, own, B.rho, f);
+  st4(B.fcdm, e, f);
+}
+
+// R2: ale_kn_sumflux(fcdm) -> mass change, ale_kn_updatebasisel (volume and
+// mass after the remap), ale_kn_fluxelvl(cnmass, fcdm, elenergy) -> face
+// energy fluxes (ale_dr_advect.f90:190-215)
+__global__ void __launch_bounds__(kBlock) k_r2(AleArgs B) {
+  REMAP_GATE(B);
+  const int e = xcd_index();
+  if (e >= B.nel) return;
+  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  const double totm = sumflux_el(e, 0, 1, c.nb, c.fc, B.fcdm);
+  const double v0 = B.vol[e], m0 = B.mass[e];
+  B.v0[e] = v0;
+  B.m0[e] = m0;
+  B.mass[e] = m0 + totm;
+  double f[4];
+  fluxelvl_el(e, 0, 1, c.nb, c.fc, B.cnmass, B.fcdm, B.en, f);
+  st4(B.flux, e, f);
+}
+
+// R3: ale_kn_updateel (energy), the element part of the nodal remap
+// (corner gathers, ale_kn_fluxbasisnd, corner mass update) and
+// ale_dr_hydroupdate (geometry on the moved mesh, density, EOS)
+__global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
+  REMAP_GATE(B);
+  const int e = xcd_index();
+  if (e >= B.nel) return;
+  int nd[4];
+  load_nd(B.ielnd, e, nd);
+  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  // ale_kn_updateel
+  const double s = sumflux_el(e, 0, 1, c.nb, c.fc, B.flux);
+  const double m1 = B.mass[e], v0 = B.v0[e];
+  double en = B.en[e];
+  if (m1 > v0 * B.dencut) en = (en * B.m0[e] + s) / m1;
+  B.en[e] = en;
+  // corner gathers of the node velocities, corner mass before the remap
+  double cu[4], cw[4], cm[4];
+  load4(B.u, nd, cu);
+  load4(B.v, nd, cw);
+  st4(B.cnu, e, cu);
+  st4(B.cnv, e, cw);
+  ld4(B.cnmass, e, cm);
+  st4(B.cnm0, e, cm);
+  // ale_kn_fluxbasisnd, corner part of ale_kn_massbasisnd
+  double ax[4], ay[4], bx[4], by[4], own[4], nbv[4], fm[4];
+  remap_faces(B, nd, B.ts->dt, ax, ay, bx, by, own, nbv);
+  ld4(B.fcdm, e, fm);
+  double cd[4], fl4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i1 = 0; i1 <= 1; ++i1) {
+    const int i2 = i1 + 2;
+    const int e1 = c.nb[i1], e2 = c.nb[i2];
+    double w3 = B.fcdm[4 * e1 + c.fc[i1]], w4 = B.fcdm[4 * e2 + c.fc[i2]];
+    if (e1 == e) w3 = 0.0;
+    if (e2 == e) w4 = 0.0;
+    const double w1 = w3 - fm[i1];
+    const double w2 = w4 - fm[i2];
+    w3 = 0.25 * (w1 - w2);
+    cd[i1] = w3;
+    cd[i2] = w3;
+    w3 = 0.25 * (w1 + w2);
+    fl4[0] = fl4[0] + w3; fl4[1] = fl4[1] + w3; fl4[2] = fl4[2] + w3; fl4[3] = fl4[3] + w3;
+  }
+  st4(B.cndm, e, cd);
+  st4(B.cnflux, e, fl4);
+  const double cm1[4] = {cm[0] + fl4[0], cm[1] + fl4[1], cm[2] + fl4[2], cm[3] + fl4[3]};
+  st4(B.cnmass, e, cm1);
+  // ale_dr_hydroupdate on the moved mesh
+  Iso g;
+  getiso(bx, by, g);
+  st4(B.cnx, e, bx);
+  st4(B.cny, e, by);
+  B.a1[e] = g.a1; B.a2[e] = g.a2; B.a3[e] = g.a3;
+  B.b1[e] = g.b1; B.b2[e] = g.b2; B.b3[e] = g.b3;
+  st4(B.cnwt, e, g.wt);
+  B.vol[e] = g.vol;
+  if (g.vol < 0.0) atomicMax(&B.dtres->negvol, e + 1);
+  const double r = m1 / g.vol;
+  B.rho[e] = r;
+  int mat = B.ielmat[e];
+  mat = mat > 0 ? mat : 0;
+  const int et = B.eos_itype[mat];
+  const double* ep = B.eos_param + BL_NEOSPARAM * mat;
+  B.pre[e] = eos_pressure(et, ep, r, en, B.pcut);
+  B.cs2[e] = eos_cs2(et, ep, r, en, B.pcut, B.ccut);
+}
+
+// R4: ale_kn_fluxndvl for both velocity components (ale_dr_advect.f90:
+// 241-268; the v flux reads the corner v gathered before the u update)
+__global__ void __launch_bounds__(kBlock) k_r4(AleArgs B) {
+  REMAP_GATE(B);
+  const int e = xcd_index();
+  if (e >= B.nel) return;
+  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  double out[4];
+  fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnu, out);
+  st4(B.fu, e, out);
+  fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnv, out);
+  st4(B.fv, e, out);
+}
+
+// R5: node sums (ale_kn_initbasisnd, ale_kn_calcbasisnd, ale_kn_massbasisnd
+// in ALE order, ale_kn_cutbasisnd), ale_kn_activend with indstatus = 2
+// (Eulerian), ale_kn_updatend for u and v (element order), and the moved
+// node positions
+__global__ void __launch_bounds__(kBlock) k_r5(AleArgs B) {
+  REMAP_GATE(B);
+  const int n = xcd_index();
+  if (n >= B.nnd) return;
+  double nv0 = 0.0, nm0 = 0.0;
+  const int b = B.ale_ptr[n], end = B.ale_ptr[n + 1];
+  for (int k = b; k < end; ++k) {
+    const int cc = B.ale_idx[k];
+    nv0 = nv0 + 0.25 * B.v0[cc >> 2];
+    nm0 = nm0 + B.cnm0[cc];
+  }
+  double nm1 = nm0;
+  for (int k = b; k < end; ++k) nm1 = nm1 + B.cnflux[B.ale_idx[k]];
+  const double cutm = B.dencut * nv0;
+  double su = 0.0, sv = 0.0;
+  for (int k = B.el_ptr[n]; k < B.el_ptr[n + 1]; ++k) {
+    const int cc = B.el_idx[k];
+    su = su + B.fu[cc];
+    sv = sv + B.fv[cc];
+  }
+  const int t = B.indtype[n];
+  double u = B.u[n], v = B.v[n];
+  if (t != -1 && t != -3 && nm1 > cutm) u = (u * nm0 + su) / nm1;
+  if (t != -2 && t != -3 && nm1 > cutm) v = (v * nm0 + sv) / nm1;
+  double wu = B.ubar[n], wv = B.vbar[n];
+  if (B.zeul) { wu = -wu; wv = -wv; }
+  const double dt = B.ts->dt;
+  B.x[n] = B.x[n] + dt * wu;
+  B.y[n] = B.y[n] + dt * wv;
+  B.u[n] = u;
+  B.v[n] = v;
 }
 
 // negative volume found by F2 (F3/F4 did not run) or F4 (F3 did): stop the
@@ -577,6 +788,35 @@ FusedArgs make_args(bl_handle_t* h) {
   return A;
 }
 
+AleArgs make_ale(bl_handle_t* h, const FusedNodesOut& O) {
+  AleArgs B{};
+  B.nel = h->sz.nel;
+  B.nnd = h->sz.nnd;
+  B.zeul = h->cfg.zeul;
+  B.zerocut = h->cfg.zerocut; B.dencut = h->cfg.dencut;
+  B.pcut = h->cfg.pcut; B.ccut = h->cfg.ccut;
+  B.ielnd = h->I(BL_IELND); B.ielel = h->I(BL_IELEL); B.ielfc = h->I(BL_IELFC);
+  B.indtype = h->I(BL_INDTYPE); B.ielmat = h->I(BL_IELMAT);
+  B.eos_itype = h->d_eos_itype; B.eos_param = h->d_eos_param;
+  B.ale_ptr = h->d_ale_ptr; B.ale_idx = h->d_ale_idx;
+  B.el_ptr = h->d_el_ptr; B.el_idx = h->d_el_idx;
+  B.x = O.x; B.y = O.y; B.u = O.u; B.v = O.v;
+  B.ubar = h->R(BL_RSCRATCH14); B.vbar = h->R(BL_RSCRATCH15);
+  B.cnwt = h->R(BL_CNWT); B.rho = h->R(BL_ELDENSITY); B.vol = h->R(BL_ELVOLUME);
+  B.mass = h->R(BL_ELMASS); B.en = h->R(BL_ELENERGY); B.pre = h->R(BL_ELPRESSURE);
+  B.cs2 = h->R(BL_ELCS2); B.cnmass = h->R(BL_CNMASS);
+  B.cnx = h->R(BL_CNX); B.cny = h->R(BL_CNY);
+  B.a1 = h->R(BL_A1); B.a2 = h->R(BL_A2); B.a3 = h->R(BL_A3);
+  B.b1 = h->R(BL_B1); B.b2 = h->R(BL_B2); B.b3 = h->R(BL_B3);
+  double** t = h->ale_tmp;
+  B.fcdm = t[0]; B.flux = t[1]; B.cnu = t[2]; B.cnv = t[3]; B.cnm0 = t[4]; B.cndm = t[5];
+  B.cnflux = t[6]; B.fu = t[7]; B.fv = t[8];
+  B.dv = t[9]; B.v0 = t[10]; B.m0 = t[11];
+  B.dtres = h->d_dtres;
+  B.ts = h->d_ts;
+  return B;
+}
+
 // one step (parity p: node set p -> node set 1-p), launched on h->stream
 int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedArgs A = make_args(h);
@@ -596,17 +836,15 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
   if (A.mx.ncp > 0) BL_LAUNCH(h, "f4_mix", k_f4_mix, nblocks(A.mx.nmx), kBlock, A, Oc);
   BL_LAUNCH(h, "f5_end", k_f5_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
-  // mixed cells before the remap: the corrector's cpu/cpv in rscratch23/24
-  if (h->cfg.zexist && mix_cpu_overlay(h, h->d_ts)) return BL_FAILURE;
   if (h->cfg.zexist) {
-    // the ALE kernels address the registry's node fields: point them at the
-    // set this step wrote
-    double* saved[4] = {h->R(BL_NDX), h->R(BL_NDY), h->R(BL_NDU), h->R(BL_NDV)};
-    h->f[BL_NDX] = O.x; h->f[BL_NDY] = O.y; h->f[BL_NDU] = O.u; h->f[BL_NDV] = O.v;
-    int rc = ale_step_device(h, h->d_ts);
-    h->f[BL_NDX] = saved[0]; h->f[BL_NDY] = saved[1];
-    h->f[BL_NDU] = saved[2]; h->f[BL_NDV] = saved[3];
-    if (rc) return BL_FAILURE;
+    // the remap on the node set this step wrote
+    AleArgs B = make_ale(h, O);
+    BL_LAUNCH(h, "r1_volflux", k_r1, nbe, kBlock, B);
+    BL_LAUNCH(h, "r2_massflux", k_r2, nbe, kBlock, B);
+    BL_LAUNCH(h, "r3_energy_update", k_r3, nbe, kBlock, B);
+    BL_LAUNCH(h, "r4_momflux", k_r4, nbe, kBlock, B);
+    BL_LAUNCH(h, "r5_nodes", k_r5, nbn, kBlock, B);
+    if (mix_hydroupdate(h, h->d_ts)) return BL_FAILURE;
     BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
   }
   return BL_SUCCESS;
@@ -616,6 +854,10 @@ int ensure_nodesets(bl_handle_t* h) {
   if (!h->cterm[0])
     for (int k = 0; k < 4; ++k)
       BL_HIP(hipMalloc(&h->cterm[k], sizeof(double) * 4 * std::max(h->sz.nel, 1)));
+  if (h->cfg.zexist && !h->ale_tmp[0])
+    for (int k = 0; k < 12; ++k)
+      BL_HIP(hipMalloc(&h->ale_tmp[k], sizeof(double) * (k < 9 ? 4 : 1) *
+                                           std::max(h->sz.nel, 1)));
   if (h->nodeset[1][0]) return BL_SUCCESS;
   const int ids[4] = {BL_NDX, BL_NDY, BL_NDU, BL_NDV};
   for (int k = 0; k < 4; ++k) {
@@ -636,7 +878,6 @@ void point_fields_at(bl_handle_t* h, int s) {
 
 int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
   invalidate_graph(h);
-  if (h->cfg.zexist && !tmp_buffer(h, 0)) return BL_FAILURE;  // no allocation in a capture
   bool timing = h->timing;
   h->timing = false;  // no event records inside a capture
   for (int p = 0; p < 2; ++p) {
@@ -721,8 +962,9 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     set_error("bl_run_graph: mesh maps, regions or EOS not set");
     return BL_FAILURE;
   }
-  // split advection, or a mesh without the limiter stencil table: host-driven
-  if ((h->cfg.zexist && h->cfg.adv_type != 1) || !h->d_outer)
+  // split advection, ALE other than Eulerian, or a mesh without the limiter
+  // stencil table: host-driven
+  if ((h->cfg.zexist && (h->cfg.adv_type != 1 || !h->cfg.zeul)) || !h->d_outer)
     return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
                   err);
   fill_err(err, BL_SUCCESS, 0, "");
@@ -760,14 +1002,14 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   // the node set holding the latest state: every step whose F3 ran wrote the
   // other set
   point_fields_at(h, (h->cur_set + d->flips) & 1);
-  if (!h->cfg.zexist) {
+  {
     int nel = h->sz.nel;
     BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
               h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
               h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
               h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
               h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
-    if (mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
+    if (!h->cfg.zexist && mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
   }
   if (taken > 0) {
     std::vector<double> hist(3 * (size_t)taken);

@@ -112,6 +112,9 @@ struct Handle {
   // corrector corner terms of the fused step, corner-major (4*nel each):
   // force x, force y, half-step corner weight, nodal-mass term
   double* cterm[4] = {};
+  // remap temporaries of the fused step (zexist): 4*nel each for the face
+  // and corner arrays, nel each for the element scalars
+  double* ale_tmp[12] = {};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
   int graph_stop_at_end = -1;
   // timing

@@ -79,6 +79,9 @@ __device__ __forceinline__ void ld4c(const double* a, int icp, double v[4]) {
 int mix_geometry(Handle* h, const DevTimestep* ts);
 int mix_density(Handle* h, const DevTimestep* ts);
 int mix_eos(Handle* h, const DevTimestep* ts);
+// geometry + density + EOS of the components in one pass (the component part
+// of ale_dr_hydroupdate), gated by ts like the remap
+int mix_hydroupdate(Handle* h, const DevTimestep* ts);
 int mix_initmass(Handle* h);
 int mix_viscinit(Handle* h);
 int mix_force(Handle* h);

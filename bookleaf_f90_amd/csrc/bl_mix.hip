@@ -33,6 +33,37 @@ __global__ void k_mx_geometry(MixArgs M, const double* __restrict__ a1,
   }
 }
 
+// the component part of ale_dr_hydroupdate (ale_dr_update.f90:33-51):
+// geometry_dr_volume.f90:66-79, density_dr_getdensity.f90:40-43 and
+// eos_dr_geteos.f90:50-55 per mixed cell
+__global__ void k_mx_hydroupdate(MixArgs M, const double* __restrict__ a1,
+                                 const double* __restrict__ a3, const double* __restrict__ b1,
+                                 const double* __restrict__ b3,
+                                 const int32_t* __restrict__ eos_itype,
+                                 const double* __restrict__ eos_param, double pcut, double ccut,
+                                 const DevTimestep* ts) {
+  MIX_GATE(ts);
+  const int imx = xcd_index();
+  if (imx >= M.nmx) return;
+  const int iel = M.imxel[imx] - 1;
+  const double w1 = a1[iel], w2 = a3[iel], w3 = b1[iel], w4 = b3[iel];
+  int icp = M.imxfcp[imx] - 1;
+  for (int ii = 0; ii < M.imxncp[imx]; ++ii, ++icp) {
+    M.cpa1[icp] = w1; M.cpa3[icp] = w2; M.cpb1[icp] = w3; M.cpb3[icp] = w4;
+    const double cvol = (4.0 * (w1 * w4 - w2 * w3)) * M.frvolume[icp];
+    M.cpvolume[icp] = cvol;
+    const double r = M.cpmass[icp] / cvol;
+    M.cpdensity[icp] = r;
+    int mat = M.icpmat[icp];
+    mat = mat > 0 ? mat : 0;
+    const int et = eos_itype[mat];
+    const double* ep = eos_param + BL_NEOSPARAM * mat;
+    const double e = M.cpenergy[icp];
+    M.cppressure[icp] = eos_pressure(et, ep, r, e, pcut);
+    M.cpcs2[icp] = eos_cs2(et, ep, r, e, pcut, ccut);
+  }
+}
+
 // density_dr_getdensity.f90:40-43
 __global__ void k_mx_density(MixArgs M, const DevTimestep* ts) {
   MIX_GATE(ts);
@@ -199,6 +230,16 @@ int mix_density(Handle* h, const DevTimestep* ts) {
 int mix_eos(Handle* h, const DevTimestep* ts) {
   if (h->sz.ncp <= 0) return BL_SUCCESS;
   BL_LAUNCH(h, "mx_eos", k_mx_eos, nblocks(h->sz.ncp), kBlock, make_mix(h),
+            (const int32_t*)h->d_eos_itype, (const double*)h->d_eos_param, h->cfg.pcut,
+            h->cfg.ccut, ts);
+  return BL_SUCCESS;
+}
+
+int mix_hydroupdate(Handle* h, const DevTimestep* ts) {
+  if (h->sz.ncp <= 0) return BL_SUCCESS;
+  BL_LAUNCH(h, "mx_hydroupdate", k_mx_hydroupdate, nblocks(h->sz.nmx), kBlock, make_mix(h),
+            (const double*)h->R(BL_A1), (const double*)h->R(BL_A3),
+            (const double*)h->R(BL_B1), (const double*)h->R(BL_B3),
             (const int32_t*)h->d_eos_itype, (const double*)h->d_eos_param, h->cfg.pcut,
             h->cfg.ccut, ts);
   return BL_SUCCESS;

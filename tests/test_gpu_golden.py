@@ -6,7 +6,7 @@ import pytest
 
 from bookleaf_f90_amd import abi
 from tests import golden_util as G
-from tests.test_gpu_graph import LAG_SCRATCH
+from tests.test_gpu_graph import excluded
 
 pytestmark = pytest.mark.gpu
 
@@ -25,5 +25,4 @@ def test_graph_runner_matches_reference_record(name):
     s = g.state()
     for k in ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv"):
         assert np.array_equal(s[k][:len(z["final_" + k])], z["final_" + k]), k
-    skip = () if p.config.get("zexist") else LAG_SCRATCH
-    assert G.digest_mismatches(s, meta["final"], skip=skip) == []
+    assert G.digest_mismatches(s, meta["final"], skip=excluded(p)) == []
