@@ -525,9 +525,11 @@ struct AleArgs {
   const double* vbar;
   double *cnwt, *rho, *vol, *mass, *en, *pre, *cs2, *cnmass;
   double *cnx, *cny, *a1, *a2, *a3, *b1, *b2, *b3;
-  // temporaries: (4, nel) face / corner arrays and (nel) scalars
+  // temporaries: (4, nel) face / corner arrays and (nel) scalars; the
+  // arrays the node pass gathers (cnm0T, cnflux, fu, fv) are corner-major,
+  // X[j*nel + e], so one corner slot of a wave's nodes is contiguous
   double *fcdm, *flux, *cnu, *cnv, *cnm0, *cndm, *cnflux, *fu, *fv;
-  double *dv, *v0, *m0;
+  double *dv, *v0, *m0, *cnm0T;
   DtResult* dtres;
   const DevTimestep* ts;
 };
@@ -638,18 +640,19 @@ __global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
     fl4[0] = fl4[0] + w3; fl4[1] = fl4[1] + w3; fl4[2] = fl4[2] + w3; fl4[3] = fl4[3] + w3;
   }
   st4(B.cndm, e, cd);
-  st4(B.cnflux, e, fl4);
+  const int nel = B.nel;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    B.cnflux[j * nel + e] = fl4[j];
+    B.cnm0T[j * nel + e] = cm[j];
+  }
   const double cm1[4] = {cm[0] + fl4[0], cm[1] + fl4[1], cm[2] + fl4[2], cm[3] + fl4[3]};
   st4(B.cnmass, e, cm1);
-  // ale_dr_hydroupdate on the moved mesh
+  // ale_dr_hydroupdate on the moved mesh: density and EOS; the geometry
+  // fields are rebuilt from the final nodes at the end of the batch (nothing
+  // in a step reads them before F4 rewrites the ones the remap needs)
   Iso g;
   getiso(bx, by, g);
-  st4(B.cnx, e, bx);
-  st4(B.cny, e, by);
-  B.a1[e] = g.a1; B.a2[e] = g.a2; B.a3[e] = g.a3;
-  B.b1[e] = g.b1; B.b2[e] = g.b2; B.b3[e] = g.b3;
-  st4(B.cnwt, e, g.wt);
-  B.vol[e] = g.vol;
   if (g.vol < 0.0) atomicMax(&B.dtres->negvol, e + 1);
   const double r = m1 / g.vol;
   B.rho[e] = r;
@@ -669,10 +672,13 @@ __global__ void __launch_bounds__(kBlock) k_r4(AleArgs B) {
   if (e >= B.nel) return;
   const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
   double out[4];
+  const int nel = B.nel;
   fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnu, out);
-  st4(B.fu, e, out);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) B.fu[j * nel + e] = out[j];
   fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnv, out);
-  st4(B.fv, e, out);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) B.fv[j * nel + e] = out[j];
 }
 
 // R5: node sums (ale_kn_initbasisnd, ale_kn_calcbasisnd, ale_kn_massbasisnd
@@ -683,21 +689,26 @@ __global__ void __launch_bounds__(kBlock) k_r5(AleArgs B) {
   REMAP_GATE(B);
   const int n = xcd_index();
   if (n >= B.nnd) return;
+  const int nel = B.nel;
   double nv0 = 0.0, nm0 = 0.0;
   const int b = B.ale_ptr[n], end = B.ale_ptr[n + 1];
   for (int k = b; k < end; ++k) {
     const int cc = B.ale_idx[k];
     nv0 = nv0 + 0.25 * B.v0[cc >> 2];
-    nm0 = nm0 + B.cnm0[cc];
+    nm0 = nm0 + B.cnm0T[(cc & 3) * nel + (cc >> 2)];
   }
   double nm1 = nm0;
-  for (int k = b; k < end; ++k) nm1 = nm1 + B.cnflux[B.ale_idx[k]];
+  for (int k = b; k < end; ++k) {
+    const int cc = B.ale_idx[k];
+    nm1 = nm1 + B.cnflux[(cc & 3) * nel + (cc >> 2)];
+  }
   const double cutm = B.dencut * nv0;
   double su = 0.0, sv = 0.0;
   for (int k = B.el_ptr[n]; k < B.el_ptr[n + 1]; ++k) {
     const int cc = B.el_idx[k];
-    su = su + B.fu[cc];
-    sv = sv + B.fv[cc];
+    const int off = (cc & 3) * nel + (cc >> 2);
+    su = su + B.fu[off];
+    sv = sv + B.fv[off];
   }
   const int t = B.indtype[n];
   double u = B.u[n], v = B.v[n];
@@ -811,7 +822,7 @@ AleArgs make_ale(bl_handle_t* h, const FusedNodesOut& O) {
   double** t = h->ale_tmp;
   B.fcdm = t[0]; B.flux = t[1]; B.cnu = t[2]; B.cnv = t[3]; B.cnm0 = t[4]; B.cndm = t[5];
   B.cnflux = t[6]; B.fu = t[7]; B.fv = t[8];
-  B.dv = t[9]; B.v0 = t[10]; B.m0 = t[11];
+  B.dv = t[9]; B.v0 = t[10]; B.m0 = t[11]; B.cnm0T = t[12];
   B.dtres = h->d_dtres;
   B.ts = h->d_ts;
   return B;
@@ -844,7 +855,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     BL_LAUNCH(h, "r3_energy_update", k_r3, nbe, kBlock, B);
     BL_LAUNCH(h, "r4_momflux", k_r4, nbe, kBlock, B);
     BL_LAUNCH(h, "r5_nodes", k_r5, nbn, kBlock, B);
-    if (mix_hydroupdate(h, h->d_ts)) return BL_FAILURE;
+    if (mix_hydroupdate(h, O.x, O.y, h->d_ts)) return BL_FAILURE;
     BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
   }
   return BL_SUCCESS;
@@ -855,8 +866,8 @@ int ensure_nodesets(bl_handle_t* h) {
     for (int k = 0; k < 4; ++k)
       BL_HIP(hipMalloc(&h->cterm[k], sizeof(double) * 4 * std::max(h->sz.nel, 1)));
   if (h->cfg.zexist && !h->ale_tmp[0])
-    for (int k = 0; k < 12; ++k)
-      BL_HIP(hipMalloc(&h->ale_tmp[k], sizeof(double) * (k < 9 ? 4 : 1) *
+    for (int k = 0; k < 13; ++k)
+      BL_HIP(hipMalloc(&h->ale_tmp[k], sizeof(double) * ((k < 9 || k == 12) ? 4 : 1) *
                                            std::max(h->sz.nel, 1)));
   if (h->nodeset[1][0]) return BL_SUCCESS;
   const int ids[4] = {BL_NDX, BL_NDY, BL_NDU, BL_NDV};

@@ -114,7 +114,7 @@ struct Handle {
   double* cterm[4] = {};
   // remap temporaries of the fused step (zexist): 4*nel each for the face
   // and corner arrays, nel each for the element scalars
-  double* ale_tmp[12] = {};
+  double* ale_tmp[13] = {};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
   int graph_stop_at_end = -1;
   // timing

@@ -80,8 +80,9 @@ int mix_geometry(Handle* h, const DevTimestep* ts);
 int mix_density(Handle* h, const DevTimestep* ts);
 int mix_eos(Handle* h, const DevTimestep* ts);
 // geometry + density + EOS of the components in one pass (the component part
-// of ale_dr_hydroupdate), gated by ts like the remap
-int mix_hydroupdate(Handle* h, const DevTimestep* ts);
+// of ale_dr_hydroupdate) from the cell geometry of the node positions x, y;
+// gated by ts like the remap
+int mix_hydroupdate(Handle* h, const double* x, const double* y, const DevTimestep* ts);
 int mix_initmass(Handle* h);
 int mix_viscinit(Handle* h);
 int mix_force(Handle* h);

@@ -36,9 +36,8 @@ __global__ void k_mx_geometry(MixArgs M, const double* __restrict__ a1,
 // the component part of ale_dr_hydroupdate (ale_dr_update.f90:33-51):
 // geometry_dr_volume.f90:66-79, density_dr_getdensity.f90:40-43 and
 // eos_dr_geteos.f90:50-55 per mixed cell
-__global__ void k_mx_hydroupdate(MixArgs M, const double* __restrict__ a1,
-                                 const double* __restrict__ a3, const double* __restrict__ b1,
-                                 const double* __restrict__ b3,
+__global__ void k_mx_hydroupdate(MixArgs M, const int32_t* __restrict__ ielnd,
+                                 const double* __restrict__ ndx, const double* __restrict__ ndy,
                                  const int32_t* __restrict__ eos_itype,
                                  const double* __restrict__ eos_param, double pcut, double ccut,
                                  const DevTimestep* ts) {
@@ -46,7 +45,12 @@ __global__ void k_mx_hydroupdate(MixArgs M, const double* __restrict__ a1,
   const int imx = xcd_index();
   if (imx >= M.nmx) return;
   const int iel = M.imxel[imx] - 1;
-  const double w1 = a1[iel], w2 = a3[iel], w3 = b1[iel], w4 = b3[iel];
+  const int4 q = reinterpret_cast<const int4*>(ielnd)[iel];
+  const double x[4] = {ndx[q.x - 1], ndx[q.y - 1], ndx[q.z - 1], ndx[q.w - 1]};
+  const double y[4] = {ndy[q.x - 1], ndy[q.y - 1], ndy[q.z - 1], ndy[q.w - 1]};
+  Iso g;
+  getiso(x, y, g);
+  const double w1 = g.a1, w2 = g.a3, w3 = g.b1, w4 = g.b3;
   int icp = M.imxfcp[imx] - 1;
   for (int ii = 0; ii < M.imxncp[imx]; ++ii, ++icp) {
     M.cpa1[icp] = w1; M.cpa3[icp] = w2; M.cpb1[icp] = w3; M.cpb3[icp] = w4;
@@ -235,13 +239,11 @@ int mix_eos(Handle* h, const DevTimestep* ts) {
   return BL_SUCCESS;
 }
 
-int mix_hydroupdate(Handle* h, const DevTimestep* ts) {
+int mix_hydroupdate(Handle* h, const double* x, const double* y, const DevTimestep* ts) {
   if (h->sz.ncp <= 0) return BL_SUCCESS;
   BL_LAUNCH(h, "mx_hydroupdate", k_mx_hydroupdate, nblocks(h->sz.nmx), kBlock, make_mix(h),
-            (const double*)h->R(BL_A1), (const double*)h->R(BL_A3),
-            (const double*)h->R(BL_B1), (const double*)h->R(BL_B3),
-            (const int32_t*)h->d_eos_itype, (const double*)h->d_eos_param, h->cfg.pcut,
-            h->cfg.ccut, ts);
+            (const int32_t*)h->I(BL_IELND), x, y, (const int32_t*)h->d_eos_itype,
+            (const double*)h->d_eos_param, h->cfg.pcut, h->cfg.ccut, ts);
   return BL_SUCCESS;
 }
 
