@@ -76,9 +76,18 @@ class Error(C.Structure):
         return bytes(self.serr).decode(errors="replace").strip()
 
 
+FLAG_TOTALS = ("volume", "mass", "ie", "ke", "pressure", "pressure_min", "pressure_max",
+               "density_min", "density_max")
+
+
 class Totals(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 "volume mass ie ke pressure_min pressure_max density_min density_max".split()]
+
+
+class FlagTotals(C.Structure):
+    """One row of hydro_dr_shortprint's region or material table."""
+    _fields_ = [(n, C.c_double) for n in FLAG_TOTALS]
 
 
 def field_shape(sizes, fid):
@@ -144,6 +153,10 @@ def declare(lib, prefix):
         "run": [vp, C.c_int32, C.c_int32, P(Timestep), P(C.c_double), P(C.c_int32),
                 P(C.c_int32), P(C.c_int32), P(Error)],
         "totals": [vp, P(Totals)],
+        "shortprint": [vp, P(FlagTotals), P(FlagTotals)],
+        "sod_l1": [vp, C.c_double, C.c_double, C.c_double, P(C.c_double)],
+        "set_diagnostics": [vp, C.c_int32, C.c_double, C.c_double, C.c_int32],
+        "diagnostics": [vp, C.c_int32, P(C.c_double), P(C.c_int32)],
     }
     for name, args in sig.items():
         f = getattr(lib, prefix + "_" + name, None)

@@ -858,7 +858,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     if (mix_hydroupdate(h, O.x, O.y, h->d_ts)) return BL_FAILURE;
     BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
   }
-  return BL_SUCCESS;
+  return launch_diag(h, O.x, O.y, O.u, O.v);
 }
 
 int ensure_nodesets(bl_handle_t* h) {

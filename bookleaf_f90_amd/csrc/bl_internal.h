@@ -116,6 +116,14 @@ struct Handle {
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[13] = {};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
+  // per-step diagnostics of the fused step: region table + Sod L1 records
+  bool diag_on = false;
+  double diag_x0 = 0.0, diag_scale = 1.0;
+  int32_t diag_max = 0;
+  double* d_diag_hist = nullptr;     // diag_max records of 2 + 9*nreg + 2
+  int32_t* d_diag_count = nullptr;   // records written
+  double* d_diag_part = nullptr;     // per-workgroup partials
+  size_t diag_part_n = 0;
   int graph_stop_at_end = -1;
   // timing
   bool timing = false;
@@ -137,6 +145,9 @@ double* tmp_buffer(Handle* h, int k);
 // ALE step launches; with d_ts the kernels read dt, the step gate and
 // ale_on from the device timestep (bl_ale.hip)
 int ale_step_device(Handle* h, const DevTimestep* d_ts);
+// the per-step diagnostics record from the node set a step wrote
+int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* ndu,
+                const double* ndv);
 // drop the captured step graphs (config or maps changed)
 void invalidate_graph(Handle* h);
 

@@ -149,6 +149,15 @@ int bl_run(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep
   for (int k = 0; k < nsteps; ++k) {
     if (stop_at_time_end && ts->time >= h->cfg.time_end) break;
     rc = host_step(h, ts, err);
+    if (rc == BL_SUCCESS && h->diag_on) {
+      // the record of this step, as the graph runner writes it
+      DevTimestep* d = h->h_ts;
+      std::memset(d, 0, sizeof(DevTimestep));
+      d->time = ts->time; d->dt = ts->dt; d->nstep = ts->nstep;
+      BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
+      BL_HIP(hipStreamSynchronize(h->stream));
+      rc = launch_diag(h, h->R(BL_NDX), h->R(BL_NDY), h->R(BL_NDU), h->R(BL_NDV));
+    }
     if (dt_hist) dt_hist[k] = ts->dt;
     if (idt_hist) idt_hist[k] = ts->idtel;
     if (ctl_hist) ctl_hist[k] = ts->sdt;

@@ -251,6 +251,24 @@ class DriverState:
         self._check(self._f("totals")(self.h, C.byref(t)), "totals")
         return t
 
+    def shortprint(self):
+        """hydro_dr_shortprint's tables: (region, material) as float arrays of
+        shape (nreg, 9) and (nmat, 9), columns in abi.FLAG_TOTALS order."""
+        nreg, nmat = self.sizes.nreg, self.sizes.nmat
+        reg = (abi.FlagTotals * max(nreg, 1))()
+        mat = (abi.FlagTotals * max(nmat, 1))()
+        self._check(self._f("shortprint")(self.h, reg, mat), "shortprint")
+        as_np = lambda t, n: np.array([[getattr(t[k], q) for q in abi.FLAG_TOTALS]
+                                       for k in range(n)]).reshape(n, len(abi.FLAG_TOTALS))
+        return as_np(reg, nreg), as_np(mat, nmat)
+
+    def sod_l1(self, time=None, x0=50.0, scale=1.0):
+        """check_kn_sod's (L1 density, L1 energy) at time (default: now)."""
+        out = (C.c_double * 2)()
+        t = self.ts.time if time is None else time
+        self._check(self._f("sod_l1")(self.h, t, x0, scale, out), "sod_l1")
+        return out[0], out[1]
+
     def state(self, names=None):
         names = names or abi.FIELDS
         return {n: self.get(n) for n in names}
@@ -309,6 +327,26 @@ class BookLeaf(DriverState):
 
     def synchronize(self):
         self._check(self.lib.bl_synchronize(self.h), "synchronize")
+
+    def set_diagnostics(self, on=True, x0=50.0, scale=1.0, maxrec=4096):
+        """Record, every step of run/run_graph, nstep, time, the region table
+        and the Sod L1 pair on the device."""
+        self._check(self._f("set_diagnostics")(self.h, int(bool(on)), x0, scale, maxrec),
+                    "set_diagnostics")
+        self._diag_max = maxrec
+
+    def diagnostics(self):
+        """The records since set_diagnostics: dict of nstep (n,), time (n,),
+        region (n, nreg, 9) and l1 (n, 2)."""
+        nreg = self.sizes.nreg
+        rec = 2 + len(abi.FLAG_TOTALS) * nreg + 2
+        cap = getattr(self, "_diag_max", 0)
+        buf = np.zeros(max(cap, 1) * rec)
+        n = C.c_int32(0)
+        self._check(self._f("diagnostics")(self.h, cap, _dptr(buf), C.byref(n)), "diagnostics")
+        r = buf[:n.value * rec].reshape(n.value, rec)
+        return {"nstep": r[:, 0].astype(np.int64), "time": r[:, 1],
+                "region": r[:, 2:2 + 9 * nreg].reshape(n.value, nreg, 9), "l1": r[:, -2:]}
 
     def timing(self, on=True):
         self._check(self.lib.bl_timing_enable(self.h, int(on)), "timing_enable")

@@ -65,6 +65,13 @@ MODULE bookleaf_hip_mod
 &                          density_min,density_max
   END TYPE bl_totals_t
 
+  ! one row of hydro_dr_shortprint's region or material table
+  ! (hydro_kn_print.F90:56-110, hydro_dr_print.f90:49-119)
+  TYPE,BIND(C) :: bl_flag_totals_t
+    REAL(KIND=C_DOUBLE) :: volume,mass,ie,ke,pressure,pressure_min,            &
+&                          pressure_max,density_min,density_max
+  END TYPE bl_flag_totals_t
+
   INTERFACE
 
     ! ---- lifetime and registry -------------------------------------------
@@ -282,6 +289,37 @@ MODULE bookleaf_hip_mod
       TYPE(C_PTR),VALUE              :: h
       TYPE(bl_totals_t),INTENT(OUT)  :: out
     END FUNCTION bl_totals
+    ! region(nreg), material(nmat)
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_shortprint(h,region,material)           &
+&                                    BIND(C,NAME="bl_shortprint")
+      IMPORT :: C_INT32_T,C_PTR,bl_flag_totals_t
+      TYPE(C_PTR),VALUE                               :: h
+      TYPE(bl_flag_totals_t),DIMENSION(*),INTENT(OUT) :: region,material
+    END FUNCTION bl_shortprint
+    ! l1(1) density, l1(2) energy
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_sod_l1(h,time,x0,scale,l1)              &
+&                                    BIND(C,NAME="bl_sod_l1")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE                       :: h
+      REAL(KIND=C_DOUBLE),VALUE               :: time,x0,scale
+      REAL(KIND=C_DOUBLE),DIMENSION(2),INTENT(OUT) :: l1
+    END FUNCTION bl_sod_l1
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_set_diagnostics(h,on,x0,scale,maxrec)   &
+&                                    BIND(C,NAME="bl_set_diagnostics")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE             :: h
+      INTEGER(KIND=C_INT32_T),VALUE :: on,maxrec
+      REAL(KIND=C_DOUBLE),VALUE     :: x0,scale
+    END FUNCTION bl_set_diagnostics
+    ! out(2+9*nreg+2, maxrec)
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_diagnostics(h,maxrec,out,nrec)          &
+&                                    BIND(C,NAME="bl_diagnostics")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE                          :: h
+      INTEGER(KIND=C_INT32_T),VALUE              :: maxrec
+      REAL(KIND=C_DOUBLE),DIMENSION(*),INTENT(OUT) :: out
+      INTEGER(KIND=C_INT32_T),INTENT(OUT)        :: nrec
+    END FUNCTION bl_diagnostics
 
     ! ---- kernel timing and diagnostics -----------------------------------
     INTEGER(KIND=C_INT32_T) FUNCTION bl_timing_enable(h,on) BIND(C,NAME="bl_timing_enable")

@@ -124,6 +124,13 @@ typedef struct bl_totals {
       density_max;
 } bl_totals_t;
 
+/* one row of hydro_kn_calc_shortprint's tables (hydro_kn_print.F90:56-110)
+ * for a region or a material: pressure is the mass-weighted sum */
+typedef struct bl_flag_totals {
+  double volume, mass, ie, ke, pressure, pressure_min, pressure_max,
+      density_min, density_max;
+} bl_flag_totals_t;
+
 typedef struct bl_handle bl_handle_t;
 
 /* ---- lifetime (data.f90:129-167 data_set / data_setinit) -------------- */
@@ -236,6 +243,27 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end,
  * volume, mass of elements with m > dencut*V, IE = sum m*e,
  * KE = sum_c 0.5*rho*cnwt_c*|u_c|^2 with u_c gathered from ndu/ndv. */
 int bl_totals(bl_handle_t* h, bl_totals_t* out);
+
+/* hydro_dr_shortprint's tables (hydro_dr_print.f90:49-119): region[nreg]
+ * over elements by ielreg, material[nmat] over pure elements by ielmat plus
+ * the components of mixed cells by icpmat (component corners take the
+ * cell's cnwt and corner velocities, utils_kn_mxgathercncp).  Sums use a
+ * fixed two-level tree (per workgroup, then over workgroups in order), so
+ * results are reproducible run to run; they differ from the serial sums in
+ * the last bits. */
+int bl_shortprint(bl_handle_t* h, bl_flag_totals_t* region, bl_flag_totals_t* material);
+/* check_kn_sod (check_kn_tests.f90:33-62, check_dr_tests.f90:56-69): volume
+ * weighted L1 of density and energy against sod(scale*(xc - x0),
+ * scale*time) at the element centroids; l1[0] density, l1[1] energy.  The
+ * reference's own check is x0 = 50, scale = 1. */
+int bl_sod_l1(bl_handle_t* h, double time, double x0, double scale, double* l1);
+/* Per-step diagnostics of bl_run_graph: with on != 0 every step records
+ * nstep, time, the region table (nreg rows of bl_flag_totals_t) and the Sod
+ * L1 pair for (x0, scale) on the device, inside the captured step.
+ * bl_diagnostics copies up to maxrec records of 2 + 9*nreg + 2 doubles each
+ * (oldest first, since the last bl_set_diagnostics) and sets *nrec. */
+int bl_set_diagnostics(bl_handle_t* h, int32_t on, double x0, double scale, int32_t maxrec);
+int bl_diagnostics(bl_handle_t* h, int32_t maxrec, double* out, int32_t* nrec);
 
 /* Kernel timing with HIP events on the handle's stream: average duration
  * (ms) of the named kernel class over the launches recorded since the last

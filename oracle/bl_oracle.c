@@ -1775,6 +1775,119 @@ int orc_totals(orc_state_t* s, bl_totals_t* out) {
   return BL_SUCCESS;
 }
 
+/* hydro_kn_init_shortprint + hydro_kn_calc_shortprint over one flag array
+ * (hydro_kn_print.F90:32-110); cnu/cnv are the corners' velocities */
+static void calc_shortprint(int nflag, int nsize, double dencut, const int32_t* flag,
+                            const double* density, const double* energy,
+                            const double* pressure, const double* volume,
+                            const double* mass, const double* cnwt, const double* cnu,
+                            const double* cnv, bl_flag_totals_t* t) {
+  for (int ii = 1; ii <= nsize; ++ii) {
+    int iflag = EL(flag, ii);
+    if (iflag <= 0) continue;
+    bl_flag_totals_t* f = &t[iflag - 1];
+    double c1 = dencut * EL(volume, ii);
+    f->volume = f->volume + EL(volume, ii);
+    if (EL(mass, ii) > c1) {
+      double w2 = EL(mass, ii);
+      f->mass = f->mass + w2;
+      double w3 = EL(energy, ii);
+      w3 = w3 * w2;
+      f->ie = f->ie + w3;
+      double w4 = EL(pressure, ii);
+      w3 = w2 * w4;
+      f->pressure = f->pressure + w3;
+      if (w4 > f->pressure_max) f->pressure_max = w4;
+      if (w4 < f->pressure_min) f->pressure_min = w4;
+      w4 = EL(density, ii);
+      if (w4 > f->density_max) f->density_max = w4;
+      if (w4 < f->density_min) f->density_min = w4;
+      for (int ic = 1; ic <= 4; ++ic) {
+        w2 = CN(cnu, ic, ii);
+        w3 = CN(cnv, ic, ii);
+        f->ke = 0.5 * CN(cnwt, ic, ii) * w4 * (w2 * w2 + w3 * w3) + f->ke;
+      }
+    }
+  }
+}
+
+static void init_shortprint(int n, bl_flag_totals_t* t) {
+  for (int k = 0; k < n; ++k) {
+    t[k].volume = 0.0; t[k].ie = 0.0; t[k].pressure = 0.0;
+    t[k].pressure_max = -DBL_MAX; t[k].pressure_min = DBL_MAX;
+    t[k].mass = 0.0; t[k].ke = 0.0;
+    t[k].density_max = -DBL_MAX; t[k].density_min = DBL_MAX;
+  }
+}
+
+/* hydro_dr_shortprint, hydro_dr_print.f90:49-119 (tables only) */
+int orc_shortprint(orc_state_t* s, bl_flag_totals_t* region, bl_flag_totals_t* material) {
+  int nel = s->sz.nel, ncp = s->sz.ncp;
+  double* cnu = (double*)malloc(sizeof(double) * 4 * (size_t)(nel > 0 ? nel : 1));
+  double* cnv = (double*)malloc(sizeof(double) * 4 * (size_t)(nel > 0 ? nel : 1));
+  k_cngather(nel, I(BL_IELND), R(BL_NDU), cnu);
+  k_cngather(nel, I(BL_IELND), R(BL_NDV), cnv);
+  init_shortprint(s->sz.nreg, region);
+  calc_shortprint(s->sz.nreg, nel, s->cfg.dencut, I(BL_IELREG), R(BL_ELDENSITY),
+                  R(BL_ELENERGY), R(BL_ELPRESSURE), R(BL_ELVOLUME), R(BL_ELMASS),
+                  R(BL_CNWT), cnu, cnv, region);
+  init_shortprint(s->sz.nmat, material);
+  calc_shortprint(s->sz.nmat, nel, s->cfg.dencut, I(BL_IELMAT), R(BL_ELDENSITY),
+                  R(BL_ELENERGY), R(BL_ELPRESSURE), R(BL_ELVOLUME), R(BL_ELMASS),
+                  R(BL_CNWT), cnu, cnv, material);
+  if (ncp > 0) { /* utils_kn_mxgathercncp of cnwt, cnu, cnv to the components */
+    double* cw = (double*)malloc(sizeof(double) * 4 * (size_t)ncp);
+    double* cu = (double*)malloc(sizeof(double) * 4 * (size_t)ncp);
+    double* cv = (double*)malloc(sizeof(double) * 4 * (size_t)ncp);
+    for (int imx = 1; imx <= s->sz.nmx; ++imx) {
+      int iel = EL(I(BL_IMXEL), imx), icp = EL(I(BL_IMXFCP), imx);
+      for (int ii = 1; ii <= EL(I(BL_IMXNCP), imx); ++ii, ++icp)
+        for (int j = 1; j <= 4; ++j) {
+          CN(cw, j, icp) = CN(R(BL_CNWT), j, iel);
+          CN(cu, j, icp) = CN(cnu, j, iel);
+          CN(cv, j, icp) = CN(cnv, j, iel);
+        }
+    }
+    calc_shortprint(s->sz.nmat, ncp, s->cfg.dencut, I(BL_ICPMAT), R(BL_CPDENSITY),
+                    R(BL_CPENERGY), R(BL_CPPRESSURE), R(BL_CPVOLUME), R(BL_CPMASS), cw, cu,
+                    cv, material);
+    free(cw); free(cu); free(cv);
+  }
+  free(cnu); free(cnv);
+  return BL_SUCCESS;
+}
+
+/* check_kn_sod + check_dr_tests (check_kn_tests.f90:33-62,
+ * check_dr_tests.f90:56-69) with the solution sod(scale*(xc-x0),
+ * scale*time); centroid of geometry_kn_getcentroid
+ * (geometry_kn_centroid.f90:29-55) */
+int orc_sod_l1(orc_state_t* s, double time, double x0, double scale, double* l1) {
+  int nel = s->sz.nel;
+  const double *cnx = R(BL_CNX), *cny = R(BL_CNY);
+  double basis = 0.0, la = 0.0, lb = 0.0;
+  for (int ii = 1; ii <= nel; ++ii) {
+    double cx = 0.0, cy = 0.0, rvol = 0.0;
+    for (int i1 = 1; i1 <= 4; ++i1) {
+      int i2 = (i1 % 4) + 1;
+      double rr = CN(cnx, i1, ii) * CN(cny, i2, ii) - CN(cnx, i2, ii) * CN(cny, i1, ii);
+      rvol = rvol + rr;
+      cx = cx + rr * (CN(cnx, i1, ii) + CN(cnx, i2, ii));
+      cy = cy + rr * (CN(cny, i1, ii) + CN(cny, i2, ii));
+    }
+    rvol = 0.5 * rvol;
+    cx = (rvol > 0.0) ? cx / (6.0 * rvol) : 0.0;
+    double r, e;
+    orc_sod_exact(scale * (cx - x0), scale * time, &r, &e);
+    double v = EL(R(BL_ELVOLUME), ii);
+    basis = basis + v;
+    la = fabs(EL(R(BL_ELDENSITY), ii) - r) * v + la;
+    lb = fabs(EL(R(BL_ELENERGY), ii) - e) * v + lb;
+  }
+  l1[0] = la / basis;
+  l1[1] = lb / basis;
+  return BL_SUCCESS;
+}
+
 /* sod(), check_kn_tests.f90:64-111 */
 void orc_sod_exact(double xx, double time, double* rho, double* e) {
   double coeff = sqrt(1.4);

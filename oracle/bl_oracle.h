@@ -69,6 +69,8 @@ int orc_run(orc_state_t* s, int32_t nsteps, int32_t stop_at_time_end,
             bl_timestep_t* ts, double* dt_hist, int32_t* idt_hist,
             int32_t* ctl_hist, int32_t* steps_taken, bl_error_t* err);
 int orc_totals(orc_state_t* s, bl_totals_t* out);
+int orc_shortprint(orc_state_t* s, bl_flag_totals_t* region, bl_flag_totals_t* material);
+int orc_sod_l1(orc_state_t* s, double time, double x0, double scale, double* l1);
 
 /* analytic Sod solution of check_kn_tests.f90:64-111 at (x,t) for the
  * reference's [0,100] tube; returns density and energy */

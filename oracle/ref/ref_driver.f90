@@ -61,6 +61,9 @@ PROGRAM bl_ref_driver
   USE ale_dr_advect_mod,       ONLY: ale_dr_advect
   USE ale_dr_update_mod,       ONLY: ale_dr_hydroupdate
   USE timerAPI_id_mod
+  USE hydro_kn_print_mod,      ONLY: hydro_kn_init_shortprint,                 &
+&                                    hydro_kn_calc_shortprint
+  USE utils_kn_gather_mod,     ONLY: utils_kn_cngather,utils_kn_mxgathercncp
 
   IMPLICIT NONE
 
@@ -279,7 +282,10 @@ PROGRAM bl_ref_driver
     ENDDO
     CALL SYSTEM_CLOCK(c1)
     wall=REAL(c1-c0,rlk)/REAL(crate,rlk)
-    IF (mode.EQ.0_ink) CALL dump(runtime%timestep%nstep)
+    IF (mode.EQ.0_ink) THEN
+      CALL dump(runtime%timestep%nstep)
+      CALL shortprint_record()
+    ENDIF
     IF (mode.EQ.2_ink) THEN
       WRITE(*,'(a,i12,a,i8,a,es16.9,a,es16.9)') 'REFTIMING nel=',nel,         &
 &       ' steps=',runtime%timestep%nstep,' seconds=',wall,                     &
@@ -327,6 +333,56 @@ CONTAINS
     WRITE(iu) 0_ink,0_ink,0_8
     ndump=ndump+1
   END SUBROUTINE dump
+
+  ! The tables of hydro_dr_shortprint (before its averaging and printing),
+  ! from the reference's own kernels on local arrays: marker 2, nreg, nmat,
+  ! then for the regions and then the materials the nine columns volume,
+  ! mass, ie, ke, pressure, pmin, pmax, dmin, dmax, each over the flags.
+  SUBROUTINE shortprint_record()
+    INTEGER(KIND=ink) :: nr,nm,ne,nn,nc,nx
+    REAL(KIND=rlk),DIMENSION(runtime%sizes%nreg) :: rv,rm,rk,ri,rdn,rdx,rp,    &
+&                                                   rpn,rpx
+    REAL(KIND=rlk),DIMENSION(runtime%sizes%nmat) :: mv,mm,mk,mi,mdn,mdx,mp,    &
+&                                                   mpn,mpx
+    REAL(KIND=rlk),DIMENSION(:,:),ALLOCATABLE :: cu,cv,pw,pu,pv
+    nr=runtime%sizes%nreg
+    nm=runtime%sizes%nmat
+    ne=runtime%sizes%nel
+    nn=runtime%sizes%nnd
+    nc=runtime%sizes%ncp
+    nx=runtime%sizes%nmx
+    ALLOCATE(cu(NCORN,ne),cv(NCORN,ne))
+    CALL utils_kn_cngather(ne,nn,dh(ielndid)%iaddr,dh(nduid)%raddr,cu)
+    CALL utils_kn_cngather(ne,nn,dh(ielndid)%iaddr,dh(ndvid)%raddr,cv)
+    CALL hydro_kn_init_shortprint(nr,rv,rm,rk,ri,rdn,rdx,rp,rpn,rpx)
+    CALL hydro_kn_calc_shortprint(nr,ne,config%hydro%global%dencut,            &
+&        dh(ielregid)%iaddr,dh(eldensityid)%raddr,dh(elenergyid)%raddr,        &
+&        dh(elpressureid)%raddr,dh(elvolumeid)%raddr,dh(elmassid)%raddr,       &
+&        dh(cnwtid)%raddr,cu,cv,rv,rm,rk,ri,rdn,rdx,rp,rpn,rpx)
+    CALL hydro_kn_init_shortprint(nm,mv,mm,mk,mi,mdn,mdx,mp,mpn,mpx)
+    CALL hydro_kn_calc_shortprint(nm,ne,config%hydro%global%dencut,            &
+&        dh(ielmatid)%iaddr,dh(eldensityid)%raddr,dh(elenergyid)%raddr,        &
+&        dh(elpressureid)%raddr,dh(elvolumeid)%raddr,dh(elmassid)%raddr,       &
+&        dh(cnwtid)%raddr,cu,cv,mv,mm,mk,mi,mdn,mdx,mp,mpn,mpx)
+    IF (nc.GT.0_ink) THEN
+      ALLOCATE(pw(NCORN,nc),pu(NCORN,nc),pv(NCORN,nc))
+      CALL utils_kn_mxgathercncp(ne,nx,nc,dh(imxelid)%iaddr,dh(imxfcpid)%iaddr,&
+&          dh(imxncpid)%iaddr,dh(cnwtid)%raddr,pw)
+      CALL utils_kn_mxgathercncp(ne,nx,nc,dh(imxelid)%iaddr,dh(imxfcpid)%iaddr,&
+&          dh(imxncpid)%iaddr,cu,pu)
+      CALL utils_kn_mxgathercncp(ne,nx,nc,dh(imxelid)%iaddr,dh(imxfcpid)%iaddr,&
+&          dh(imxncpid)%iaddr,cv,pv)
+      CALL hydro_kn_calc_shortprint(nm,nc,config%hydro%global%dencut,          &
+&          dh(icpmatid)%iaddr,dh(cpdensityid)%raddr,dh(cpenergyid)%raddr,      &
+&          dh(cppressureid)%raddr,dh(cpvolumeid)%raddr,dh(cpmassid)%raddr,     &
+&          pw,pu,pv,mv,mm,mk,mi,mdn,mdx,mp,mpn,mpx)
+      DEALLOCATE(pw,pu,pv)
+    ENDIF
+    DEALLOCATE(cu,cv)
+    WRITE(iu) 2_ink,nr,nm
+    WRITE(iu) rv,rm,ri,rk,rp,rpn,rpx,rdn,rdx
+    WRITE(iu) mv,mm,mi,mk,mp,mpn,mpx,mdn,mdx
+  END SUBROUTINE shortprint_record
 
   SUBROUTINE rview(id,p)
     INTEGER(KIND=ink),                     INTENT(IN)  :: id

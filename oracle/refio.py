@@ -61,8 +61,9 @@ def write_problem(p, path, nsteps=None, dump_every=0):
 
 def read_dumps(path):
     """Returns (dumps, steps): dumps is a list of dicts with 'label', 'nstep',
-    'idtel', 'dt', 'time', 'dts' and one array per dataAPI field name;
-    steps is a list of (nstep, idtel, dt, time, sdt)."""
+    'idtel', 'dt', 'time', 'dts' and one array per dataAPI field name (the
+    last dump of a full run also holds 'shortprint': region and material
+    tables of shape (nflag, 9), columns in abi.FLAG_TOTALS order); steps is a list of (nstep, idtel, dt, time, sdt)."""
     data = open(path, "rb").read()
     pos = 0
 
@@ -72,7 +73,7 @@ def read_dumps(path):
         pos += a.nbytes
         return a
 
-    dumps, steps = [], []
+    dumps, steps, shortprint = [], [], {}
     while pos < len(data):
         marker = int(take(np.int32)[0])
         if marker == -9:
@@ -82,6 +83,13 @@ def read_dumps(path):
             dt, time = take(np.float64, 2)
             sdt = bytes(take(np.uint8, 8)).decode()
             steps.append((int(nstep), int(idtel), float(dt), float(time), sdt))
+            continue
+        if marker == 2:
+            # hydro_dr_shortprint's tables: 9 columns over the regions, then
+            # over the materials, stored column by column
+            nr, nm = (int(v) for v in take(np.int32, 2))
+            shortprint["region"] = take(np.float64, 9 * nr).reshape(9, nr).T.copy()
+            shortprint["material"] = take(np.float64, 9 * nm).reshape(9, nm).T.copy()
             continue
         assert marker == 1, marker
         label, nstep, idtel = (int(v) for v in take(np.int32, 3))
@@ -95,6 +103,8 @@ def read_dumps(path):
             dt_ = {1: np.float64, 2: np.int32, 3: np.uint8}[kind]
             d[abi.NAME[fid]] = take(dt_, count).copy()
         dumps.append(d)
+    if shortprint and dumps:
+        dumps[-1]["shortprint"] = shortprint
     return dumps, steps
 
 

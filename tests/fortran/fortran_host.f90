@@ -39,6 +39,7 @@ PROGRAM fortran_host
   TYPE(bl_timestep_t)                          :: ts
   TYPE(bl_error_t)                             :: err
   TYPE(C_PTR)                                  :: h
+  TYPE(bl_flag_totals_t),ALLOCATABLE           :: regtab(:),mattab(:)
 
   CALL GET_COMMAND_ARGUMENT(1,fin)
   CALL GET_COMMAND_ARGUMENT(2,fout)
@@ -128,9 +129,13 @@ PROGRAM fortran_host
   CALL chk(bl_download(h,NDY,C_LOC(y(1)),INT(nnd,C_INT64_T)),'y')
   CALL chk(bl_download(h,NDU,C_LOC(u(1)),INT(nnd,C_INT64_T)),'u')
   CALL chk(bl_download(h,NDV,C_LOC(v(1)),INT(nnd,C_INT64_T)),'v')
+  ! hydro_dr_shortprint's tables
+  ALLOCATE(regtab(nreg),mattab(nmat))
+  CALL chk(bl_shortprint(h,regtab,mattab),'bl_shortprint')
   OPEN(UNIT=11,FILE=TRIM(fout),ACCESS='STREAM',FORM='UNFORMATTED',            &
 &      STATUS='REPLACE')
   WRITE(11) nsteps,dth,idth,ctlh,rho,en,pre,x,y,u,v
+  WRITE(11) regtab,mattab
   CLOSE(11)
   CALL chk(bl_destroy(h),'bl_destroy')
   WRITE(*,'(A,I0,A)') 'fortran_host: ',nsteps,' steps done'

@@ -9,7 +9,8 @@ compressed .npz holding
   - SHA-256 digests of every dataAPI field after every driver call of step 1
     and after the last step, and
   - the final density, energy, pressure and node coordinates/velocities in
-    full.
+    full, and
+  - hydro_dr_shortprint's region and material tables after the last step.
 Digests are taken over a canonical form (-0.0 -> +0.0, one NaN pattern), the
 same IEEE equality the parity tests use.
 
@@ -87,6 +88,9 @@ def main():
             "ref_time": np.array([s[3] for s in steps]),
         })
         arrays.update({"final_" + k: last[k] for k in FULL})
+        # hydro_dr_shortprint's tables after the last step
+        arrays["final_shortprint_region"] = last["shortprint"]["region"]
+        arrays["final_shortprint_material"] = last["shortprint"]["material"]
         arrays["meta"] = np.frombuffer(json.dumps(meta).encode(), np.uint8)
         out = os.path.join(HERE, name + ".npz")
         np.savez_compressed(out, **arrays)

@@ -52,3 +52,18 @@ def digest_mismatches(state, record, skip=()):
         if digest(np.asarray(state[n])[:count]) != d:
             bad.append(n)
     return bad
+
+
+# hydro_dr_shortprint table columns: 0-4 sums, 5-8 minima/maxima
+TABLE_SUM_RTOL = 1e-12
+
+
+def assert_table_matches(got, ref, what=""):
+    """A device shortprint table against a serial one: the minima and maxima
+    exactly, the sums to TABLE_SUM_RTOL (a tree of positive terms in another
+    order, |error| <= n eps sum)."""
+    got, ref = np.asarray(got), np.asarray(ref)
+    assert got.shape == ref.shape, what
+    assert np.array_equal(got[:, 5:], ref[:, 5:]), what
+    np.testing.assert_allclose(got[:, :5], ref[:, :5], rtol=TABLE_SUM_RTOL, atol=0.0,
+                               err_msg=what)

@@ -1,7 +1,9 @@
 """A Fortran host (tests/fortran/fortran_host.f90) drives the library through
 the iso_c_binding module bookleaf_f90_amd/fortran/bookleaf_hip_mod.f90: the
 reference's solver sequence over the per-driver entry points, then the graph
-runner.  Its dt sequence and final state must equal the oracle's bit for bit."""
+runner.  Its dt sequence and final state must equal the oracle's bit for bit,
+and the conservation tables it reads through bl_shortprint the oracle's to a
+summation-order tolerance."""
 import os
 import subprocess
 
@@ -10,12 +12,13 @@ import pytest
 
 from bookleaf_f90_amd import problem as P
 from bookleaf_f90_amd import build
+from tests import golden_util as G
 
 pytestmark = pytest.mark.gpu
 EXE = os.path.join(build.FORTRAN_BUILD, "fortran_host")
 
 
-def _read(path, nel, nnd):
+def _read(path, nel, nnd, nreg, nmat):
     data = open(path, "rb").read()
     n = int(np.frombuffer(data, np.int32, 1)[0])
     pos = 4
@@ -27,6 +30,10 @@ def _read(path, nel, nnd):
                           ("ndv", np.float64, nnd)):
         out[name] = np.frombuffer(data, dt, cnt, pos)
         pos += out[name].nbytes
+    for name, nf in (("region", nreg), ("material", nmat)):
+        out[name] = np.frombuffer(data, np.float64, 9 * nf, pos).reshape(nf, 9)
+        pos += out[name].nbytes
+    assert pos == len(data)
     return out
 
 
@@ -46,7 +53,7 @@ def test_fortran_host_matches_oracle(case, tmp_path, oracle_lib):
     r = subprocess.run([EXE, fin, fout, str(nhost)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    f = _read(fout, p.nel, p.nnd)
+    f = _read(fout, p.nel, p.nnd, p.nreg, p.nmat)
     o = Oracle(p)
     dt, idt, ctl = o.run(nsteps)
     assert np.array_equal(f["dt"], dt)
@@ -54,3 +61,6 @@ def test_fortran_host_matches_oracle(case, tmp_path, oracle_lib):
     assert np.array_equal(f["ctl"], ctl)
     for k in ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv"):
         assert np.array_equal(f[k], o.get(k)[:len(f[k])]), k
+    reg, mat = o.shortprint()
+    G.assert_table_matches(f["region"], reg, "region")
+    G.assert_table_matches(f["material"], mat, "material")

@@ -1,6 +1,8 @@
 """The device-resident runner (bl_run_graph) against the reference's own
 recorded results (tests/golden/, written from oracle/_ref): dt sequence and
-fields after many steps, bit for bit, with no oracle in between."""
+fields after many steps, bit for bit, and the conservation tables of
+hydro_dr_shortprint to a summation-order tolerance, with no oracle in
+between."""
 import numpy as np
 import pytest
 
@@ -26,3 +28,6 @@ def test_graph_runner_matches_reference_record(name):
     for k in ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv"):
         assert np.array_equal(s[k][:len(z["final_" + k])], z["final_" + k]), k
     assert G.digest_mismatches(s, meta["final"], skip=excluded(p)) == []
+    reg, mat = g.shortprint()
+    G.assert_table_matches(reg, z["final_shortprint_region"], "region")
+    G.assert_table_matches(mat, z["final_shortprint_material"], "material")

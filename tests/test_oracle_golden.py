@@ -1,7 +1,8 @@
 """The CPU oracle against the committed golden fixtures (tests/golden/), which
 hold the reference's own results (oracle/_ref, flang build of the reference
 sources) for small problems: every field after every driver call of step 1,
-the dt sequence of many steps and every field after the last step.  These pin
+the dt sequence of many steps, every field after the last step and the
+conservation tables of hydro_dr_shortprint then.  These pin
 oracle/bl_oracle.c where the reference sources are not present."""
 import numpy as np
 import pytest
@@ -42,3 +43,7 @@ def test_oracle_steps_match_reference_record(name, oracle_lib):
     for k in ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv"):
         assert np.array_equal(s[k][:len(z["final_" + k])], z["final_" + k]), k
     assert G.digest_mismatches(s, meta["final"]) == []
+    # hydro_dr_shortprint's tables, same serial order as the reference
+    reg, mat = o.shortprint()
+    assert np.array_equal(reg, z["final_shortprint_region"])
+    assert np.array_equal(mat, z["final_shortprint_material"])

@@ -234,6 +234,8 @@ def test_sod_deck_l1_matches_reference_run(eulerian, expected, oracle_lib):
     dt, _, _ = o.run(5000, stop_at_time_end=True)
     assert len(dt) == 384
     assert "%.8E" % _sod_l1(o, p) == expected
+    # the library's check_kn_sod restatement, same serial order
+    assert o.sod_l1()[0] == _sod_l1(o, p)
 
 
 def test_sod_mm_deck_l1_matches_reference_run(oracle_lib):
@@ -244,6 +246,7 @@ def test_sod_mm_deck_l1_matches_reference_run(oracle_lib):
     dt, _, _ = o.run(5000, stop_at_time_end=True)
     assert len(dt) == 384
     assert "%.8E" % _sod_l1(o, p) == "4.18911702E-02"
+    assert o.sod_l1()[0] == _sod_l1(o, p)
 
 
 def test_sod_exact_matches_closed_form_states(oracle_lib):
