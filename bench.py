@@ -77,6 +77,8 @@ def parse():
                     help="sedov: Lagrangian 1024^2 Sedov (the headline); sod2mat: Eulerian "
                          "(ALE remap every step) two-material Sod on 2048x256 with mixed "
                          "interface cells")
+    ap.add_argument("--diagnostics", action="store_true",
+                    help="record the region table and Sod L1 on the device every step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -133,6 +135,8 @@ def main():
     nel, nnd = prob.nel, prob.nnd
     g = BookLeaf(prob)
     print("# workload %s nel=%d nnd=%d" % (wname, nel, nnd), file=sys.stderr, flush=True)
+    if args.diagnostics:
+        g.set_diagnostics(True, maxrec=args.warmup + args.steps + args.profile_steps)
     if args.warmup:
         g.run(args.warmup, runner=args.runner)
     g.synchronize()
@@ -193,7 +197,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": wname, "nel": nel, "nnd": nnd, "runner": args.runner},
+        "config": {"workload": wname, "nel": nel, "nnd": nnd, "runner": args.runner,
+                   "diagnostics": bool(args.diagnostics)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,

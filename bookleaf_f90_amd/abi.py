@@ -76,6 +76,9 @@ class Error(C.Structure):
         return bytes(self.serr).decode(errors="replace").strip()
 
 
+# node relaxation types (setup_mesh's region types)
+RELAX_OFF, RELAX_LIN1, RELAX_LIN2, RELAX_EQUI = 0, 1, 2, 3
+
 FLAG_TOTALS = ("volume", "mass", "ie", "ke", "pressure", "pressure_min", "pressure_max",
                "density_min", "density_max")
 
@@ -157,6 +160,8 @@ def declare(lib, prefix):
         "sod_l1": [vp, C.c_double, C.c_double, C.c_double, P(C.c_double)],
         "set_diagnostics": [vp, C.c_int32, C.c_double, C.c_double, C.c_int32],
         "diagnostics": [vp, C.c_int32, P(C.c_double), P(C.c_int32)],
+        "set_relaxation": [vp, C.c_int32, C.c_int32, C.c_double, C.c_double],
+        "relax_mesh": [vp, C.c_int32, C.c_int32, C.c_double, C.c_double],
     }
     for name, args in sig.items():
         f = getattr(lib, prefix + "_" + name, None)

@@ -26,7 +26,7 @@ namespace bl {
 #define ALE_GATE(ts) \
   if ((ts) && ((ts)->skip || !(ts)->ale_on)) return
 
-// ale_kn_getmeshstatus (Eulerian: indstatus = 2)
+// ale_kn_getmeshstatus (Eulerian, or node relaxation: indstatus = 2)
 __global__ void k_meshstatus(int nnd, int32_t* __restrict__ st, const DevTimestep* ts) {
   ALE_GATE(ts);
   int i = xcd_index();
@@ -313,7 +313,7 @@ int bl_ale_active(bl_handle_t* h, const bl_timestep_t* ts) {
 namespace bl {
 
 static int meshstatus(bl_handle_t* h, const DevTimestep* dts) {
-  if (h->cfg.zeul) {
+  if (h->cfg.zeul || h->relax_type != BL_RELAX_OFF) {
     int nnd = h->sz.nnd;
     BL_LAUNCH(h, "ale_meshstatus", k_meshstatus, nblocks(nnd), kBlock, nnd,
               h->I(BL_ISCRATCH11), dts);
@@ -323,6 +323,11 @@ static int meshstatus(bl_handle_t* h, const DevTimestep* dts) {
 
 static int fluxvolume(bl_handle_t* h, double dt, const DevTimestep* dts) {
   int nnd = h->sz.nnd, nel = h->sz.nel;
+  // the relaxation's mesh velocity replaces the Lagrangian one in store4/5
+  if (!h->cfg.zeul && h->relax_type != BL_RELAX_OFF &&
+      relax_velocity(h, h->R(BL_NDX), h->R(BL_NDY), h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15),
+                     dt, dts))
+    return BL_FAILURE;
   BL_LAUNCH(h, "ale_fluxvol_nodes", k_fluxvol_nodes, nblocks(nnd), kBlock, nnd, h->cfg.zeul,
             dt, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), h->R(BL_RSCRATCH12),
             h->R(BL_RSCRATCH13), h->R(BL_NDX), h->R(BL_NDY), dts);

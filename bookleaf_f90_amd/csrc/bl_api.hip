@@ -613,11 +613,12 @@ int bl_destroy(bl_handle_t* h) {
                   h->d_eos_itype, h->d_eos_param, h->d_hyd_ptr, h->d_hyd_idx,
                   h->d_ale_ptr, h->d_ale_idx, h->d_el_ptr, h->d_el_idx, h->d_part,
                   h->d_parti, h->d_ticket, h->d_dtres, h->d_ts, h->d_hist, h->d_outer,
-                  h->d_diag_hist, h->d_diag_count, h->d_diag_part};
+                  h->d_diag_hist, h->d_diag_count, h->d_diag_part, h->d_ring};
   for (void* p : bufs) if (p) hipFree(p);
   for (double* p : h->d_tmp) if (p) hipFree(p);
   for (double* p : h->cterm) if (p) hipFree(p);
   for (double* p : h->ale_tmp) if (p) hipFree(p);
+  for (double* p : h->relax_tmp) if (p) hipFree(p);
   if (h->h_dtres) hipHostFree(h->h_dtres);
   if (h->h_ts) hipHostFree(h->h_ts);
   for (hipEvent_t ev : h->ev_pool) hipEventDestroy(ev);
@@ -820,6 +821,7 @@ int bl_build_maps(bl_handle_t* h) {
     for (int j = 0; j < 4; ++j) { nodes.push_back(ielnd[4 * (size_t)iel + j]); corners.push_back(4 * iel + j); }
   if (build_csr(h, nodes, corners, &h->d_el_ptr, &h->d_el_idx)) return BL_FAILURE;
   if (build_outer(h, ielnd)) return BL_FAILURE;
+  if (h->relax_type != BL_RELAX_OFF && relax_rebuild(h)) return BL_FAILURE;
   h->maps_built = true;
   invalidate_graph(h);
   return BL_SUCCESS;

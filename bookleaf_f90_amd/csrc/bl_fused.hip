@@ -497,8 +497,8 @@ k_f4_state(FusedArgs A, FusedNodes O) {
 }
 
 // ------------------------------------------------------- remap (Eulerian)
-// The unsplit remap of ale_dr_advect (adv_type = 1) with zeul, in five grid
-// kernels.  Intermediates live in the handle's own buffers (ale_tmp) rather
+// The unsplit remap of ale_dr_advect (adv_type = 1) with zeul or the node
+// relaxation's mesh velocity, in five grid kernels.  Intermediates live in the handle's own buffers (ale_tmp) rather
 // than the alestep scratch ids, which nothing reads after the step.  On a
 // conforming mesh the neighbour's flux volume of a shared face is a function
 // of the two shared nodes only, so each element computes it itself with the
@@ -850,6 +850,10 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   if (h->cfg.zexist) {
     // the remap on the node set this step wrote
     AleArgs B = make_ale(h, O);
+    // node relaxation: its mesh velocity overwrites ubar (B.zeul = 0)
+    if (!h->cfg.zeul &&
+        relax_velocity(h, O.x, O.y, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), 0.0, h->d_ts))
+      return BL_FAILURE;
     BL_LAUNCH(h, "r1_volflux", k_r1, nbe, kBlock, B);
     BL_LAUNCH(h, "r2_massflux", k_r2, nbe, kBlock, B);
     BL_LAUNCH(h, "r3_energy_update", k_r3, nbe, kBlock, B);
@@ -973,9 +977,11 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     set_error("bl_run_graph: mesh maps, regions or EOS not set");
     return BL_FAILURE;
   }
-  // split advection, ALE other than Eulerian, or a mesh without the limiter
-  // stencil table: host-driven
-  if ((h->cfg.zexist && (h->cfg.adv_type != 1 || !h->cfg.zeul)) || !h->d_outer)
+  // split advection, ALE with neither Eulerian nor relaxation mesh motion, or
+  // a mesh without the limiter stencil table: host-driven
+  if ((h->cfg.zexist &&
+       (h->cfg.adv_type != 1 || (!h->cfg.zeul && h->relax_type == BL_RELAX_OFF))) ||
+      !h->d_outer)
     return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
                   err);
   fill_err(err, BL_SUCCESS, 0, "");

@@ -116,6 +116,11 @@ struct Handle {
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[13] = {};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
+  // node relaxation as the ALE mesh motion (bl_set_relaxation)
+  int32_t relax_type = 0, relax_nsweep = 1;
+  double relax_om = 1.0, relax_tol = 0.0;
+  int32_t* d_ring = nullptr;      // (8, nnd) stencil nodes, -1: not relaxed
+  double* relax_tmp[4] = {};      // sweep ping-pong, nnd each
   // per-step diagnostics of the fused step: region table + Sod L1 records
   bool diag_on = false;
   double diag_x0 = 0.0, diag_scale = 1.0;
@@ -145,6 +150,11 @@ double* tmp_buffer(Handle* h, int k);
 // ALE step launches; with d_ts the kernels read dt, the step gate and
 // ale_on from the device timestep (bl_ale.hip)
 int ale_step_device(Handle* h, const DevTimestep* d_ts);
+// the relaxation's ALE mesh velocity (x_R - x)/dt into (wu, wv), gated by
+// ts when given; the stencil table rebuilt from the connectivity
+int relax_velocity(Handle* h, const double* x, const double* y, double* wu, double* wv,
+                   double dt, const DevTimestep* ts);
+int relax_rebuild(Handle* h);
 // the per-step diagnostics record from the node set a step wrote
 int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* ndu,
                 const double* ndv);

@@ -1,0 +1,294 @@
+// bl_relax.hip -- node relaxation: the stencil of the mesh generator's SOR
+// (setup_mesh_region, setup_mesh.F90:795-877) as Jacobi sweeps on the
+// device, used as the ALE step's mesh motion (bl_set_relaxation) and on its
+// own (bl_relax_mesh).
+//
+// Each node's 8-node stencil is a table built once on the host from the
+// connectivity: (8, nnd) int32, 0-based, in the order of the weights r1..r8
+// of setup_mesh.F90:848-857 -- (lp,km) (lp,kk) (lp,kp) (ll,km) (ll,kp)
+// (lm,km) (lm,kk) (lm,kp) -- and -1 for nodes that are not relaxed
+// (boundary nodes, or fewer or more than four elements around them).  A
+// sweep is one thread per node: 32 B of table, the node and its eight
+// neighbours' coordinates (shared with the neighbouring threads through the
+// cache), 16 B out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bl_internal.h"
+#include "bl_math.h"
+
+namespace bl {
+
+struct RelaxArgs {
+  int nnd, type;
+  double om, tol, zerocut;
+  const int32_t* ring;
+};
+
+// one node of one sweep (setup_mesh.F90:797-863); (xo, yo) the position
+// after it.  rr of the generator is y, ss is x.
+__device__ __forceinline__ void relax_node(const RelaxArgs& R, int n, const double* __restrict__ xi,
+                                           const double* __restrict__ yi, double& xo,
+                                           double& yo) {
+  const double xn = xi[n], yn = yi[n];
+  xo = xn;
+  yo = yn;
+  const int4 qa = reinterpret_cast<const int4*>(R.ring)[2 * n];
+  const int4 qb = reinterpret_cast<const int4*>(R.ring)[2 * n + 1];
+  if (qa.x < 0) return;
+  const int nb[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+  double x[8], y[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    x[i] = xi[nb[i]];
+    y[i] = yi[nb[i]];
+  }
+  double r1, r2, r3, r4, r5, r6, r7, r8;
+  if (R.type == BL_RELAX_EQUI) {
+    // (ll,kp) = 4, (ll,km) = 3, (lp,kk) = 1, (lm,kk) = 6
+    const double r_psi = 0.5 * (y[4] - y[3]);
+    const double s_psi = 0.5 * (x[4] - x[3]);
+    const double r_phi = 0.5 * (y[1] - y[6]);
+    const double s_phi = 0.5 * (x[1] - x[6]);
+    const double zeta = s_psi * r_phi - s_phi * r_psi;
+    if (fabs(zeta) < R.zerocut) return;
+    const double alph = s_psi * s_psi + r_psi * r_psi;
+    const double beta = s_phi * s_psi + r_phi * r_psi;
+    const double gamm = s_phi * s_phi + r_phi * r_phi;
+    r1 = 0.5 * beta; r2 = alph; r3 = -r1; r4 = gamm;
+    r5 = r4; r6 = r3; r7 = r2; r8 = r1;
+    double fac = r1 + r2 + r3 + r4 + r5 + r6 + r7 + r8;
+    if (fac < 1.0e-24) {
+      fac = fac * 1.0e14;
+      r1 = r1 * 1.0e14; r2 = r2 * 1.0e14; r3 = r3 * 1.0e14; r4 = r4 * 1.0e14;
+      r5 = r5 * 1.0e14; r6 = r6 * 1.0e14; r7 = r7 * 1.0e14; r8 = r8 * 1.0e14;
+    }
+    r1 = r1 / fac; r2 = r2 / fac; r3 = r3 / fac; r4 = r4 / fac;
+    r5 = r5 / fac; r6 = r6 / fac; r7 = r7 / fac; r8 = r8 / fac;
+  } else {
+    // LIN1: r_wgt(2) = r_wgt(7) = 1; LIN2: r_wgt(4) = r_wgt(5) = 1; each
+    // divided by their sum (setup_mesh.F90:626-660)
+    const double w = 1.0 / 2.0;
+    r1 = 0.0; r2 = 0.0; r3 = 0.0; r4 = 0.0; r5 = 0.0; r6 = 0.0; r7 = 0.0; r8 = 0.0;
+    if (R.type == BL_RELAX_LIN1) { r2 = w; r7 = w; } else { r4 = w; r5 = w; }
+  }
+  // s_i = r_i for every type here
+  const double dr = r1 * y[0] + r2 * y[1] + r3 * y[2] + r4 * y[3] + r5 * y[4] + r6 * y[5] +
+                    r7 * y[6] + r8 * y[7] - yn;
+  const double ds = r1 * x[0] + r2 * x[1] + r3 * x[2] + r4 * x[3] + r5 * x[4] + r6 * x[5] +
+                    r7 * x[6] + r8 * x[7] - xn;
+  if ((fabs(dr) > R.tol) || (fabs(ds) > R.tol)) {
+    yo = yn + R.om * dr;
+    xo = xn + R.om * ds;
+  }
+}
+
+#define RELAX_GATE(ts) \
+  if ((ts) && ((ts)->skip || !(ts)->ale_on)) return
+
+__global__ void k_relax_sweep(RelaxArgs R, const double* __restrict__ xi,
+                              const double* __restrict__ yi, double* __restrict__ xo,
+                              double* __restrict__ yo, const DevTimestep* ts) {
+  RELAX_GATE(ts);
+  const int n = xcd_index();
+  if (n >= R.nnd) return;
+  double a, b;
+  relax_node(R, n, xi, yi, a, b);
+  xo[n] = a;
+  yo[n] = b;
+}
+
+// the last sweep of the ALE mesh motion: w = (x_R - x)/dt, x the Lagrangian
+// positions (xl, yl), x_R this sweep's result from (xi, yi)
+__global__ void k_relax_velocity(RelaxArgs R, const double* __restrict__ xl,
+                                 const double* __restrict__ yl, const double* __restrict__ xi,
+                                 const double* __restrict__ yi, double* __restrict__ wu,
+                                 double* __restrict__ wv, double dt, const DevTimestep* ts) {
+  RELAX_GATE(ts);
+  if (ts) dt = ts->dt;
+  const int n = xcd_index();
+  if (n >= R.nnd) return;
+  double a, b;
+  relax_node(R, n, xi, yi, a, b);
+  wu[n] = (a - xl[n]) / dt;
+  wv[n] = (b - yl[n]) / dt;
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+namespace {
+
+// the stencil table of every node (same frame rule as the C-ABI states):
+// per node, its elements as (element, corner) in ascending element order;
+// e0 the first with the node at corner 1, else the first; then walk
+// counter-clockwise, the next element being the one whose corner after the
+// node is the current element's corner before it
+int build_ring(bl_handle_t* h) {
+  const int nel = h->sz.nel, nnd = h->sz.nnd;
+  std::vector<int32_t> ielnd(4 * (size_t)nel), indtype(nnd);
+  if (bl_download(h, BL_IELND, ielnd.data(), 4 * (int64_t)nel) ||
+      bl_download(h, BL_INDTYPE, indtype.data(), nnd))
+    return BL_FAILURE;
+  std::vector<int32_t> first(nnd + 1, 0), slots(4 * (size_t)nel);
+  for (int k = 0; k < 4 * nel; ++k) {
+    const int n = ielnd[k] - 1;
+    if (n < 0 || n >= nnd) { set_error("relaxation: ielnd out of range"); return BL_FAILURE; }
+    first[n + 1] += 1;
+  }
+  for (int n = 0; n < nnd; ++n) first[n + 1] += first[n];
+  std::vector<int32_t> pos(first.begin(), first.end() - 1);
+  for (int k = 0; k < 4 * nel; ++k) slots[pos[ielnd[k] - 1]++] = k;  // k = 4*e + corner
+  auto at = [&](int slot, int d) { return ielnd[4 * (size_t)(slot >> 2) + ((slot + d) & 3)] - 1; };
+  std::vector<int32_t> ring(8 * (size_t)std::max(nnd, 1), -1);
+  for (int n = 0; n < nnd; ++n) {
+    const int b = first[n], e = first[n + 1];
+    if (indtype[n] <= 0 || e - b != 4) continue;
+    int cur = slots[b];
+    for (int k = b; k < e; ++k)
+      if ((slots[k] & 3) == 0) { cur = slots[k]; break; }
+    // counter-clockwise from +k: (ll,kp) (lp,kp) (lp,kk) (lp,km) (ll,km)
+    // (lm,km) (lm,kk) (lm,kp)
+    int ccw[8];
+    bool seen[4] = {false, false, false, false};
+    bool ok = true;
+    for (int q = 0; q < 4 && ok; ++q) {
+      if (q > 0) {
+        const int want = at(cur, 3);
+        int nxt = -1;
+        for (int k = b; k < e; ++k)
+          if (!seen[k - b] && at(slots[k], 1) == want) { nxt = k; break; }
+        if (nxt < 0) { ok = false; break; }
+        cur = slots[nxt];
+        seen[nxt - b] = true;
+      } else {
+        for (int k = b; k < e; ++k)
+          if (slots[k] == cur) seen[k - b] = true;
+      }
+      if (q == 0) ccw[0] = at(cur, 1);
+      ccw[2 * q + 1] = at(cur, 2);
+      if (q < 3) ccw[2 * q + 2] = at(cur, 3);
+    }
+    if (!ok || at(cur, 3) != ccw[0]) continue;
+    int32_t* r = &ring[8 * (size_t)n];
+    r[0] = ccw[3]; r[1] = ccw[2]; r[2] = ccw[1]; r[3] = ccw[4];
+    r[4] = ccw[0]; r[5] = ccw[5]; r[6] = ccw[6]; r[7] = ccw[7];
+  }
+  if (h->d_ring) hipFree(h->d_ring);
+  h->d_ring = nullptr;
+  BL_HIP(hipMalloc(&h->d_ring, sizeof(int32_t) * ring.size()));
+  BL_HIP(hipMemcpy(h->d_ring, ring.data(), sizeof(int32_t) * ring.size(), hipMemcpyHostToDevice));
+  return BL_SUCCESS;
+}
+
+int ensure_tmp(bl_handle_t* h) {
+  if (h->relax_tmp[0]) return BL_SUCCESS;
+  for (int k = 0; k < 4; ++k)
+    BL_HIP(hipMalloc(&h->relax_tmp[k], sizeof(double) * std::max(h->sz.nnd, 1)));
+  return BL_SUCCESS;
+}
+
+bool args_ok(int32_t type, int32_t nsweep, double om, double tol) {
+  return type >= BL_RELAX_OFF && type <= BL_RELAX_EQUI && nsweep >= 1 && om > 0.0 && tol >= 0.0;
+}
+
+RelaxArgs make_relax(bl_handle_t* h, int type, double om, double tol) {
+  RelaxArgs R;
+  R.nnd = h->sz.nnd;
+  R.type = type;
+  R.om = om;
+  R.tol = tol;
+  R.zerocut = h->cfg.zerocut;
+  R.ring = h->d_ring;
+  return R;
+}
+
+}  // namespace
+
+namespace bl {
+
+int relax_rebuild(Handle* h) {
+  if (h->relax_type == BL_RELAX_OFF && !h->d_ring) return BL_SUCCESS;
+  return build_ring(static_cast<bl_handle_t*>(h));
+}
+
+// the ALE mesh velocity of the relaxation into (wu, wv) from the positions
+// (x, y); with ts the kernels are gated like the other ALE kernels and dt
+// is the device timestep's
+int relax_velocity(Handle* hh, const double* x, const double* y, double* wu, double* wv,
+                   double dt, const DevTimestep* ts) {
+  bl_handle_t* h = static_cast<bl_handle_t*>(hh);
+  const int nnd = h->sz.nnd;
+  if (!h->d_ring || !h->relax_tmp[0]) {
+    set_error("relaxation: stencil table not built (bl_set_relaxation)");
+    return BL_FAILURE;
+  }
+  RelaxArgs R = make_relax(h, h->relax_type, h->relax_om, h->relax_tol);
+  const double* xi = x;
+  const double* yi = y;
+  for (int k = 0; k + 1 < h->relax_nsweep; ++k) {
+    double* xo = h->relax_tmp[2 * (k & 1)];
+    double* yo = h->relax_tmp[2 * (k & 1) + 1];
+    BL_LAUNCH(h, "relax_sweep", k_relax_sweep, nblocks(nnd), kBlock, R, xi, yi, xo, yo, ts);
+    xi = xo;
+    yi = yo;
+  }
+  BL_LAUNCH(h, "relax_velocity", k_relax_velocity, nblocks(nnd), kBlock, R, x, y, xi, yi, wu,
+            wv, dt, ts);
+  return BL_SUCCESS;
+}
+
+}  // namespace bl
+
+extern "C" {
+
+int bl_set_relaxation(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol) {
+  if (!h) return BL_FAILURE;
+  if (!args_ok(type, nsweep, om, tol)) {
+    set_error("bl_set_relaxation: type 0..3, nsweep >= 1, om > 0 and tol >= 0");
+    return BL_FAILURE;
+  }
+  BL_HIP(hipStreamSynchronize(h->stream));
+  invalidate_graph(h);
+  h->relax_type = type;
+  h->relax_nsweep = nsweep;
+  h->relax_om = om;
+  h->relax_tol = tol;
+  if (type == BL_RELAX_OFF) return BL_SUCCESS;
+  if (build_ring(h) || ensure_tmp(h)) return BL_FAILURE;
+  return BL_SUCCESS;
+}
+
+int bl_relax_mesh(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol) {
+  if (!h) return BL_FAILURE;
+  if (!args_ok(type, nsweep, om, tol) || type == BL_RELAX_OFF) {
+    set_error("bl_relax_mesh: type 1..3, nsweep >= 1, om > 0 and tol >= 0");
+    return BL_FAILURE;
+  }
+  if (build_ring(h) || ensure_tmp(h)) return BL_FAILURE;
+  const int nnd = h->sz.nnd;
+  RelaxArgs R = make_relax(h, type, om, tol);
+  double* x = h->R(BL_NDX);
+  double* y = h->R(BL_NDY);
+  const double* xi = x;
+  const double* yi = y;
+  for (int k = 0; k < nsweep; ++k) {
+    const bool last = k + 1 == nsweep;
+    double* xo = last ? h->relax_tmp[2] : h->relax_tmp[2 * (k & 1)];
+    double* yo = last ? h->relax_tmp[3] : h->relax_tmp[2 * (k & 1) + 1];
+    if (last && xi == h->relax_tmp[2]) { xo = h->relax_tmp[0]; yo = h->relax_tmp[1]; }
+    BL_LAUNCH(h, "relax_sweep", k_relax_sweep, nblocks(nnd), kBlock, R, xi, yi, xo, yo,
+              (const DevTimestep*)nullptr);
+    xi = xo;
+    yi = yo;
+  }
+  BL_HIP(hipMemcpyAsync(x, xi, sizeof(double) * nnd, hipMemcpyDeviceToDevice, h->stream));
+  BL_HIP(hipMemcpyAsync(y, yi, sizeof(double) * nnd, hipMemcpyDeviceToDevice, h->stream));
+  BL_HIP(hipStreamSynchronize(h->stream));
+  return BL_SUCCESS;
+}
+
+}  // extern "C"

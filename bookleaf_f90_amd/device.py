@@ -105,6 +105,8 @@ class DriverState:
         self._check(self._f("geometry_getgeometry")(self.h, C.byref(err)), "getgeometry", err)
         self._check(self._f("init_state")(self.h), "init_state")
         self._check(self._f("hydro_init")(self.h), "hydro_init")
+        if p.relax:
+            self.set_relaxation(*p.relax)
         self.ts = abi.Timestep()
         self.ts.time = p.config["time_start"]
         self.ts.dt = p.config["dt_initial"]
@@ -268,6 +270,15 @@ class DriverState:
         t = self.ts.time if time is None else time
         self._check(self._f("sod_l1")(self.h, t, x0, scale, out), "sod_l1")
         return out[0], out[1]
+
+    def set_relaxation(self, rtype=abi.RELAX_EQUI, nsweep=1, om=1.0, tol=0.0):
+        """The ALE step's node-relaxation mesh motion (zexist, not zeul)."""
+        self._check(self._f("set_relaxation")(self.h, rtype, nsweep, om, tol),
+                    "set_relaxation")
+
+    def relax_mesh(self, rtype=abi.RELAX_EQUI, nsweep=1, om=1.0, tol=0.0):
+        """nsweep relaxation sweeps on ndx, ndy in place."""
+        self._check(self._f("relax_mesh")(self.h, rtype, nsweep, om, tol), "relax_mesh")
 
     def state(self, names=None):
         names = names or abi.FIELDS

@@ -21,6 +21,9 @@ MODULE bookleaf_hip_mod
 &                                      BL_SLEN=130
   INTEGER(KIND=C_INT32_T),PARAMETER :: BL_DT_MAXIMUM=1,BL_DT_GROWTH=2,        &
 &                                      BL_DT_INITIAL=3,BL_DT_CFL=4,BL_DT_DIV=5
+  ! node relaxation types (setup_mesh's region types)
+  INTEGER(KIND=C_INT32_T),PARAMETER :: BL_RELAX_OFF=0,BL_RELAX_LIN1=1,         &
+&                                      BL_RELAX_LIN2=2,BL_RELAX_EQUI=3
   INTEGER(KIND=C_INT32_T),PARAMETER :: BL_KIND_R64=1,BL_KIND_I32=2,           &
 &                                      BL_KIND_BOOL=3
 
@@ -320,6 +323,22 @@ MODULE bookleaf_hip_mod
       REAL(KIND=C_DOUBLE),DIMENSION(*),INTENT(OUT) :: out
       INTEGER(KIND=C_INT32_T),INTENT(OUT)        :: nrec
     END FUNCTION bl_diagnostics
+
+    ! ---- node relaxation --------------------------------------------------
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_set_relaxation(h,rtype,nsweep,om,tol)   &
+&                                    BIND(C,NAME="bl_set_relaxation")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE             :: h
+      INTEGER(KIND=C_INT32_T),VALUE :: rtype,nsweep
+      REAL(KIND=C_DOUBLE),VALUE     :: om,tol
+    END FUNCTION bl_set_relaxation
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_mesh(h,rtype,nsweep,om,tol)       &
+&                                    BIND(C,NAME="bl_relax_mesh")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE             :: h
+      INTEGER(KIND=C_INT32_T),VALUE :: rtype,nsweep
+      REAL(KIND=C_DOUBLE),VALUE     :: om,tol
+    END FUNCTION bl_relax_mesh
 
     ! ---- kernel timing and diagnostics -----------------------------------
     INTEGER(KIND=C_INT32_T) FUNCTION bl_timing_enable(h,on) BIND(C,NAME="bl_timing_enable")

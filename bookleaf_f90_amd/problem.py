@@ -71,6 +71,8 @@ class Problem:
     frvolume: np.ndarray = None    # (ncp) volume fraction
     cpdensity: np.ndarray = None   # (ncp)
     cpenergy: np.ndarray = None    # (ncp)
+    # ALE node relaxation (bl_set_relaxation): (type, nsweep, om, tol)
+    relax: tuple = None
 
     def __post_init__(self):
         if self.kappareg is None:
@@ -288,14 +290,59 @@ def sedov(n=1024, length=1.2, jitter=0.1, seed=12345, energy_total=0.25, gamma=1
         nsteps=nsteps, dims=(n, n))
 
 
+def with_relaxation(p, rtype=abi.RELAX_EQUI, nsweep=1, om=1.0, tol=0.0):
+    """p with ALE on every step and node relaxation as its mesh motion
+    (zexist, not zeul)."""
+    p.config["zexist"] = 1
+    p.config["zeul"] = 0
+    p.config["mintime"] = -np.finfo(np.float64).max
+    p.config["maxtime"] = np.finfo(np.float64).max
+    p.relax = (int(rtype), int(nsweep), float(om), float(tol))
+    p.name = p.name + "_relax%d" % rtype
+    return p
+
+
+def mesh_problem(name, ielnd, ndx, ndy, indtype, gamma=1.4, density=1.0, energy=1.0, **cfg):
+    """A uniform ideal-gas state at rest on a given mesh (one region, one
+    material)."""
+    nel, nnd = len(ielnd) // 4, len(ndx)
+    itype, param = ideal_eos([gamma])
+    return Problem(name=name, ielnd=np.asarray(ielnd, np.int32), ndx=np.array(ndx, np.float64),
+                   ndy=np.array(ndy, np.float64), indtype=np.asarray(indtype, np.int32),
+                   ielreg=np.ones(nel, np.int32), ielmat=np.ones(nel, np.int32),
+                   density=np.full(nel, density), energy=np.full(nel, energy),
+                   ndu=np.zeros(nnd), ndv=np.zeros(nnd), nmat=1, nreg=1, eos_itype=itype,
+                   eos_param=param, config=dict(DEFAULT_CONFIG, **cfg))
+
+
+def harmonic_map(n, eps=0.25, jitter=0.2, seed=7):
+    """An n x n mesh of the unit logical square under the inverse of the
+    harmonic map xi = x + eps (x^2 - y^2), eta = y.  Returns (ielnd, ndx,
+    ndy, indtype, exact_x, exact_y): boundary nodes at their exact positions,
+    interior nodes displaced from theirs by U(-jitter, jitter) cell widths,
+    and the exact positions of every node.  xi and eta are harmonic in
+    (x, y), so the exact positions solve the equipotential (Winslow)
+    equations; the grid is not orthogonal (eps != 0)."""
+    ielnd, xi, eta, indtype = structured_mesh(n, n, 0.0, 1.0, 0.0, 1.0)
+    ex = (-1.0 + np.sqrt(1.0 + 4.0 * eps * (xi + eps * eta * eta))) / (2.0 * eps)
+    ey = eta.copy()
+    rng = np.random.default_rng(seed)
+    inner = indtype > 0
+    ndx, ndy = ex.copy(), ey.copy()
+    ndx[inner] += rng.uniform(-jitter, jitter, inner.sum()) / n
+    ndy[inner] += rng.uniform(-jitter, jitter, inner.sum()) / n
+    return ielnd, ndx, ndy, indtype, ex, ey
+
+
 def sod2mat(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps=300,
-            eulerian=True, **cfg):
+            eulerian=True, jitter=0.0, **cfg):
     """Two-material Sod on [0,L]x[0,L*ny/nx] (square cells): material 1
     (gamma 1.4, rho 1, p 1) left of x = L/2 + A sin(2 pi k y/H + phi), material
     2 (gamma 1.67, rho 0.125, p 0.1) right of it, k and phi drawn from the
     seed.  Each element takes the material of its centroid (pure cells)."""
     height = length * ny / nx
-    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, length, 0.0, height)
+    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, length, 0.0, height,
+                                               jitter=jitter, seed=seed)
     rng = np.random.default_rng(seed)
     k = int(rng.integers(1, 4))
     phi = float(rng.uniform(0.0, 2.0 * math.pi))
@@ -346,7 +393,7 @@ def sod_mm_deck_small():
 
 
 def sod2mat_mixed(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps=300,
-                  eulerian=True, sub=8, **cfg):
+                  eulerian=True, sub=8, jitter=0.0, **cfg):
     """sod2mat with mixed interface cells: the volume fraction of material 2
     in each element is the fraction of its sub x sub sample points (bilinear
     in the element) right of the interface; elements with both materials
@@ -354,7 +401,7 @@ def sod2mat_mixed(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps
     states, and element density and energy are the volume- and
     mass-weighted means."""
     p = sod2mat(nx=nx, ny=ny, length=length, amplitude=amplitude, seed=seed, nsteps=nsteps,
-                eulerian=eulerian, **cfg)
+                eulerian=eulerian, jitter=jitter, **cfg)
     height = length * ny / nx
     rng = np.random.default_rng(seed)
     k = int(rng.integers(1, 4))

@@ -265,6 +265,32 @@ int bl_sod_l1(bl_handle_t* h, double time, double x0, double scale, double* l1);
 int bl_set_diagnostics(bl_handle_t* h, int32_t on, double x0, double scale, int32_t maxrec);
 int bl_diagnostics(bl_handle_t* h, int32_t maxrec, double* out, int32_t* nrec);
 
+/* Node relaxation with the stencil of the mesh generator's SOR
+ * (setup_mesh_region, setup_mesh.F90:795-877; fixed weights :620-660).
+ * type is the generator's region type: BL_RELAX_LIN1 averages the two
+ * l-neighbours, BL_RELAX_LIN2 the two k-neighbours, BL_RELAX_EQUI uses the
+ * equipotential (Winslow) weights of the node's 8 neighbours
+ * (setup_mesh.F90:804-847).  A node moves by om*(stencil - x) when either
+ * coordinate's residual exceeds tol (setup_mesh.F90:858-862).  Sweeps are
+ * Jacobi sweeps: every node reads the previous sweep's positions (the
+ * generator sweeps Gauss-Seidel in (l,k) order).  Nodes relaxed: interior
+ * (indtype > 0) with four elements around them.  The logical frame of node
+ * n is that of the lowest-numbered element with n at corner 1 (else the
+ * lowest-numbered element around n): k toward that element's corner 2, l
+ * toward its corner 4, as in the generator's meshes (setup_mesh_transfer). */
+enum { BL_RELAX_OFF = 0, BL_RELAX_LIN1 = 1, BL_RELAX_LIN2 = 2, BL_RELAX_EQUI = 3 };
+/* The mesh-motion option of the ALE step for zexist and not zeul, in the
+ * empty "Other options" of ale_kn_getmeshstatus and ale_kn_getmeshvelocity
+ * (ale_kn_getmeshstatus.f90:41-43, ale_kn_getmeshvelocity.f90:42-44): every
+ * node is marked moving (indstatus = 2, as Eulerian), and the mesh velocity
+ * is (x_R - x)/dt, x_R the positions after nsweep sweeps from the
+ * Lagrangian positions x.  type BL_RELAX_OFF (the default) leaves the
+ * reference's behaviour: no mesh motion option. */
+int bl_set_relaxation(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol);
+/* nsweep sweeps on ndx, ndy in place (mesh set-up); the geometry is not
+ * recomputed. */
+int bl_relax_mesh(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol);
+
 /* Kernel timing with HIP events on the handle's stream: average duration
  * (ms) of the named kernel class over the launches recorded since the last
  * bl_timing_reset. */

@@ -42,6 +42,11 @@ static inline int fmodulo_i(int a, int p) {
 struct orc_state {
   bl_sizes_t sz;
   bl_config_t cfg;
+  /* mesh-motion option of the ALE step (bl_set_relaxation) */
+  int32_t relax_type, relax_nsweep;
+  double relax_om, relax_tol;
+  int32_t* ring; /* (8, nnd) 0-based stencil nodes, -1: node not relaxed */
+  int32_t relax_plant; /* tests only: a deliberate mistake in the stencil */
   int32_t nreg;
   double* kappareg;
   double* pmeritreg;
@@ -139,6 +144,7 @@ void orc_destroy(orc_state_t* s) {
   for (int id = 1; id <= BL_NFIELD; ++id) free(s->f[id]);
   free(s->kappareg); free(s->pmeritreg); free(s->zdtnotreg);
   free(s->zmidlength); free(s->eos_itype); free(s->eos_param);
+  free(s->ring);
   free(s);
 }
 
@@ -1534,9 +1540,184 @@ int orc_ale_active(orc_state_t* s, const bl_timestep_t* ts) {
   return (ts->time >= s->cfg.mintime) && (ts->time <= s->cfg.maxtime);
 }
 
-/* ale_dr_getmeshstatus, ale_dr_getmeshstatus.f90:32-49 */
+/* ---- node relaxation ------------------------------------------------ */
+
+/* The 8-node stencil of each node, in the order of the weights r1..r8 of
+ * setup_mesh_region (setup_mesh.F90:848-857):
+ *   (lp,km) (lp,kk) (lp,kp) (ll,km) (ll,kp) (lm,km) (lm,kk) (lm,kp).
+ * Frame: e0 is the lowest-numbered element with n at corner 1, else the
+ * lowest-numbered element around n; with n at corner j of e0, +k points to
+ * corner j+1 and +l to corner j+3 (counter-clockwise corners).  Walking
+ * counter-clockwise around n from +k: the next element e' after e has its
+ * corner after n equal to e's corner before n. */
+static void build_ring(orc_state_t* s) {
+  int nel = s->sz.nel, nnd = s->sz.nnd;
+  const int32_t *ielnd = I(BL_IELND), *indtype = I(BL_INDTYPE);
+  free(s->ring);
+  s->ring = (int32_t*)malloc(sizeof(int32_t) * 8 * (size_t)(nnd > 0 ? nnd : 1));
+  for (int64_t k = 0; k < 8 * (int64_t)nnd; ++k) s->ring[k] = -1;
+  int* cnt = (int*)calloc(nnd + 1, sizeof(int));
+  for (int64_t k = 0; k < 4 * (int64_t)nel; ++k) cnt[ielnd[k] - 1] += 1;
+  int* start = (int*)calloc(nnd + 1, sizeof(int));
+  for (int n = 0; n < nnd; ++n) start[n + 1] = start[n] + cnt[n];
+  int* fill = (int*)malloc(sizeof(int) * (nnd + 1));
+  memcpy(fill, start, sizeof(int) * (nnd + 1));
+  int* lst = (int*)malloc(sizeof(int) * 4 * (size_t)(nel > 0 ? nel : 1));
+  for (int e = 0; e < nel; ++e) /* ascending element order per node */
+    for (int j = 0; j < 4; ++j) {
+      int n = ielnd[4 * (int64_t)e + j] - 1;
+      lst[fill[n]++] = 4 * e + j;
+    }
+  for (int n = 0; n < nnd; ++n) {
+    if (indtype[n] <= 0 || start[n + 1] - start[n] != 4) continue;
+    int c0 = lst[start[n]];
+    for (int k = start[n]; k < start[n + 1]; ++k)
+      if ((lst[k] & 3) == 0) { c0 = lst[k]; break; }
+#define NODE_AT(c, d) (ielnd[4 * (int64_t)((c) >> 2) + (((c) + (d)) & 3)] - 1)
+    int P[8];
+    int used[4] = {c0, -1, -1, -1};
+    P[0] = NODE_AT(c0, 1);
+    P[1] = NODE_AT(c0, 2);
+    P[2] = NODE_AT(c0, 3);
+    int ok = 1, cur = c0;
+    for (int step = 1; step < 4 && ok; ++step) {
+      int prev = NODE_AT(cur, 3), nxt = -1;
+      for (int k = start[n]; k < start[n + 1]; ++k) {
+        int c = lst[k];
+        if (c == used[0] || c == used[1] || c == used[2]) continue;
+        if (NODE_AT(c, 1) == prev) { nxt = c; break; }
+      }
+      if (nxt < 0) { ok = 0; break; }
+      used[step] = nxt;
+      cur = nxt;
+      P[2 * step + 1] = NODE_AT(cur, 2);
+      if (step < 3) P[2 * step + 2] = NODE_AT(cur, 3);
+    }
+    if (!ok || NODE_AT(cur, 3) != P[0]) continue; /* ring not closed */
+#undef NODE_AT
+    /* P: counter-clockwise from +k: (ll,kp) (lp,kp) (lp,kk) (lp,km)
+     * (ll,km) (lm,km) (lm,kk) (lm,kp) */
+    int32_t* r = s->ring + 8 * (int64_t)n;
+    r[0] = P[3]; r[1] = P[2]; r[2] = P[1]; r[3] = P[4];
+    r[4] = P[0]; r[5] = P[5]; r[6] = P[6]; r[7] = P[7];
+  }
+  free(cnt); free(start); free(fill); free(lst);
+}
+
+/* one node of one sweep (setup_mesh.F90:797-863): x (ss) and y (rr) read
+ * from (xi, yi), the result written to (xo, yo) */
+static void relax_node(int type, double om, double tol, double zerocut, int plant,
+                       const int32_t* nb, const double* xi, const double* yi,
+                       int n, double* xo, double* yo) {
+  double xn = xi[n], yn = yi[n];
+  xo[n] = xn;
+  yo[n] = yn;
+  if (nb[0] < 0) return;
+  double r1, r2, r3, r4, r5, r6, r7, r8;
+  if (type == BL_RELAX_EQUI) {
+    /* (ll,kp) is nb[4], (ll,km) nb[3], (lp,kk) nb[1], (lm,kk) nb[6] */
+    double r_psi = 0.5 * (yi[nb[4]] - yi[nb[3]]);
+    double s_psi = 0.5 * (xi[nb[4]] - xi[nb[3]]);
+    double r_phi = 0.5 * (yi[nb[1]] - yi[nb[6]]);
+    double s_phi = 0.5 * (xi[nb[1]] - xi[nb[6]]);
+    double zeta = s_psi * r_phi - s_phi * r_psi;
+    if (fabs(zeta) < zerocut) return;
+    double alph = s_psi * s_psi + r_psi * r_psi;
+    double beta = s_phi * s_psi + r_phi * r_psi;
+    double gamm = s_phi * s_phi + r_phi * r_phi;
+    if (plant == 1) { double t = alph; alph = gamm; gamm = t; }
+    if (plant == 2) beta = -beta;
+    r1 = 0.5 * beta; r2 = alph; r3 = -r1; r4 = gamm;
+    r5 = r4; r6 = r3; r7 = r2; r8 = r1;
+    double fac = r1 + r2 + r3 + r4 + r5 + r6 + r7 + r8;
+    if (fac < 1.0e-24) {
+      fac = fac * 1.0e14;
+      r1 = r1 * 1.0e14; r2 = r2 * 1.0e14; r3 = r3 * 1.0e14; r4 = r4 * 1.0e14;
+      r5 = r5 * 1.0e14; r6 = r6 * 1.0e14; r7 = r7 * 1.0e14; r8 = r8 * 1.0e14;
+    }
+    r1 = r1 / fac; r2 = r2 / fac; r3 = r3 / fac; r4 = r4 / fac;
+    r5 = r5 / fac; r6 = r6 / fac; r7 = r7 / fac; r8 = r8 / fac;
+  } else {
+    /* LIN1: r_wgt(2) = r_wgt(7) = 1, LIN2: r_wgt(4) = r_wgt(5) = 1, each
+     * divided by their sum (setup_mesh.F90:626-660) */
+    double w = 1.0 / 2.0;
+    r1 = 0.0; r2 = 0.0; r3 = 0.0; r4 = 0.0; r5 = 0.0; r6 = 0.0; r7 = 0.0; r8 = 0.0;
+    if (type == BL_RELAX_LIN1) { r2 = w; r7 = w; } else { r4 = w; r5 = w; }
+  }
+  /* s_i = r_i for every type here */
+  double dr = r1 * yi[nb[0]] + r2 * yi[nb[1]] + r3 * yi[nb[2]] + r4 * yi[nb[3]] +
+              r5 * yi[nb[4]] + r6 * yi[nb[5]] + r7 * yi[nb[6]] + r8 * yi[nb[7]] - yn;
+  double ds = r1 * xi[nb[0]] + r2 * xi[nb[1]] + r3 * xi[nb[2]] + r4 * xi[nb[3]] +
+              r5 * xi[nb[4]] + r6 * xi[nb[5]] + r7 * xi[nb[6]] + r8 * xi[nb[7]] - xn;
+  if ((fabs(dr) > tol) || (fabs(ds) > tol)) {
+    yo[n] = yn + om * dr;
+    xo[n] = xn + om * ds;
+  }
+}
+
+/* nsweep Jacobi sweeps from (x, y) into (xr, yr) */
+static void relax_sweeps(orc_state_t* s, int type, int nsweep, double om, double tol,
+                         const double* x, const double* y, double* xr, double* yr) {
+  int nnd = s->sz.nnd;
+  if (!s->ring) build_ring(s);
+  double* tx = (double*)malloc(sizeof(double) * (nnd > 0 ? nnd : 1));
+  double* ty = (double*)malloc(sizeof(double) * (nnd > 0 ? nnd : 1));
+  memcpy(xr, x, sizeof(double) * nnd);
+  memcpy(yr, y, sizeof(double) * nnd);
+  for (int k = 0; k < nsweep; ++k) {
+    memcpy(tx, xr, sizeof(double) * nnd);
+    memcpy(ty, yr, sizeof(double) * nnd);
+    for (int n = 0; n < nnd; ++n)
+      relax_node(type, om, tol, s->cfg.zerocut, s->relax_plant, s->ring + 8 * (int64_t)n, tx,
+                 ty, n, xr, yr);
+  }
+  free(tx); free(ty);
+}
+
+static int relax_args_ok(int32_t type, int32_t nsweep, double om, double tol) {
+  return type >= BL_RELAX_OFF && type <= BL_RELAX_EQUI && nsweep >= 1 && om > 0.0 &&
+         tol >= 0.0;
+}
+
+int orc_set_relaxation(orc_state_t* s, int32_t type, int32_t nsweep, double om,
+                       double tol) {
+  if (!relax_args_ok(type, nsweep, om, tol)) return BL_FAILURE;
+  s->relax_type = type;
+  s->relax_nsweep = nsweep;
+  s->relax_om = om;
+  s->relax_tol = tol;
+  build_ring(s);
+  return BL_SUCCESS;
+}
+
+int orc_relax_mesh(orc_state_t* s, int32_t type, int32_t nsweep, double om, double tol) {
+  if (!relax_args_ok(type, nsweep, om, tol) || type == BL_RELAX_OFF) return BL_FAILURE;
+  int nnd = s->sz.nnd;
+  build_ring(s);
+  double* xr = (double*)malloc(sizeof(double) * (nnd > 0 ? nnd : 1));
+  double* yr = (double*)malloc(sizeof(double) * (nnd > 0 ? nnd : 1));
+  relax_sweeps(s, type, nsweep, om, tol, R(BL_NDX), R(BL_NDY), xr, yr);
+  memcpy(R(BL_NDX), xr, sizeof(double) * nnd);
+  memcpy(R(BL_NDY), yr, sizeof(double) * nnd);
+  free(xr); free(yr);
+  return BL_SUCCESS;
+}
+
+/* tests only: plant a mistake in the EQUI stencil (1: alpha and gamma
+ * swapped, 2: beta of the wrong sign; 0: none), to show that the tests of
+ * the stencil detect it */
+void orc_relax_plant(orc_state_t* s, int32_t which) { s->relax_plant = which; }
+
+/* the stencil table (8, nnd), for tests */
+const int32_t* orc_relax_ring(orc_state_t* s) {
+  if (!s->ring) build_ring(s);
+  return s->ring;
+}
+
+/* ale_dr_getmeshstatus, ale_dr_getmeshstatus.f90:32-49 (relaxation marks
+ * every node moving, as zeul does) */
 int orc_ale_getmeshstatus(orc_state_t* s) {
-  if (s->cfg.zeul) {
+  if (s->cfg.zeul || s->relax_type != BL_RELAX_OFF) {
     int32_t* st = I(BL_ISCRATCH11);
     for (int ind = 0; ind < s->sz.nnd; ++ind) st[ind] = 2;
   }
@@ -1547,11 +1728,24 @@ int orc_ale_getmeshstatus(orc_state_t* s) {
 int orc_ale_getfluxvolume(orc_state_t* s, const bl_timestep_t* ts) {
   int nnd = s->sz.nnd;
   double *st4 = R(BL_RSCRATCH14), *st5 = R(BL_RSCRATCH15);
-  if (s->cfg.zeul) /* ale_kn_getmeshvelocity.f90:28-46 */
+  if (s->cfg.zeul) { /* ale_kn_getmeshvelocity.f90:28-46 */
     for (int ind = 0; ind < nnd; ++ind) {
       st4[ind] = -st4[ind];
       st5[ind] = -st5[ind];
     }
+  } else if (s->relax_type != BL_RELAX_OFF) {
+    /* relaxation: w = (x_R - x)/dt from the Lagrangian positions */
+    double* xr = (double*)malloc(sizeof(double) * (nnd > 0 ? nnd : 1));
+    double* yr = (double*)malloc(sizeof(double) * (nnd > 0 ? nnd : 1));
+    relax_sweeps(s, s->relax_type, s->relax_nsweep, s->relax_om, s->relax_tol, R(BL_NDX),
+                 R(BL_NDY), xr, yr);
+    const double *x = R(BL_NDX), *y = R(BL_NDY);
+    for (int ind = 0; ind < nnd; ++ind) {
+      st4[ind] = (xr[ind] - x[ind]) / ts->dt;
+      st5[ind] = (yr[ind] - y[ind]) / ts->dt;
+    }
+    free(xr); free(yr);
+  }
   k_copy(nnd, R(BL_NDX), R(BL_RSCRATCH12));
   k_copy(nnd, R(BL_NDY), R(BL_RSCRATCH13));
   k_getvertex(nnd, ts->dt, R(BL_NDX), R(BL_NDY), st4, st5);

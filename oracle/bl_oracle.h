@@ -72,6 +72,13 @@ int orc_totals(orc_state_t* s, bl_totals_t* out);
 int orc_shortprint(orc_state_t* s, bl_flag_totals_t* region, bl_flag_totals_t* material);
 int orc_sod_l1(orc_state_t* s, double time, double x0, double scale, double* l1);
 
+/* node relaxation (bl_set_relaxation / bl_relax_mesh of the C-ABI) */
+int orc_set_relaxation(orc_state_t* s, int32_t type, int32_t nsweep, double om,
+                       double tol);
+int orc_relax_mesh(orc_state_t* s, int32_t type, int32_t nsweep, double om, double tol);
+const int32_t* orc_relax_ring(orc_state_t* s);
+void orc_relax_plant(orc_state_t* s, int32_t which);
+
 /* analytic Sod solution of check_kn_tests.f90:64-111 at (x,t) for the
  * reference's [0,100] tube; returns density and energy */
 void orc_sod_exact(double x, double t, double* rho, double* e);

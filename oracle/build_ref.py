@@ -4,7 +4,9 @@
 TEST INFRASTRUCTURE ONLY.  Compiles the serial (no MPI, no OpenMP, local
 mesh set-up) source set of the reference *where it lies* under
 /root/reference/src with AMD flang, in module-dependency order, and links it
-with oracle/ref/ref_driver.f90 into oracle/_ref/bookleaf_ref.  Nothing is
+with oracle/ref/ref_driver.f90 into oracle/_ref/bookleaf_ref, and with
+oracle/ref/ref_meshgen.f90 (the reference's mesh generator on a deck's &MESH
+namelist) into oracle/_ref/bookleaf_meshgen.  Nothing is
 copied out of the reference tree; outputs go only to oracle/_ref/.  (The
 reference's own deck reader cannot be used as shipped: under flang it aborts
 at setup_IC.f90:1282, where a PACK result is assigned to a longer character
@@ -120,7 +122,8 @@ def main():
     os.makedirs(objdir, exist_ok=True)
     os.makedirs(moddir, exist_ok=True)
     files = order(reference_sources())
-    hosts = {"bookleaf_ref": os.path.join(HERE, "ref", "ref_driver.f90")}
+    hosts = {"bookleaf_ref": os.path.join(HERE, "ref", "ref_driver.f90"),
+             "bookleaf_meshgen": os.path.join(HERE, "ref", "ref_meshgen.f90")}
 
     def compile_one(f):
         base = os.path.splitext(os.path.basename(f))[0]

@@ -49,6 +49,10 @@ def load_library():
         lib.orc_step.argtypes = [vp, P(abi.Timestep), P(abi.Error)]
         lib.orc_sod_exact.argtypes = [C.c_double, C.c_double, P(C.c_double), P(C.c_double)]
         lib.orc_sod_exact.restype = None
+        lib.orc_relax_ring.argtypes = [vp]
+        lib.orc_relax_ring.restype = P(C.c_int32)
+        lib.orc_relax_plant.argtypes = [vp, C.c_int32]
+        lib.orc_relax_plant.restype = None
         _LIB = lib
     return _LIB
 
@@ -88,6 +92,17 @@ class Oracle(DriverState):
     def set(self, name, values):
         v = self.view(name)
         v[:len(values)] = np.asarray(values, v.dtype)
+
+    def relax_ring(self):
+        """The relaxation stencil table, (nnd, 8) 0-based node ids, -1 for
+        nodes that are not relaxed."""
+        nnd = self.sizes.nnd
+        p = self.lib.orc_relax_ring(self.h)
+        return np.ctypeslib.as_array(p, shape=(8 * nnd,)).reshape(nnd, 8).copy()
+
+    def relax_plant(self, which):
+        """Tests only: plant a mistake in the oracle's EQUI stencil."""
+        self.lib.orc_relax_plant(self.h, which)
 
     def close(self):
         if getattr(self, "h", None):

@@ -18,7 +18,15 @@ MIXED = ("imxel", "imxfcp", "imxncp", "icpmat", "icpnext", "icpprev", "frvolume"
 
 
 def names():
-    return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """The step records (make_golden.py); meshgen_*.npz are the mesh
+    generator's (make_meshgen.py)."""
+    return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(f).startswith("meshgen_"))
+
+
+def load_meshgen(name):
+    z = np.load(os.path.join(GOLDEN, "meshgen_%s.npz" % name))  # plain arrays, no pickle
+    return {k: z[k] for k in z.files}
 
 
 def load(name):

@@ -63,6 +63,10 @@ def test_multistep_dt_sequence_and_state(name, refio, oracle_lib):
     assert np.array_equal(idt, ref_idt)
     assert [abi.DT_NAMES[c] for c in ctl] == ref_ctl
     assert diff_fields(o.state(), dumps[-1], ncp=p.ncp) == []
+    # hydro_dr_shortprint's tables from the reference's own kernels
+    reg, mat = o.shortprint()
+    assert np.array_equal(reg, dumps[-1]["shortprint"]["region"])
+    assert np.array_equal(mat, dumps[-1]["shortprint"]["material"])
 
 
 def test_cfl_and_div_controls_are_exercised(oracle_lib):
