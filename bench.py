@@ -57,6 +57,27 @@ KERNEL_BYTES = {
     # ielnd 16, ielmat 4, m, e 16, corner force x/y 64; writes e, rho, p,
     # cs2 32 | x,y 16, ubar,vbar 16
     "f4_state": (132, 32),
+    # remap (ALE step), per element | per node:
+    # ielnd, ielel, ielfc 48, cnwt 32, rho 8; writes dv 8, face mass flux 32
+    # | x, y, mesh velocity 32
+    "r1_volflux": (128, 32),
+    # ielel, ielfc 32, face mass flux 32, V, m 16, cnmass 32, e 8; writes
+    # v0, m0, m 24, face energy flux 32
+    "r2_massflux": (176, 0),
+    # ielnd, ielel, ielfc 48, energy flux 32, m, v0, m0 24, e r/w 16, cnmass
+    # r/w 64, face mass flux 32, ielmat 4; writes corner u, v, m0, dm,
+    # mass flux, m0 (corner-major) 192, rho, p, cs2 24 | u, v, x, y, mesh
+    # velocity 48
+    "r3_energy_update": (444, 48),
+    # ielel, ielfc 32, corner m0, dm, u, v 128; writes corner fluxes 64
+    "r4_momflux": (224, 0),
+    # v0 8, corner m0, mass flux 64, corner fluxes u, v 64 | CSR ptr/idx
+    # (two orders) 40, indtype 4, u, v, x, y r/w 64, mesh velocity 16
+    "r5_nodes": (136, 124),
+    # node relaxation sweeps: stencil table 32, x, y 16, writes 16 | the last
+    # sweep also reads the Lagrangian x, y 16
+    "relax_sweep": (0, 64),
+    "relax_velocity": (0, 80),
 }
 # per Lagrangian step at the driver boundary: 1664 B/element + 272 B/node,
 # +84 B/element with the anti-hourglass force on
@@ -73,7 +94,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--runner", choices=["run", "run_graph"], default="run_graph")
     ap.add_argument("--n", type=int, default=1024, help="mesh is n x n (sedov)")
-    ap.add_argument("--workload", choices=["sedov", "sod2mat"], default="sedov",
+    ap.add_argument("--workload", choices=["sedov", "sod2mat", "relax"], default="sedov",
                     help="sedov: Lagrangian 1024^2 Sedov (the headline); sod2mat: Eulerian "
                          "(ALE remap every step) two-material Sod on 2048x256 with mixed "
                          "interface cells")
@@ -90,7 +111,8 @@ def parse():
 
 def cpu_baseline(prob, steps):
     from oracle import refio
-    if refio.available():
+    # the reference has no node-relaxation mesh motion: the C oracle runs it
+    if refio.available() and not prob.relax:
         with tempfile.TemporaryDirectory() as d:
             fin, fout = os.path.join(d, "p.bin"), os.path.join(d, "o.bin")
             refio.write_problem(prob, fin, nsteps=steps)
@@ -128,10 +150,17 @@ def main():
         prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps, kappa=KAPPA)
         wname = "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_%dsteps" % (
             args.n, args.n, args.warmup + args.steps)
-    else:
+    elif args.workload == "sod2mat":
         prob = P.sod2mat_mixed(nx=2048, ny=256, nsteps=args.warmup + args.steps)
         wname = "sod2mat_eulerian_2048x256_mixed_interface_seed2024_%dsteps" % (
             args.warmup + args.steps)
+    else:
+        from bookleaf_f90_amd import abi
+        prob = P.with_relaxation(
+            P.sod2mat_mixed(nx=1024, ny=512, nsteps=args.warmup + args.steps, eulerian=False,
+                            jitter=0.1), abi.RELAX_EQUI, nsweep=2, om=0.5)
+        wname = ("sod2mat_ale_equi_relaxation_1024x512_mixed_interface_jitter0.1_seed2024_"
+                 "nsweep2_om0.5_%dsteps" % (args.warmup + args.steps))
     nel, nnd = prob.nel, prob.nnd
     g = BookLeaf(prob)
     print("# workload %s nel=%d nnd=%d" % (wname, nel, nnd), file=sys.stderr, flush=True)
@@ -182,7 +211,10 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(prob, args.cpu_steps)
     dbe, dbn = DRIVER_BOUNDARY if args.workload == "sedov" else DRIVER_BOUNDARY_EUL
     step_bytes = dbe * nel + dbn * nnd
-    fused_bytes = sum(be * nel + bn * nnd for be, bn in KERNEL_BYTES.values())
+    # algorithmic bytes of the kernels one step launches (from the profile pass)
+    fused_bytes = sum((KERNEL_BYTES[k][0] * nel + KERNEL_BYTES[k][1] * nnd) *
+                      kern[k]["launches"] / max(args.profile_steps, 1)
+                      for k in kern if k in KERNEL_BYTES)
     out = {
         "metric": "element-steps/s FP64 at 1M cells, %HBM peak; learned vs plain ALE at "
                   "equal L1(rho)",

@@ -235,46 +235,69 @@ __global__ void k_l1(int nel, const int32_t* __restrict__ ielnd, const double* _
   block_flag_reduce(1, 0, r, part);
 }
 
-// one workgroup: part[b][f][q] over b in order -> out[f][q]
-__global__ void k_flag_final(int nb, int nflag, const double* __restrict__ part,
-                             double* __restrict__ out) {
-  for (int i = threadIdx.x; i < nflag * kNq; i += blockDim.x) {
-    const int q = i % kNq;
-    double a = q_identity(q);
-    for (int b = 0; b < nb; ++b) a = q_combine(q, a, part[(size_t)b * nflag * kNq + i]);
-    out[i] = a;
+// one column of the per-workgroup partials (part[b * stride + off], b <
+// nb) reduced by one workgroup: each thread combines b = t, t + kBlock, ...
+// in order, then a fixed pairwise tree over the threads in LDS.  The result
+// depends only on nb and the data.  Every thread of the workgroup must call
+// it; all get the result.
+__device__ double column_reduce(int nb, size_t stride, size_t off, int q,
+                                const double* __restrict__ part) {
+  __shared__ double sh[kBlock];
+  const int t = threadIdx.x;
+  double a = q_identity(q);
+  for (int b = t; b < nb; b += kBlock) a = q_combine(q, a, part[(size_t)b * stride + off]);
+  sh[t] = a;
+  __syncthreads();
+#pragma unroll
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = q_combine(q, sh[t], sh[t + s]);
+    __syncthreads();
   }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
 }
 
-// one workgroup: the history record of this step, [nstep, time, region
-// table (nreg x 9), L1 rho, L1 e], from the partials of k_sp_elements
-// (part) and k_l1 (part2); then the record counter
-__global__ void k_diag_record(int nb, int nreg, const double* __restrict__ part,
-                              const double* __restrict__ part2, double* __restrict__ hist,
-                              int32_t* __restrict__ counter, int maxrec,
-                              const DevTimestep* ts) {
+// one workgroup per column i of an (nflag x kNq) table: out[i]
+__global__ void __launch_bounds__(kBlock) k_flag_final(int nb, int nflag,
+                                                       const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  const int i = blockIdx.x;
+  const double r = column_reduce(nb, (size_t)nflag * kNq, i, i % kNq, part);
+  if (threadIdx.x == 0) out[i] = r;
+}
+
+// the columns of this step's record: nreg x kNq region columns from the
+// partials of k_sp_elements (part), then the three L1 sums from k_l1's
+// (part2): one workgroup per column, into col
+__global__ void __launch_bounds__(kBlock) k_diag_columns(int nb, int nreg,
+                                                         const double* __restrict__ part,
+                                                         const double* __restrict__ part2,
+                                                         double* __restrict__ col,
+                                                         const DevTimestep* ts) {
+  if (ts->skip) return;
+  const int c = blockIdx.x, nt = nreg * kNq;
+  const double r = c < nt ? column_reduce(nb, nt, c, c % kNq, part)
+                          : column_reduce(nb, kNq, c - nt, 0, part2);
+  if (threadIdx.x == 0) col[c] = r;
+}
+
+// the history record of this step, [nstep, time, region table (nreg x 9),
+// L1 rho, L1 e], from the columns; then the record counter
+__global__ void k_diag_record(int nreg, const double* __restrict__ col,
+                              double* __restrict__ hist, int32_t* __restrict__ counter,
+                              int maxrec, const DevTimestep* ts) {
   if (ts->skip) return;
   const int k = *counter;
   if (k >= maxrec) return;
-  const int rec_len = 2 + kNq * nreg + 2;
+  const int nt = nreg * kNq, rec_len = 2 + nt + 2;
   double* dst = hist + (size_t)k * rec_len;
-  for (int i = threadIdx.x; i < nreg * kNq; i += blockDim.x) {
-    const int q = i % kNq;
-    double a = q_identity(q);
-    for (int b = 0; b < nb; ++b) a = q_combine(q, a, part[(size_t)b * nreg * kNq + i]);
-    dst[2 + i] = a;
-  }
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) dst[2 + i] = col[i];
   if (threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int b = 0; b < nb; ++b) {
-      s0 = s0 + part2[(size_t)b * kNq];
-      s1 = s1 + part2[(size_t)b * kNq + 1];
-      s2 = s2 + part2[(size_t)b * kNq + 2];
-    }
     dst[0] = (double)ts->nstep;
     dst[1] = ts->time;
-    dst[2 + kNq * nreg] = s1 / s0;
-    dst[3 + kNq * nreg] = s2 / s0;
+    dst[2 + nt] = col[nt + 1] / col[nt];
+    dst[3 + nt] = col[nt + 2] / col[nt];
   }
   __syncthreads();
   if (threadIdx.x == 0) *counter = k + 1;
@@ -308,8 +331,8 @@ double* diag_part(bl_handle_t* h, size_t n) {
 // device), then to out on the host
 int final_to_host(bl_handle_t* h, int nflag, unsigned nb, const double* part, double* dout,
                   double* out) {
-  hipLaunchKernelGGL(k_flag_final, dim3(1), dim3(kBlock), 0, h->stream, (int)nb, nflag, part,
-                     dout);
+  hipLaunchKernelGGL(k_flag_final, dim3(nflag * kNq), dim3(kBlock), 0, h->stream, (int)nb,
+                     nflag, part, dout);
   hipError_t e0 = hipGetLastError();
   hipError_t e1 = hipMemcpyAsync(out, dout, sizeof(double) * kNq * nflag,
                                  hipMemcpyDeviceToHost, h->stream);
@@ -343,9 +366,11 @@ int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* n
             (const double*)nullptr, (const double*)h->R(BL_ELDENSITY),
             (const double*)h->R(BL_ELENERGY), h->diag_x0, h->diag_scale,
             (const double*)&h->d_ts->time, 0.0, part2, (const DevTimestep*)h->d_ts);
-  BL_LAUNCH(h, "diag_record", k_diag_record, 1, kBlock, (int)nb, nreg, (const double*)part,
-            (const double*)part2, h->d_diag_hist, h->d_diag_count, h->diag_max,
-            (const DevTimestep*)h->d_ts);
+  double* col = part2 + (size_t)nb * kNq;
+  BL_LAUNCH(h, "diag_columns", k_diag_columns, nreg * kNq + 3, kBlock, (int)nb, nreg,
+            (const double*)part, (const double*)part2, col, (const DevTimestep*)h->d_ts);
+  BL_LAUNCH(h, "diag_record", k_diag_record, 1, 64, nreg, (const double*)col,
+            h->d_diag_hist, h->d_diag_count, h->diag_max, (const DevTimestep*)h->d_ts);
   return BL_SUCCESS;
 }
 
@@ -431,7 +456,7 @@ int bl_set_diagnostics(bl_handle_t* h, int32_t on, double x0, double scale, int3
   }
   const int nreg = h->nreg;
   const unsigned nb = nblocks(h->sz.nel);
-  if (!diag_part(h, (size_t)nb * (nreg + 1) * kNq)) return BL_FAILURE;
+  if (!diag_part(h, (size_t)nb * (nreg + 1) * kNq + (size_t)nreg * kNq + 3)) return BL_FAILURE;
   if (h->d_diag_hist) hipFree(h->d_diag_hist);
   h->d_diag_hist = nullptr;
   const size_t rec = 2 + kNq * (size_t)nreg + 2;

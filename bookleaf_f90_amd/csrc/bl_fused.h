@@ -27,6 +27,8 @@ struct FusedArgs {
   int reg_tables, reg_tables_force, zsp, zhg, zexist, write_geometry;
   double zcut, zerocut, dencut, accut2, cvisc1, cvisc2, cfl_sf, div_sf, ccut, pcut;
   double dt_g, dt_min, dt_max, dt_initial, mintime, maxtime;
+  double time_end;        // with stop_at_time_end: steps from time_end on do nothing
+  int stop_at_time_end;
   // connectivity
   const int32_t* ielnd;
   const int32_t* ielel;
@@ -78,12 +80,16 @@ struct FusedArgs {
   // reductions and device timestep
   double* part;
   int32_t* parti;
-  uint32_t* ticket;
+  int32_t* work;          // F1's worklist of elements whose nodes move unequally
+  uint32_t* work_count;   // its length (reset by F1b)
   DtResult* dtres;
   DevTimestep* ts;
   double* hist;
   int hist_cap;
   MixArgs mx;  // mixed cells (mx.ncp == 0: none)
 };
+
+// F1L's fixed grid: enough workgroups to fill the chip at F1L's occupancy
+constexpr unsigned kLimiterBlocks = 512;
 
 }  // namespace bl

@@ -48,18 +48,12 @@
 
 namespace bl {
 
-// ---------------------------------------------------------------- F0
-__global__ void k_f0_begin(DevTimestep* __restrict__ ts, int stop_at_time_end,
-                           double time_end, DtResult* __restrict__ res) {
-  if (threadIdx.x != 0) return;
-  int skip = ts->done;
-  if (!skip && stop_at_time_end && ts->time >= time_end) {
-    ts->done = 1;
-    skip = 1;
-  }
-  ts->skip = skip;
-  res->negvol = 0;
-  res->negvol2 = 0;
+// whether this step does nothing: the batch is done (an error, or time_end
+// reached), or stop_at_time_end and time >= time_end (inf_control_solver's
+// loop test).  F1 and F1b decide it from the timestep as the step begins;
+// F1b then records it in ts->skip for the kernels that follow.
+__device__ __forceinline__ bool step_skipped(const FusedArgs& A, const DevTimestep* ts) {
+  return ts->done || (A.stop_at_time_end && ts->time >= A.time_end);
 }
 
 __device__ __forceinline__ void load4(const double* __restrict__ a, const int nd[4],
@@ -88,91 +82,128 @@ __device__ __forceinline__ void edge_nodes(const FusedNodes& N, int a, int b, do
 }
 
 // ---------------------------------------------------------------- F1
-// The four neighbour edges of each limiter direction come from the outer-node
-// table: one end is a node of e, already in registers, the other is the
-// neighbour's far node (a mesh without that structure runs host-driven)
+// The limiter (hydro_kn_limiteviscosity) and edge viscosity
+// (hydro_kn_geteviscosity) of element e: corner forces (qx, qy) and elvisc
+// (ev, 0.0 on entry).  The four neighbour edges of each limiter direction
+// come from the outer-node table: one end is a node of e, already in
+// registers, the other is the neighbour's far node (a mesh without that
+// structure runs host-driven).
+__device__ __forceinline__ void f1_viscosity(const FusedArgs& A, const FusedNodes& N, int e,
+                                             const int nd[4], const double x[4],
+                                             const double y[4], const double u[4],
+                                             const double v[4], double cs2e, double rhoe,
+                                             double& ev, double qx[4], double qy[4]) {
+  const int4 nb4 = reinterpret_cast<const int4*>(A.ielel)[e];
+  const int nb[4] = {nb4.x - 1, nb4.y - 1, nb4.z - 1, nb4.w - 1};
+  const int4 o0 = reinterpret_cast<const int4*>(A.outer)[2 * e];
+  const int4 o1 = reinterpret_cast<const int4*>(A.outer)[2 * e + 1];
+  const int on[8] = {o0.x - 1, o0.y - 1, o0.z - 1, o0.w - 1,
+                     o1.x - 1, o1.y - 1, o1.z - 1, o1.w - 1};
+  double ou[4], ov[4], ox[4], oy[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    int g = (f + 1) & 3;
+    ou[f] = u[g] - u[f]; ov[f] = v[g] - v[f]; ox[f] = x[g] - x[f]; oy[f] = y[g] - y[f];
+  }
+  double ed[4];
+#pragma unroll
+  for (int iside = 0; iside < 2; ++iside) {
+    const int is1 = (iside + 3) & 3, is2 = iside + 1;
+    const int in1 = nb[iside], in2 = nb[iside + 2];
+    double s1, s2, s3, s4;
+    double au, av, ax, ay, bu, bv, bx, by;
+    const int* o = on + 4 * iside;
+    // in1 edge f1+1, in2 edge f2+3
+    edge_nodes(N, nd[iside], o[0], au, av, ax, ay);
+    edge_nodes(N, o[2], nd[(iside + 3) & 3], bu, bv, bx, by);
+    limiter_ratios(ou[is1], ov[is1], ox[is1], oy[is1], au, av, ax, ay, bu, bv, bx, by,
+                   A.zerocut, s1, s2);
+    // in1 edge f1+3, in2 edge f2+1
+    edge_nodes(N, o[1], nd[iside + 1], au, av, ax, ay);
+    edge_nodes(N, nd[iside + 2], o[3], bu, bv, bx, by);
+    limiter_ratios(ou[is2], ov[is2], ox[is2], oy[is2], au, av, ax, ay, bu, bv, bx, by,
+                   A.zerocut, s3, s4);
+    if (in1 == e) {
+      int ic1 = nd[iside], ic2 = nd[(iside + 1) & 3];
+      if (A.indtype[ic1] < 0 && A.indtype[ic2] < 0 && in2 != e) { s1 = 1.0; s3 = 1.0; }
+      else { s1 = 0.0; s3 = 0.0; }
+    }
+    if (in2 == e) {
+      int ic1 = nd[iside + 2], ic2 = nd[(iside + 3) & 3];
+      if (A.indtype[ic1] < 0 && A.indtype[ic2] < 0 && in1 != e) { s2 = 1.0; s4 = 1.0; }
+      else { s2 = 0.0; s4 = 0.0; }
+    }
+    int ins = nb[is1];
+    ed[is1] = limiter_edge(s1, s2, ou[is1], ov[is1], cs2e, A.cs2[ins], rhoe, A.rho[ins],
+                           A.cvisc1, A.cvisc2, A.zerocut, ev);
+    ins = nb[is2];
+    ed[is2] = limiter_edge(s3, s4, ou[is2], ov[is2], cs2e, A.cs2[ins], rhoe, A.rho[ins],
+                           A.cvisc1, A.cvisc2, A.zerocut, ev);
+  }
+  geteviscosity(x, y, u, v, ed, ed, A.zerocut, qx, qy);
+}
+
+// the CFL term of e (hydro_kn_getdtcfl with cs2 + elvisc, hydro_dr_getcs2;
+// a mixed cell takes the frmass average of its components instead)
+__device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double x[4],
+                                         const double y[4], double cs2e, double ev) {
+  double c2 = cs2e + ev;
+  if (A.mx.ncp > 0) {
+    const int im = A.ielmat[e];
+    if (im < 0) c2 = mix_cs2(A.mx, -im - 1);
+  }
+  int dtnot = 0, mid = 0;
+  if (A.reg_tables) {
+    int ireg = A.ielreg[e] - 1;
+    dtnot = A.zdtnotreg[ireg];
+    mid = A.zmidlength[ireg];
+  }
+  return dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
+}
+
+// F1: every element.  Four nodes with one velocity make every own-edge
+// velocity difference zero, so geteviscosity zeroes every edge (|du| <=
+// zerocut) whatever the limiter gives, and each edge's elvisc term is
+// w3 * 0 = +0 (finite states): such an element gets its zero viscosity and
+// CFL term here.  Every other element is appended to the worklist for
+// F1L, which computes its viscosity and CFL term; its CFL slot here is
+// DBL_MAX.  The DIV term of every element is computed here.  The CFL and
+// DIV selections are first-index minima/maxima over (value, index) pairs,
+// so the split and the worklist order do not change the result.
 __global__ void __launch_bounds__(kBlock)
 k_f1_getdt(FusedArgs A, FusedNodes N) {
-  DevTimestep* ts = A.ts;
-  if (ts->skip) return;
+  if (step_skipped(A, A.ts)) return;
   const int e = xcd_index();
   double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false;
   if (e < A.nel) {
     int nd[4];
     load_nd(A.ielnd, e, nd);
     double x[4], y[4], u[4], v[4];
     load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
-    const int4 nb4 = reinterpret_cast<const int4*>(A.ielel)[e];
-    const int nb[4] = {nb4.x - 1, nb4.y - 1, nb4.z - 1, nb4.w - 1};
-    const int4 o0 = reinterpret_cast<const int4*>(A.outer)[2 * e];
-    const int4 o1 = reinterpret_cast<const int4*>(A.outer)[2 * e + 1];
-    const int on[8] = {o0.x - 1, o0.y - 1, o0.z - 1, o0.w - 1,
-                       o1.x - 1, o1.y - 1, o1.z - 1, o1.w - 1};
-    double ou[4], ov[4], ox[4], oy[4];
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      int g = (f + 1) & 3;
-      ou[f] = u[g] - u[f]; ov[f] = v[g] - v[f]; ox[f] = x[g] - x[f]; oy[f] = y[g] - y[f];
+    moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] &&
+               v[0] == v[2] && v[0] == v[3]);
+    if (!moving) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      st4(A.cnviscx, e, z);
+      st4(A.cnviscy, e, z);
+      A.elvisc[e] = 0.0;
+      cflv = f1_cfl(A, e, x, y, A.cs2[e], 0.0);
     }
-    const double cs2e = A.cs2[e], rhoe = A.rho[e];
-    double ev = 0.0, ed[4];
-#pragma unroll
-    for (int iside = 0; iside < 2; ++iside) {
-      const int is1 = (iside + 3) & 3, is2 = iside + 1;
-      const int in1 = nb[iside], in2 = nb[iside + 2];
-      double s1, s2, s3, s4;
-      double au, av, ax, ay, bu, bv, bx, by;
-      const int* o = on + 4 * iside;
-      // in1 edge f1+1, in2 edge f2+3
-      edge_nodes(N, nd[iside], o[0], au, av, ax, ay);
-      edge_nodes(N, o[2], nd[(iside + 3) & 3], bu, bv, bx, by);
-      limiter_ratios(ou[is1], ov[is1], ox[is1], oy[is1], au, av, ax, ay, bu, bv, bx, by,
-                     A.zerocut, s1, s2);
-      // in1 edge f1+3, in2 edge f2+1
-      edge_nodes(N, o[1], nd[iside + 1], au, av, ax, ay);
-      edge_nodes(N, nd[iside + 2], o[3], bu, bv, bx, by);
-      limiter_ratios(ou[is2], ov[is2], ox[is2], oy[is2], au, av, ax, ay, bu, bv, bx, by,
-                     A.zerocut, s3, s4);
-      if (in1 == e) {
-        int ic1 = nd[iside], ic2 = nd[(iside + 1) & 3];
-        if (A.indtype[ic1] < 0 && A.indtype[ic2] < 0 && in2 != e) { s1 = 1.0; s3 = 1.0; }
-        else { s1 = 0.0; s3 = 0.0; }
-      }
-      if (in2 == e) {
-        int ic1 = nd[iside + 2], ic2 = nd[(iside + 3) & 3];
-        if (A.indtype[ic1] < 0 && A.indtype[ic2] < 0 && in1 != e) { s2 = 1.0; s4 = 1.0; }
-        else { s2 = 0.0; s4 = 0.0; }
-      }
-      int ins = nb[is1];
-      ed[is1] = limiter_edge(s1, s2, ou[is1], ov[is1], cs2e, A.cs2[ins], rhoe, A.rho[ins],
-                             A.cvisc1, A.cvisc2, A.zerocut, ev);
-      ins = nb[is2];
-      ed[is2] = limiter_edge(s3, s4, ou[is2], ov[is2], cs2e, A.cs2[ins], rhoe, A.rho[ins],
-                             A.cvisc1, A.cvisc2, A.zerocut, ev);
-    }
-    double qx[4], qy[4];
-    geteviscosity(x, y, u, v, ed, ed, A.zerocut, qx, qy);
-    st4(A.cnviscx, e, qx);
-    st4(A.cnviscy, e, qy);
-    A.elvisc[e] = ev;
-    // CFL (hydro_kn_getdtcfl) with cs2 + elvisc (hydro_dr_getcs2); a mixed
-    // cell takes the frmass average of its components instead
-    double c2 = cs2e + ev;
-    if (A.mx.ncp > 0) {
-      const int im = A.ielmat[e];
-      if (im < 0) c2 = mix_cs2(A.mx, -im - 1);
-    }
-    int dtnot = 0, mid = 0;
-    if (A.reg_tables) {
-      int ireg = A.ielreg[e] - 1;
-      dtnot = A.zdtnotreg[ireg];
-      mid = A.zmidlength[ireg];
-    }
-    cflv = dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
     // divergence (hydro_kn_getdtdiv) from the geometry at x^n
     Iso g;
     getiso(x, y, g);
     divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
+  }
+  // worklist append, one atomic per wave
+  const uint64_t mask = __ballot(moving);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int lead = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(A.work_count, (uint32_t)__popcll(mask));
+    base = __shfl(base, lead, 64);
+    if (moving) A.work[base + __popcll(mask & ((1ull << lane) - 1ull))] = e;
   }
   const int nb = gridDim.x;
   block_argmax_first(divv, e + 1, A.part, A.parti);
@@ -180,12 +211,49 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
   block_argmin_first(cflv, e + 1, A.part + 2 * nb, A.parti + 2 * nb);
 }
 
+// F1L: the worklist's elements, a fixed grid striding over the count F1
+// left; the CFL partials of its workgroups follow F1's (part + 3 * nb1)
+__global__ void __launch_bounds__(kBlock)
+k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
+  if (step_skipped(A, A.ts)) return;
+  const uint32_t count = min(*A.work_count, (uint32_t)A.nel);
+  double best = DBL_MAX;
+  int bi = 0x7fffffff;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
+       k += gridDim.x * blockDim.x) {
+    const int e = A.work[k];
+    int nd[4];
+    load_nd(A.ielnd, e, nd);
+    double x[4], y[4], u[4], v[4];
+    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+    const double cs2e = A.cs2[e], rhoe = A.rho[e];
+    double ev = 0.0, qx[4], qy[4];
+    f1_viscosity(A, N, e, nd, x, y, u, v, cs2e, rhoe, ev, qx, qy);
+    st4(A.cnviscx, e, qx);
+    st4(A.cnviscy, e, qy);
+    A.elvisc[e] = ev;
+    double c = f1_cfl(A, e, x, y, cs2e, ev);
+    if (!(c == c)) c = DBL_MAX;  // a NaN never wins the strict '<' scan
+    better_min(best, bi, c, e + 1);
+  }
+  block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
+}
+
 // F1b: one workgroup -- combine the partials of F1, then the hydro_dr_getdt
 // candidates and time_dr_calc / time_dr_end (time_dr_advance.f90:38-132)
-__global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb) {
+__global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb, int nbl) {
   DevTimestep* ts = A.ts;
-  if (ts->skip) return;
-  final_dt_reduce(nb, nb, A.part, A.parti, A.dtres);
+  const bool skip = step_skipped(A, ts);
+  __syncthreads();  // every thread has read ts before thread 0 writes it
+  if (threadIdx.x == 0) {
+    if (skip) ts->done = 1;
+    ts->skip = skip ? 1 : 0;
+    A.dtres->negvol = 0;
+    A.dtres->negvol2 = 0;
+    *A.work_count = 0;  // F1L has read it
+  }
+  if (skip) return;
+  final_dt_reduce(nb, nb + nbl, A.part, A.parti, A.dtres);
   if (threadIdx.x != 0) return;
   const DtResult& r = *A.dtres;
   if (r.cfl_min < 0.0) {  // hydro_kn_getdt.f90:75-78
@@ -498,8 +566,9 @@ k_f4_state(FusedArgs A, FusedNodes O) {
 
 // ------------------------------------------------------- remap (Eulerian)
 // The unsplit remap of ale_dr_advect (adv_type = 1) with zeul or the node
-// relaxation's mesh velocity, in five grid kernels.  Intermediates live in the handle's own buffers (ale_tmp) rather
-// than the alestep scratch ids, which nothing reads after the step.  On a
+// relaxation's mesh velocity, in five grid kernels.  Intermediates live in
+// the handle's own buffers (ale_tmp) rather than the alestep scratch ids,
+// which nothing reads after the step.  On a
 // conforming mesh the neighbour's flux volume of a shared face is a function
 // of the two shared nodes only, so each element computes it itself with the
 // neighbour's argument order (bit-identical to the neighbour's own value).
@@ -791,7 +860,8 @@ FusedArgs make_args(bl_handle_t* h) {
   A.cnfx = h->R(BL_RSCRATCH23); A.cnfy = h->R(BL_RSCRATCH24);
   A.fxT = h->cterm[0]; A.fyT = h->cterm[1]; A.wtT = h->cterm[2]; A.mT = h->cterm[3];
   A.ubar = h->R(BL_RSCRATCH14); A.vbar = h->R(BL_RSCRATCH15);
-  A.part = h->d_part; A.parti = h->d_parti; A.ticket = h->d_ticket; A.dtres = h->d_dtres;
+  A.part = h->d_part; A.parti = h->d_parti; A.dtres = h->d_dtres;
+  A.work = h->d_work; A.work_count = h->d_ticket;
   A.ts = h->d_ts;
   A.hist = h->d_hist;
   A.hist_cap = (int)h->hist_cap;
@@ -831,16 +901,18 @@ AleArgs make_ale(bl_handle_t* h, const FusedNodesOut& O) {
 // one step (parity p: node set p -> node set 1-p), launched on h->stream
 int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedArgs A = make_args(h);
+  A.stop_at_time_end = stop_at_time_end;
+  A.time_end = h->cfg.time_end;
   const int nel = h->sz.nel, nnd = h->sz.nnd;
   const unsigned nbe = nblocks(nel), nbn = nblocks(nnd);
   FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
   FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
-  BL_LAUNCH(h, "f0_begin", k_f0_begin, 1, 64, h->d_ts, stop_at_time_end, h->cfg.time_end,
-            h->d_dtres);
+  const unsigned nbl = std::min(nbe, kLimiterBlocks);
   BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
-  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe);
+  BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, kBlock, A, N, (int)nbe);
+  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe, (int)nbl);
   BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
   if (A.mx.ncp > 0) BL_LAUNCH(h, "f2_mix", k_f2_mix, nblocks(A.mx.nmx), kBlock, A, N);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
@@ -866,6 +938,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
 }
 
 int ensure_nodesets(bl_handle_t* h) {
+  if (!h->d_work) BL_HIP(hipMalloc(&h->d_work, sizeof(int32_t) * std::max(h->sz.nel, 1)));
   if (!h->cterm[0])
     for (int k = 0; k < 4; ++k)
       BL_HIP(hipMalloc(&h->cterm[k], sizeof(double) * 4 * std::max(h->sz.nel, 1)));

@@ -112,6 +112,8 @@ struct Handle {
   // corrector corner terms of the fused step, corner-major (4*nel each):
   // force x, force y, half-step corner weight, nodal-mass term
   double* cterm[4] = {};
+  // F1's worklist (nel): elements whose nodes do not share one velocity
+  int32_t* d_work = nullptr;
   // remap temporaries of the fused step (zexist): 4*nel each for the face
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[13] = {};
