@@ -17,7 +17,7 @@ Reported next to the throughput:
                 KERNEL_BYTES) over its average duration, timed with HIP events
                 on the library's stream in a profile pass of the same run;
                 traffic is the rocprofv3 FETCH_SIZE+WRITE_SIZE of that kernel
-                from profiles/traffic_sedov1024.json when present
+                from profiles/traffic_<workload>.json when present
   cpu_baseline  the reference's own Fortran (oracle/_ref, flang -O2, serial)
                 timed on this host, one core, on the first --cpu-steps steps of
                 the same workload ("port": the C oracle when _ref is absent)
@@ -44,9 +44,11 @@ HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 KAPPA = 0.01       # anti-hourglass coefficient (input/sedov_lag_small kappaall)
 # algorithmic bytes per launch of each fused kernel: (bytes/element, bytes/node)
 KERNEL_BYTES = {
-    # ielnd, ielel 32, limiter stencil 32; rho, cs2 16; writes cnviscx/y 64,
-    # elvisc 8 | node x,y,u,v 32, indtype 4
-    "f1_getdt": (152, 36),
+    # every element: ielnd 16, cs2 8; writes cnviscx/y 64, elvisc 8 | node
+    # x,y,u,v 32.  The limiter's stencil (ielel, outer nodes, neighbour rho
+    # and cs2) is read by f1_limiter for the elements on F1's worklist only,
+    # a data-dependent count, so f1_limiter has no entry here
+    "f1_getdt": (96, 32),
     # ielnd 16, ielmat 4, ielreg 4 (hourglass kappa), p, m, e 24, cnviscx/y
     # 64, cnmass 32; writes corner force x/y, weight, mass term 128 |
     # node x,y,u,v 32
@@ -69,7 +71,8 @@ KERNEL_BYTES = {
     # mass flux, m0 (corner-major) 192, rho, p, cs2 24 | u, v, x, y, mesh
     # velocity 48
     "r3_energy_update": (444, 48),
-    # ielel, ielfc 32, corner m0, dm, u, v 128; writes corner fluxes 64
+    # ielel, ielfc 32, corner m0, dm, u, v 128; writes corner fluxes 64 (a
+    # corner with zero mass flux reads only its flux: at rest fewer bytes)
     "r4_momflux": (224, 0),
     # v0 8, corner m0, mass flux 64, corner fluxes u, v 64 | CSR ptr/idx
     # (two orders) 40, indtype 4, u, v, x, y r/w 64, mesh velocity 16
@@ -195,8 +198,8 @@ def main():
         alg = be * nel + bn * nnd
         avg_s = kern[dom]["avg_ms"] * 1e-3
         traffic = None
-        tf = os.path.join(ROOT, "profiles", "traffic_sedov1024.json")
-        if os.path.exists(tf) and args.workload == "sedov" and args.n == 1024:
+        tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+        if os.path.exists(tf) and args.n == 1024:
             traffic = json.load(open(tf)).get(dom)
         roof = {"kernel": dom, "bound": "hbm", "achieved": alg / avg_s / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg / avg_s / HBM_PEAK,
