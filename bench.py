@@ -59,6 +59,8 @@ KERNEL_BYTES = {
     # ielnd 16, ielmat 4, m, e 16, corner force x/y 64; writes e, rho, p,
     # cs2 32 | x,y 16, ubar,vbar 16
     "f4_state": (132, 32),
+    # ensemble F1: F1's bytes and the element instance id 2
+    "e1_getdt": (98, 32),
     # remap (ALE step), per element | per node:
     # ielnd, ielel, ielfc 48, cnwt 32, rho 8; writes dv 8, face mass flux 32
     # | x, y, mesh velocity 32
@@ -93,14 +95,19 @@ DRIVER_BOUNDARY_EUL = (1664 + 504, 272 + 108)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=490)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 490 for sedov, 190 for ensemble, 290 else)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--runner", choices=["run", "run_graph"], default="run_graph")
     ap.add_argument("--n", type=int, default=1024, help="mesh is n x n (sedov)")
-    ap.add_argument("--workload", choices=["sedov", "sod2mat", "relax"], default="sedov",
+    ap.add_argument("--workload", choices=["sedov", "sod2mat", "relax", "ensemble"],
+                    default="sedov",
                     help="sedov: Lagrangian 1024^2 Sedov (the headline); sod2mat: Eulerian "
                          "(ALE remap every step) two-material Sod on 2048x256 with mixed "
-                         "interface cells")
+                         "interface cells; relax: two-material Sod 1024x512 with ALE node "
+                         "relaxation; ensemble: 512 Sod/Sedov instances of 64^2..512^2 "
+                         "stepped together")
+    ap.add_argument("--instances", type=int, default=512, help="ensemble size")
     ap.add_argument("--diagnostics", action="store_true",
                     help="record the region table and Sod L1 on the device every step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -109,7 +116,10 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="launch the plain nel-double copy kernel (hydro_setpredictor) this "
                          "many times, to scale profiler byte counters")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"sedov": 490, "ensemble": 190}.get(args.workload, 290)
+    return args
 
 
 def cpu_baseline(prob, steps):
@@ -140,6 +150,39 @@ def cpu_baseline(prob, steps):
             % (steps, sec)}
 
 
+def cpu_baseline_ensemble(params, steps):
+    """The reference's Fortran (oracle/_ref, serial) on the ensemble's first
+    P instances, one process per instance, all P at once on P host cores;
+    the throughput is the sum of the processes' own element-steps/s."""
+    from bookleaf_f90_amd import problem as P
+    from oracle import refio
+    if not refio.available():
+        return None
+    nproc = min(16, os.cpu_count() or 1, len(params))
+    with tempfile.TemporaryDirectory() as d:
+        procs = []
+        for k in range(nproc):
+            fin, fout = os.path.join(d, "p%d.bin" % k), os.path.join(d, "o%d.bin" % k)
+            refio.write_problem(P.ensemble_instance(params[k]), fin, nsteps=steps)
+            procs.append(subprocess.Popen(["bash", "-c", 'ulimit -s unlimited; exec "$0" "$@"',
+                                           refio.REF_EXE, fin, fout, "2"],
+                                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                          text=True))
+        rates, nels = [], 0
+        for k, pr in enumerate(procs):
+            out, _ = pr.communicate(timeout=900)
+            m = re.search(r"elsteps_per_s=\s*(\S+)", out)
+            if pr.returncode != 0 or not m:
+                return None
+            rates.append(float(m.group(1)))
+            nels += params[k]["nx"] * params[k]["ny"]
+    return {"value": sum(rates), "unit": "element-steps/s", "cores": nproc, "kind": "reference",
+            "sample": "oracle/_ref (the reference's Fortran, flang -O2 -ffp-contract=off, "
+                      "serial), %d processes at once, one per instance 1-%d of the ensemble "
+                      "(%d elements together), steps 1-%d each; the sum of their own "
+                      "element-steps/s" % (nproc, nproc, nels, steps)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -147,9 +190,14 @@ def main():
               "GPU (see DESIGN.md); --gpus %d refused" % args.gpus)
         sys.exit(2)
     from bookleaf_f90_amd import problem as P
-    from bookleaf_f90_amd.device import BookLeaf
+    from bookleaf_f90_amd.device import BookLeaf, Ensemble
 
-    if args.workload == "sedov":
+    prob = None
+    if args.workload == "ensemble":
+        params = P.ensemble_params(args.instances, seed=2025)
+        wname = ("ensemble_%d_sod_sedov_instances_64^2..512^2_seed2025_%dsteps"
+                 % (args.instances, args.warmup + args.steps))
+    elif args.workload == "sedov":
         prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps, kappa=KAPPA)
         wname = "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_%dsteps" % (
             args.n, args.n, args.warmup + args.steps)
@@ -164,16 +212,22 @@ def main():
                             jitter=0.1), abi.RELAX_EQUI, nsweep=2, om=0.5)
         wname = ("sod2mat_ale_equi_relaxation_1024x512_mixed_interface_jitter0.1_seed2024_"
                  "nsweep2_om0.5_%dsteps" % (args.warmup + args.steps))
-    nel, nnd = prob.nel, prob.nnd
-    g = BookLeaf(prob)
+    if prob is None:
+        g = Ensemble(params)
+        nel, nnd = g.nel, g.nnd
+        run = g.run
+    else:
+        nel, nnd = prob.nel, prob.nnd
+        g = BookLeaf(prob)
+        run = lambda n: g.run(n, runner=args.runner)  # noqa: E731
     print("# workload %s nel=%d nnd=%d" % (wname, nel, nnd), file=sys.stderr, flush=True)
     if args.diagnostics:
         g.set_diagnostics(True, maxrec=args.warmup + args.steps + args.profile_steps)
     if args.warmup:
-        g.run(args.warmup, runner=args.runner)
+        run(args.warmup)
     g.synchronize()
     t0 = time.perf_counter()
-    g.run(args.steps, runner=args.runner)
+    run(args.steps)
     g.synchronize()
     sec = time.perf_counter() - t0
     value = nel * args.steps / sec
@@ -183,7 +237,7 @@ def main():
     # stream, over a further profile pass of the same run and kernels
     g.timing(True)
     g.timing_reset()
-    g.run(args.profile_steps, runner=args.runner)
+    run(args.profile_steps)
     kern = {}
     for name in g.timing_names():
         ms, n = g.timing_query(name)
@@ -211,8 +265,14 @@ def main():
     for _ in range(args.calibrate_copy):
         g.hydro_setpredictor()  # k_copy_r: reads and writes 8*nel bytes
     g.synchronize()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(prob, args.cpu_steps)
-    dbe, dbn = DRIVER_BOUNDARY if args.workload == "sedov" else DRIVER_BOUNDARY_EUL
+    if args.no_cpu_baseline:
+        cpu = None
+    elif prob is None:
+        cpu = cpu_baseline_ensemble(params, min(args.cpu_steps, 10))
+    else:
+        cpu = cpu_baseline(prob, args.cpu_steps)
+    dbe, dbn = (DRIVER_BOUNDARY if args.workload in ("sedov", "ensemble")
+                else DRIVER_BOUNDARY_EUL)
     step_bytes = dbe * nel + dbn * nnd
     # algorithmic bytes of the kernels one step launches (from the profile pass)
     fused_bytes = sum((KERNEL_BYTES[k][0] * nel + KERNEL_BYTES[k][1] * nnd) *

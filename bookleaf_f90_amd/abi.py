@@ -76,6 +76,15 @@ class Error(C.Structure):
         return bytes(self.serr).decode(errors="replace").strip()
 
 
+class Instance(C.Structure):
+    """bl_instance_t: one instance of an ensemble."""
+    _fields_ = [(n, C.c_int32) for n in "nx ny kind pad".split()] + \
+               [(n, C.c_double) for n in
+                "length height gamma kappa rho_l e_l rho_r e_r".split()]
+
+
+INSTANCE_SOD, INSTANCE_SEDOV = 0, 1
+
 # node relaxation types (setup_mesh's region types)
 RELAX_OFF, RELAX_LIN1, RELAX_LIN2, RELAX_EQUI = 0, 1, 2, 3
 

@@ -87,7 +87,40 @@ struct FusedArgs {
   double* hist;
   int hist_cap;
   MixArgs mx;  // mixed cells (mx.ncp == 0: none)
+  // ensembles (null otherwise): the instance of each element and node, and
+  // each instance's dt, which F2-F4 use in place of the timestep's
+  const uint16_t* einst;
+  const uint16_t* ninst;
+  const double* inst_dt;
 };
+
+// the timestep of one ensemble instance
+struct InstTs {
+  double time, dt;
+  int32_t nstep, idtel, sdt, pad;
+};
+
+// ensemble step state: instance element offsets (ninst + 1), per-instance
+// timesteps and worklist lengths, F1's two partial slots per workgroup
+// (DIV a, DIV b, CFL a, CFL b, each nb long: a workgroup spans at most two
+// instances) and F1L's partials (one per F1L workgroup), and the dt
+// history, (step, instance, 3)
+struct EnsArgs {
+  int ninst;
+  const int32_t* eoff;
+  InstTs* inst;
+  double* inst_dt;
+  uint32_t* cnt;
+  double* part;
+  int32_t* parti;
+  const int32_t* lboff;  // F1L workgroups of instance k: lboff[k] .. lboff[k+1]-1
+  double* lpart;
+  int32_t* lparti;
+  double* hist;
+  int hist_cap;
+};
+// F1L gives instance k ceil(nel_k / kEnsChunk) workgroups
+constexpr int kEnsChunk = 4096;
 
 // F1L's fixed grid: enough workgroups to fill the chip at F1L's occupancy
 constexpr unsigned kLimiterBlocks = 512;

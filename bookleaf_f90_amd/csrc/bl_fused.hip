@@ -211,6 +211,23 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
   block_argmin_first(cflv, e + 1, A.part + 2 * nb, A.parti + 2 * nb);
 }
 
+// viscosity, elvisc and CFL term of a worklist element (a NaN CFL term
+// becomes DBL_MAX: it never wins the strict '<' scan)
+__device__ __forceinline__ double f1_moving(const FusedArgs& A, const FusedNodes& N, int e) {
+  int nd[4];
+  load_nd(A.ielnd, e, nd);
+  double x[4], y[4], u[4], v[4];
+  load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+  const double cs2e = A.cs2[e], rhoe = A.rho[e];
+  double ev = 0.0, qx[4], qy[4];
+  f1_viscosity(A, N, e, nd, x, y, u, v, cs2e, rhoe, ev, qx, qy);
+  st4(A.cnviscx, e, qx);
+  st4(A.cnviscy, e, qy);
+  A.elvisc[e] = ev;
+  const double c = f1_cfl(A, e, x, y, cs2e, ev);
+  return (c == c) ? c : DBL_MAX;
+}
+
 // F1L: the worklist's elements, a fixed grid striding over the count F1
 // left; the CFL partials of its workgroups follow F1's (part + 3 * nb1)
 __global__ void __launch_bounds__(kBlock)
@@ -222,21 +239,26 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
        k += gridDim.x * blockDim.x) {
     const int e = A.work[k];
-    int nd[4];
-    load_nd(A.ielnd, e, nd);
-    double x[4], y[4], u[4], v[4];
-    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
-    const double cs2e = A.cs2[e], rhoe = A.rho[e];
-    double ev = 0.0, qx[4], qy[4];
-    f1_viscosity(A, N, e, nd, x, y, u, v, cs2e, rhoe, ev, qx, qy);
-    st4(A.cnviscx, e, qx);
-    st4(A.cnviscy, e, qy);
-    A.elvisc[e] = ev;
-    double c = f1_cfl(A, e, x, y, cs2e, ev);
-    if (!(c == c)) c = DBL_MAX;  // a NaN never wins the strict '<' scan
-    better_min(best, bi, c, e + 1);
+    better_min(best, bi, f1_moving(A, N, e), e + 1);
   }
   block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
+}
+
+// time_dr_calc / time_dr_end's selection (time_dr_advance.f90:38-132):
+// strict '<' minimum over MAXIMUM, GROWTH, INITIAL (first step), CFL, DIV
+__device__ __forceinline__ void select_dt(const FusedArgs& A, double dt_prev, int nstep,
+                                          double cfl_min, int cfl_idx, double div_max,
+                                          int div_idx, double& best, int& bidx, int& bsdt) {
+  best = A.dt_max;
+  bidx = -1;
+  bsdt = BL_DT_MAXIMUM;
+  double c = A.dt_g * dt_prev;
+  if (c < best) { best = c; bidx = -1; bsdt = BL_DT_GROWTH; }
+  if (nstep == 0 && A.dt_initial < best) { best = A.dt_initial; bidx = -1; bsdt = BL_DT_INITIAL; }
+  c = A.cfl_sf * sqrt(cfl_min);
+  if (c < best) { best = c; bidx = cfl_idx; bsdt = BL_DT_CFL; }
+  c = A.div_sf / div_max;
+  if (c < best) { best = c; bidx = div_idx; bsdt = BL_DT_DIV; }
 }
 
 // F1b: one workgroup -- combine the partials of F1, then the hydro_dr_getdt
@@ -260,15 +282,9 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
     ts->err = 2; ts->done = 1; ts->skip = 1; ts->err_step = ts->nstep + 1;
     return;
   }
-  double best = A.dt_max;
-  int bidx = -1, bsdt = BL_DT_MAXIMUM;
-  double c = A.dt_g * ts->dt;
-  if (c < best) { best = c; bidx = -1; bsdt = BL_DT_GROWTH; }
-  if (ts->nstep == 0 && A.dt_initial < best) { best = A.dt_initial; bidx = -1; bsdt = BL_DT_INITIAL; }
-  c = A.cfl_sf * sqrt(r.cfl_min);
-  if (c < best) { best = c; bidx = r.cfl_idx; bsdt = BL_DT_CFL; }
-  c = A.div_sf / r.div_max;
-  if (c < best) { best = c; bidx = r.div_idx; bsdt = BL_DT_DIV; }
+  double best;
+  int bidx, bsdt;
+  select_dt(A, ts->dt, ts->nstep, r.cfl_min, r.cfl_idx, r.div_max, r.div_idx, best, bidx, bsdt);
   ts->nstep = ts->nstep + 1;
   ts->dt = best;
   ts->idtel = bidx;
@@ -284,6 +300,167 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
   }
   if (best < A.dt_min) {  // time_dr_advance.f90:122-127
     ts->err = 3; ts->done = 1; ts->skip = 1; ts->err_step = ts->nstep;
+  }
+}
+
+// ---------------------------------------------------------------- ensembles
+// Many independent instances stepped together: the mesh is the disjoint
+// union of the instances' meshes, instance k owning elements [eoff[k],
+// eoff[k+1]).  Every instance has at least kBlock elements, so a workgroup
+// of F1 spans at most two instances (a and b); it keeps two partial slots of
+// each control.  F1L gives each instance one workgroup per kEnsChunk of its
+// elements, striding over the instance's own worklist segment, and E1b
+// selects each instance's dt in one workgroup per instance.  Each instance's selection is the first-index
+// extremum over its own (value, index) pairs, as a run of that instance
+// alone makes it.
+
+// E1: F1 with per-instance partial slots and worklist segments
+__global__ void __launch_bounds__(kBlock)
+k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
+  if (step_skipped(A, A.ts)) return;
+  const int lb = xcd_block(), nb = gridDim.x;
+  const int e = lb * kBlock + threadIdx.x;
+  const int ia = A.einst[lb * kBlock];
+  const int ib = A.einst[min(lb * kBlock + kBlock - 1, A.nel - 1)];
+  const int my = e < A.nel ? (int)A.einst[e] : ib;
+  double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false;
+  if (e < A.nel) {
+    int nd[4];
+    load_nd(A.ielnd, e, nd);
+    double x[4], y[4], u[4], v[4];
+    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+    moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] &&
+               v[0] == v[2] && v[0] == v[3]);
+    if (!moving) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      st4(A.cnviscx, e, z);
+      st4(A.cnviscy, e, z);
+      A.elvisc[e] = 0.0;
+      cflv = f1_cfl(A, e, x, y, A.cs2[e], 0.0);
+      if (!(cflv == cflv)) cflv = DBL_MAX;
+    }
+    Iso g;
+    getiso(x, y, g);
+    divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
+    if (!(divv == divv)) divv = -1.0;
+  }
+  // worklist append into each instance's segment, one atomic per wave and
+  // instance
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int k = s == 0 ? ia : ib;
+    if (s == 1 && ib == ia) break;
+    const uint64_t mask = __ballot(moving && my == k);
+    if (mask) {
+      const int lead = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0;
+      if (lane == lead) base = atomicAdd(E.cnt + k, (uint32_t)__popcll(mask));
+      base = __shfl(base, lead, 64);
+      if (moving && my == k)
+        A.work[E.eoff[k] + base + __popcll(mask & ((1ull << lane) - 1ull))] = e;
+    }
+  }
+  block_reduce_at<false>(my == ia ? divv : -1.0, e + 1, E.part + lb, E.parti + lb);
+  __syncthreads();
+  block_reduce_at<false>(my == ib ? divv : -1.0, e + 1, E.part + nb + lb, E.parti + nb + lb);
+  __syncthreads();
+  block_reduce_at<true>(my == ia ? cflv : DBL_MAX, e + 1, E.part + 2 * nb + lb,
+                        E.parti + 2 * nb + lb);
+  __syncthreads();
+  block_reduce_at<true>(my == ib ? cflv : DBL_MAX, e + 1, E.part + 3 * nb + lb,
+                        E.parti + 3 * nb + lb);
+}
+
+// E1L: instance k's workgroups lboff[k] .. lboff[k+1]-1 stride over its
+// worklist segment
+__global__ void __launch_bounds__(kBlock)
+k_e1_limiter(FusedArgs A, FusedNodes N, EnsArgs E) {
+  if (step_skipped(A, A.ts)) return;
+  int lo = 0, hi = E.ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (E.lboff[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const int k = lo, sub = blockIdx.x - E.lboff[k], nbk = E.lboff[k + 1] - E.lboff[k];
+  const int base = E.eoff[k];
+  const uint32_t count = min(E.cnt[k], (uint32_t)(E.eoff[k + 1] - base));
+  double best = DBL_MAX;
+  int bi = 0x7fffffff;
+  for (uint32_t q = sub * kBlock + threadIdx.x; q < count; q += nbk * kBlock) {
+    const int e = A.work[base + q];
+    better_min(best, bi, f1_moving(A, N, e), e + 1);
+  }
+  block_reduce_at<true>(best, bi, E.lpart + blockIdx.x, E.lparti + blockIdx.x);
+}
+
+// E1b: one workgroup per instance -- its partials, then its dt selection;
+// workgroup 0 also runs the step prologue of the whole ensemble
+__global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, int nb) {
+  DevTimestep* ts = A.ts;
+  const int k = blockIdx.x, t = threadIdx.x;
+  const bool skip = step_skipped(A, ts);
+  __syncthreads();
+  if (t == 0) {
+    if (k == 0) {
+      if (skip) ts->done = 1;
+      ts->skip = skip ? 1 : 0;
+      A.dtres->negvol = 0;
+      A.dtres->negvol2 = 0;
+      if (!skip) ts->nstep = ts->nstep + 1;
+    }
+    E.cnt[k] = 0;  // E1L has read it
+  }
+  if (skip) return;
+  const int lb0 = E.eoff[k] / kBlock, lb1 = (E.eoff[k + 1] - 1) / kBlock;
+  double va = -1.0, vc = DBL_MAX;
+  int xa = 0x7fffffff, xc = 0x7fffffff;
+  for (int lb = lb0 + t; lb <= lb1; lb += kBlock) {
+    const int slot = (int)A.einst[lb * kBlock] == k ? 0 : 1;
+    better_max(va, xa, E.part[slot * nb + lb], E.parti[slot * nb + lb]);
+    better_min(vc, xc, E.part[(2 + slot) * nb + lb], E.parti[(2 + slot) * nb + lb]);
+  }
+  for (int b = E.lboff[k] + t; b < E.lboff[k + 1]; b += kBlock)
+    better_min(vc, xc, E.lpart[b], E.lparti[b]);
+  // thread 0 writes both results and is the only reader
+  double sva, svc;
+  int32_t sxa, sxc;
+  block_reduce_at<false>(va, xa, &sva, &sxa);
+  __syncthreads();
+  block_reduce_at<true>(vc, xc, &svc, &sxc);
+  if (t != 0) return;
+  double div_max;
+  int div_idx;
+  if (sva > DBL_MIN) { div_max = sva; div_idx = sxa; }
+  else { div_max = DBL_MIN; div_idx = 0; }
+  InstTs& it = E.inst[k];
+  const int off = E.eoff[k];
+  if (svc < 0.0) {  // hydro_kn_getdt.f90:75-78
+    ts->err = 2; ts->done = 1; ts->skip = 1; ts->err_step = it.nstep + 1;
+    ts->err_idx = k;
+    return;
+  }
+  double best;
+  int bidx, bsdt;
+  select_dt(A, it.dt, it.nstep, svc, sxc - off, div_max, div_idx > 0 ? div_idx - off : 0,
+            best, bidx, bsdt);
+  const int hstep = it.nstep - ts->hist_base;
+  if (E.hist && hstep >= 0 && hstep < E.hist_cap) {
+    double* hrec = E.hist + 3 * ((size_t)hstep * E.ninst + k);
+    hrec[0] = best;
+    hrec[1] = (double)bidx;
+    hrec[2] = (double)bsdt;
+  }
+  it.nstep = it.nstep + 1;
+  it.dt = best;
+  it.idtel = bidx;
+  it.sdt = bsdt;
+  it.time = it.time + best;
+  E.inst_dt[k] = best;
+  if (best < A.dt_min) {  // time_dr_advance.f90:122-127
+    ts->err = 3; ts->done = 1; ts->skip = 1; ts->err_step = it.nstep;
+    ts->err_idx = k;
   }
 }
 
@@ -410,7 +587,7 @@ k_f2_force(FusedArgs A, FusedNodes N) {
   if (ts->skip) return;
   const int e = xcd_index();
   if (e >= A.nel) return;
-  const double dt = ts->dt;
+  const double dt = A.einst ? A.inst_dt[A.einst[e]] : ts->dt;
   const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
   int nd[4];
   load_nd(A.ielnd, e, nd);
@@ -486,7 +663,7 @@ k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   if (ts->skip || A.dtres->negvol) return;
   const int n = xcd_index();
   if (n >= A.nnd) return;
-  const double dt = ts->dt;
+  const double dt = A.ninst ? A.inst_dt[A.ninst[n]] : ts->dt;
   const double h = 0.5 * dt;
   double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
   const int b = A.hyd_ptr[n], end = A.hyd_ptr[n + 1];
@@ -530,7 +707,7 @@ k_f4_state(FusedArgs A, FusedNodes O) {
   if (ts->skip || A.dtres->negvol) return;
   const int e = xcd_index();
   if (e >= A.nel) return;
-  const double dt = ts->dt;
+  const double dt = A.einst ? A.inst_dt[A.einst[e]] : ts->dt;
   int nd[4];
   load_nd(A.ielnd, e, nd);
   double x[4], y[4], ub[4], vb[4];
@@ -1040,12 +1217,172 @@ void fill_err(bl_error_t* err, int ierr, int iout, const char* msg) {
 
 }  // namespace
 
+namespace {
+
+EnsArgs make_ens(bl_handle_t* h) {
+  EnsArgs E{};
+  E.ninst = h->ens_n;
+  E.eoff = h->d_eoff;
+  E.inst = static_cast<InstTs*>(h->d_inst);
+  E.inst_dt = h->d_inst_dt;
+  E.cnt = h->d_cnt;
+  const size_t nb = nblocks(h->sz.nel);
+  E.part = h->d_epart;
+  E.parti = h->d_eparti;
+  E.lboff = h->d_lboff;
+  E.lpart = h->d_epart + 4 * nb;
+  E.lparti = h->d_eparti + 4 * nb;
+  E.hist = h->d_ehist;
+  E.hist_cap = (int)h->ehist_cap;
+  return E;
+}
+
+// one ensemble step (parity p: node set p -> node set 1-p)
+int launch_ensemble_step(bl_handle_t* h, int p) {
+  FusedArgs A = make_args(h);
+  A.stop_at_time_end = 0;
+  A.time_end = h->cfg.time_end;
+  A.einst = h->d_einst;
+  A.ninst = h->d_ninst;
+  A.inst_dt = h->d_inst_dt;
+  EnsArgs E = make_ens(h);
+  const int nel = h->sz.nel, nnd = h->sz.nnd;
+  const unsigned nbe = nblocks(nel), nbn = nblocks(nnd);
+  FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
+  FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
+                  h->nodeset[1 - p][3]};
+  FusedNodes Oc{O.x, O.y, O.u, O.v};
+  BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nbe, kBlock, A, N, E);
+  BL_LAUNCH(h, "e1_limiter", k_e1_limiter, (unsigned)h->ens_lblocks, kBlock, A, N, E);
+  BL_LAUNCH(h, "e1b_select", k_e1b_select, (unsigned)h->ens_n, kBlock, A, E, (int)nbe);
+  BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
+  BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
+  BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
+  BL_LAUNCH(h, "f5_end", k_f5_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  return BL_SUCCESS;
+}
+
+int capture_ensemble(bl_handle_t* h) {
+  invalidate_graph(h);
+  bool timing = h->timing;
+  h->timing = false;
+  for (int p = 0; p < 2; ++p) {
+    hipGraph_t g = nullptr;
+    BL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_ensemble_step(h, p);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc != BL_SUCCESS || e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      h->timing = timing;
+      set_error("ensemble step graph capture failed");
+      return BL_FAILURE;
+    }
+    e = hipGraphInstantiate(&h->graph_exec[p], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      h->timing = timing;
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      return BL_FAILURE;
+    }
+  }
+  h->timing = timing;
+  h->graph_stop_at_end = -2;  // marks the ensemble graphs
+  return BL_SUCCESS;
+}
+
+}  // namespace
+
 extern "C" {
+
+int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* idt_hist,
+                    int32_t* ctl_hist, int32_t* steps_taken, bl_error_t* err) {
+  if (!h || h->ens_n <= 0) {
+    set_error("bl_ensemble_run: not an ensemble handle");
+    return BL_FAILURE;
+  }
+  fill_err(err, BL_SUCCESS, 0, "");
+  if (nsteps <= 0) { if (steps_taken) *steps_taken = 0; return BL_SUCCESS; }
+  if (ensure_nodesets(h)) return BL_FAILURE;
+  if (h->ehist_cap < nsteps) {
+    if (h->d_ehist) hipFree(h->d_ehist);
+    h->d_ehist = nullptr;
+    BL_HIP(hipMalloc(&h->d_ehist, sizeof(double) * 3 * (size_t)nsteps * h->ens_n));
+    h->ehist_cap = nsteps;
+    invalidate_graph(h);
+  }
+  if (!h->graph_exec[0] || h->graph_stop_at_end != -2)
+    if (capture_ensemble(h)) return BL_FAILURE;
+  DevTimestep* d = h->h_ts;
+  BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
+  BL_HIP(hipStreamSynchronize(h->stream));
+  const int start = d->nstep;
+  d->done = 0; d->skip = 0; d->err = 0; d->err_step = 0; d->err_idx = 0; d->flips = 0;
+  d->hist_base = start;
+  BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
+  int p = h->cur_set;
+  for (int k = 0; k < nsteps; ++k) {
+    if (h->timing) {
+      if (launch_ensemble_step(h, p)) return BL_FAILURE;
+    } else {
+      BL_HIP(hipGraphLaunch(h->graph_exec[p], h->stream));
+    }
+    p ^= 1;
+  }
+  BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
+  BL_HIP(hipStreamSynchronize(h->stream));
+  const int taken = d->nstep - start;
+  point_fields_at(h, (h->cur_set + d->flips) & 1);
+  {
+    int nel = h->sz.nel;
+    BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
+              h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
+              h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
+              h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
+              h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
+  }
+  if (taken > 0 && (dt_hist || idt_hist || ctl_hist)) {
+    const size_t n = (size_t)taken * h->ens_n;
+    std::vector<double> hist(3 * n);
+    BL_HIP(hipMemcpyAsync(hist.data(), h->d_ehist, sizeof(double) * 3 * n,
+                          hipMemcpyDeviceToHost, h->stream));
+    BL_HIP(hipStreamSynchronize(h->stream));
+    for (size_t q = 0; q < n; ++q) {
+      if (dt_hist) dt_hist[q] = hist[3 * q];
+      if (idt_hist) idt_hist[q] = (int32_t)hist[3 * q + 1];
+      if (ctl_hist) ctl_hist[q] = (int32_t)hist[3 * q + 2];
+    }
+  }
+  BL_HIP(hipStreamSynchronize(h->stream));
+  if (steps_taken) *steps_taken = taken;
+  switch (d->err) {
+    case 1:
+      fill_err(err, d->err_idx, BL_HALT_SINGLE,
+               "ERROR: negative volume in geometry_dr_getgeometry");
+      set_error("negative volume in geometry_dr_getgeometry");
+      return BL_FAILURE;
+    case 2:
+      fill_err(err, BL_FAILURE, BL_HALT_SINGLE,
+               "ERROR: negative CFL condition in hydro_dr_getdt");
+      set_error("negative CFL condition in hydro_dr_getdt (instance " +
+                std::to_string(d->err_idx) + ")");
+      return BL_FAILURE;
+    case 3:
+      fill_err(err, BL_FAILURE, BL_HALT_ALL, "ERROR: dt < dt_min");
+      set_error("dt < dt_min (instance " + std::to_string(d->err_idx) + ")");
+      return BL_FAILURE;
+    default:
+      return BL_SUCCESS;
+  }
+}
 
 int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep_t* ts,
                  double* dt_hist, int32_t* idt_hist, int32_t* ctl_hist, int32_t* steps_taken,
                  bl_error_t* err) {
   if (!h || !ts) return BL_FAILURE;
+  if (h->ens_n > 0) {
+    set_error("bl_run_graph: an ensemble handle steps with bl_ensemble_run");
+    return BL_FAILURE;
+  }
   if (!h->maps_built || !h->d_eos_itype || !h->d_zdtnotreg) {
     set_error("bl_run_graph: mesh maps, regions or EOS not set");
     return BL_FAILURE;

@@ -49,7 +49,7 @@ struct DevTimestep {
   int32_t err_idx;   // -iel of the last negative-volume element
   int32_t hist_n;    // steps recorded in the history buffer
   int32_t flips;     // steps whose node update wrote the other node set
-  int32_t pad_i;
+  int32_t hist_base;  // ensembles: instance step count at the batch's start
 };
 
 // per-launch kernel timing with HIP events on the handle's stream
@@ -118,6 +118,21 @@ struct Handle {
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[13] = {};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
+  // ensembles (bl_ensemble_create): instances stepped together
+  int32_t ens_n = 0;
+  std::vector<int32_t> ens_eoff, ens_noff;  // host copies, ens_n + 1 each
+  int32_t* d_eoff = nullptr;
+  int32_t* d_lboff = nullptr;      // F1L workgroup offsets per instance
+  int32_t ens_lblocks = 0;
+  uint16_t* d_einst = nullptr;
+  uint16_t* d_ninst = nullptr;
+  void* d_inst = nullptr;          // InstTs[ens_n]
+  double* d_inst_dt = nullptr;
+  uint32_t* d_cnt = nullptr;
+  double* d_epart = nullptr;
+  int32_t* d_eparti = nullptr;
+  double* d_ehist = nullptr;
+  int64_t ehist_cap = 0;
   // node relaxation as the ALE mesh motion (bl_set_relaxation)
   int32_t relax_type = 0, relax_nsweep = 1;
   double relax_om = 1.0, relax_tol = 0.0;

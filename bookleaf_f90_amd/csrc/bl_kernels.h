@@ -33,9 +33,9 @@ __device__ __forceinline__ void wave_reduce(double& v, int& i) {
   }
 }
 
-// reduce over the workgroup; thread 0 writes the partial of this workgroup
+// reduce over the workgroup; thread 0 writes the result to (*outv, *outi)
 template <bool IS_MIN>
-__device__ __forceinline__ void block_reduce_store(double v, int i, double* part, int32_t* parti) {
+__device__ __forceinline__ void block_reduce_at(double v, int i, double* outv, int32_t* outi) {
   __shared__ double sv[kBlock / 64];
   __shared__ int si[kBlock / 64];
   wave_reduce<IS_MIN>(v, i);
@@ -49,9 +49,15 @@ __device__ __forceinline__ void block_reduce_store(double v, int i, double* part
       if (IS_MIN) better_min(bv, bi, sv[w], si[w]);
       else better_max(bv, bi, sv[w], si[w]);
     }
-    part[blockIdx.x] = bv;
-    parti[blockIdx.x] = bi;
+    *outv = bv;
+    *outi = bi;
   }
+}
+
+// the partial of this workgroup at part[blockIdx.x]
+template <bool IS_MIN>
+__device__ __forceinline__ void block_reduce_store(double v, int i, double* part, int32_t* parti) {
+  block_reduce_at<IS_MIN>(v, i, part + blockIdx.x, parti + blockIdx.x);
 }
 
 __device__ __forceinline__ void block_argmin_first(double v, int i, double* part, int32_t* parti) {

@@ -51,6 +51,11 @@ def load_library(path=LIBPATH):
         lib.bl_init_mesh.argtypes = [vp]
         lib.bl_synchronize.argtypes = [vp]
         lib.bl_run_graph.argtypes = lib.bl_run.argtypes
+        lib.bl_ensemble_create.argtypes = [P(abi.Instance), C.c_int32, P(abi.Config), P(vp)]
+        lib.bl_ensemble_run.argtypes = [vp, C.c_int32, P(C.c_double), P(C.c_int32),
+                                        P(C.c_int32), P(C.c_int32), P(abi.Error)]
+        lib.bl_ensemble_offsets.argtypes = [vp, P(C.c_int32), P(C.c_int32)]
+        lib.bl_ensemble_times.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_int32)]
         lib.bl_timing_enable.argtypes = [vp, C.c_int32]
         lib.bl_timing_reset.argtypes = [vp]
         lib.bl_timing_query.argtypes = [vp, C.c_char_p, P(C.c_double), P(C.c_int64)]
@@ -386,3 +391,74 @@ class BookLeaf(DriverState):
             self.close()
         except Exception:
             pass
+
+
+class Ensemble(BookLeaf):
+    """Independent instances generated and stepped together on the device
+    (bl_ensemble_create / bl_ensemble_run); params as problem.ensemble_params."""
+
+    def __init__(self, params, config=None):
+        from . import problem as P
+        self.lib = load_library()
+        self.prefix = "bl"
+        self.params = list(params)
+        cfg = abi.Config()
+        for k, v in (config or P.ENSEMBLE_CONFIG).items():
+            setattr(cfg, k, v)
+        arr = P.instances_array(self.params)
+        h = C.c_void_p()
+        if self.lib.bl_ensemble_create(arr, len(arr), C.byref(cfg), C.byref(h)) != abi.SUCCESS:
+            raise BookLeafError("bl_ensemble_create: " + self.lib.bl_last_error().decode())
+        self.h = h
+        n = len(arr)
+        self.eoff = np.zeros(n + 1, np.int32)
+        self.noff = np.zeros(n + 1, np.int32)
+        self._check(self.lib.bl_ensemble_offsets(self.h, _iptr(self.eoff), _iptr(self.noff)),
+                    "ensemble_offsets")
+        self.nel, self.nnd = int(self.eoff[-1]), int(self.noff[-1])
+
+    def run(self, nsteps):
+        """nsteps steps of every instance; returns (dt, idtel, control), each
+        (steps, ninst)."""
+        n = len(self.params)
+        dt = np.zeros(max(nsteps, 1) * n)
+        idt = np.zeros(max(nsteps, 1) * n, np.int32)
+        ctl = np.zeros(max(nsteps, 1) * n, np.int32)
+        taken = C.c_int32(0)
+        err = abi.Error()
+        rc = self.lib.bl_ensemble_run(self.h, nsteps, _dptr(dt), _iptr(idt), _iptr(ctl),
+                                      C.byref(taken), C.byref(err))
+        m = taken.value
+        self.last_history = tuple(a[:m * n].reshape(m, n).copy() for a in (dt, idt, ctl))
+        self._check(rc, "ensemble_run", err)
+        return self.last_history
+
+    def times(self):
+        n = len(self.params)
+        t, dt, ns = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
+        self._check(self.lib.bl_ensemble_times(self.h, _dptr(t), _dptr(dt), _iptr(ns)),
+                    "ensemble_times")
+        return t, dt, ns
+
+    def instance_field(self, name, k, data=None):
+        """Instance k's part of a field (element, corner or node layout), with
+        node and element index contents made instance-local."""
+        a = self.get(name) if data is None else data
+        kind, count = self._info(name)
+        e0, e1 = int(self.eoff[k]), int(self.eoff[k + 1])
+        n0, n1 = int(self.noff[k]), int(self.noff[k + 1])
+        if name in ("ndx", "ndy", "ndu", "ndv", "indtype"):
+            return a[n0:n1].copy()
+        if count >= 4 * self.nel and count != self.nnd:
+            out = a[4 * e0:4 * e1].copy()
+        else:
+            out = a[e0:e1].copy()
+        if name == "ielnd":
+            out = out - n0
+        elif name in ("ielel", "ielsort1", "ielsort2"):
+            out = out - e0
+        elif name == "ielreg":
+            out = out - 2 * k
+        elif name == "ielmat":
+            out = out - k
+        return out

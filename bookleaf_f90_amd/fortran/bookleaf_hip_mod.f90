@@ -68,6 +68,12 @@ MODULE bookleaf_hip_mod
 &                          density_min,density_max
   END TYPE bl_totals_t
 
+  ! one instance of an ensemble (bl_ensemble_create)
+  TYPE,BIND(C) :: bl_instance_t
+    INTEGER(KIND=C_INT32_T) :: nx,ny,kind,pad
+    REAL(KIND=C_DOUBLE)     :: length,height,gamma,kappa,rho_l,e_l,rho_r,e_r
+  END TYPE bl_instance_t
+
   ! one row of hydro_dr_shortprint's region or material table
   ! (hydro_kn_print.F90:56-110, hydro_dr_print.f90:49-119)
   TYPE,BIND(C) :: bl_flag_totals_t
@@ -323,6 +329,41 @@ MODULE bookleaf_hip_mod
       REAL(KIND=C_DOUBLE),DIMENSION(*),INTENT(OUT) :: out
       INTEGER(KIND=C_INT32_T),INTENT(OUT)        :: nrec
     END FUNCTION bl_diagnostics
+
+    ! ---- ensembles ---------------------------------------------------------
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_ensemble_create(inst,ninst,cfg,h)        &
+&                                    BIND(C,NAME="bl_ensemble_create")
+      IMPORT :: C_INT32_T,C_PTR,bl_instance_t,bl_config_t
+      TYPE(bl_instance_t),DIMENSION(*),INTENT(IN) :: inst
+      INTEGER(KIND=C_INT32_T),VALUE                :: ninst
+      TYPE(bl_config_t),INTENT(IN)                 :: cfg
+      TYPE(C_PTR),INTENT(OUT)                      :: h
+    END FUNCTION bl_ensemble_create
+    ! histories (ninst, nsteps): step-major as Fortran's (instance, step)
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_ensemble_run(h,nsteps,dt_hist,idt_hist,  &
+&                                    ctl_hist,steps_taken,err)                   &
+&                                    BIND(C,NAME="bl_ensemble_run")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE,bl_error_t
+      TYPE(C_PTR),VALUE                            :: h
+      INTEGER(KIND=C_INT32_T),VALUE                :: nsteps
+      REAL(KIND=C_DOUBLE),DIMENSION(*),INTENT(OUT)     :: dt_hist
+      INTEGER(KIND=C_INT32_T),DIMENSION(*),INTENT(OUT) :: idt_hist,ctl_hist
+      INTEGER(KIND=C_INT32_T),INTENT(OUT)          :: steps_taken
+      TYPE(bl_error_t),INTENT(OUT)                 :: err
+    END FUNCTION bl_ensemble_run
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_ensemble_offsets(h,eoff,noff)            &
+&                                    BIND(C,NAME="bl_ensemble_offsets")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE                                :: h
+      INTEGER(KIND=C_INT32_T),DIMENSION(*),INTENT(OUT) :: eoff,noff
+    END FUNCTION bl_ensemble_offsets
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_ensemble_times(h,time,dt,nstep)          &
+&                                    BIND(C,NAME="bl_ensemble_times")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE                                :: h
+      REAL(KIND=C_DOUBLE),DIMENSION(*),INTENT(OUT)     :: time,dt
+      INTEGER(KIND=C_INT32_T),DIMENSION(*),INTENT(OUT) :: nstep
+    END FUNCTION bl_ensemble_times
 
     ! ---- node relaxation --------------------------------------------------
     INTEGER(KIND=C_INT32_T) FUNCTION bl_set_relaxation(h,rtype,nsweep,om,tol)   &

@@ -290,6 +290,73 @@ def sedov(n=1024, length=1.2, jitter=0.1, seed=12345, energy_total=0.25, gamma=1
         nsteps=nsteps, dims=(n, n))
 
 
+# ---- ensembles ----------------------------------------------------------
+
+ENSEMBLE_CONFIG = dict(DEFAULT_CONFIG, time_end=1.0e30, dt_initial=1.0e-3)
+
+
+def ensemble_params(ninst=512, seed=2025, nmin=64, nmax=512):
+    """Instances of a Sod/Sedov ensemble, one dict each: a square n x n mesh
+    with n uniform over nmin..nmax, Sod or Sedov with even odds, gamma
+    U[1.3, 1.67]; Sod: left rho = 1, p = 1 and right rho, p lower by ratios
+    log-uniform over [2, 20] and [2, 100] on [0,1]^2; Sedov: rho = 1, E =
+    0.25 in the corner element on [0,1.2]^2 with anti-hourglass coefficient
+    0.01.  numpy default_rng(seed), drawn in this order per instance."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ninst):
+        n = int(rng.integers(nmin, nmax + 1))
+        kind = int(rng.integers(0, 2))
+        gamma = float(rng.uniform(1.3, 1.67))
+        dr = float(np.exp(rng.uniform(np.log(2.0), np.log(20.0))))
+        pr = float(np.exp(rng.uniform(np.log(2.0), np.log(100.0))))
+        if kind == abi.INSTANCE_SOD:
+            rl, pl, rr, prr = 1.0, 1.0, 1.0 / dr, 1.0 / pr
+            out.append(dict(nx=n, ny=n, kind=kind, length=1.0, height=1.0, gamma=gamma,
+                            kappa=0.0, rho_l=rl, e_l=pl / ((gamma - 1.0) * rl), rho_r=rr,
+                            e_r=prr / ((gamma - 1.0) * rr)))
+        else:
+            out.append(dict(nx=n, ny=n, kind=kind, length=1.2, height=1.2, gamma=gamma,
+                            kappa=0.01, rho_l=1.0, e_l=0.25, rho_r=1.0, e_r=0.0))
+    return out
+
+
+def instances_array(params):
+    arr = (abi.Instance * len(params))()
+    for a, q in zip(arr, params):
+        for k, v in q.items():
+            setattr(a, k, v)
+    return arr
+
+
+def ensemble_instance(q, nsteps=200, config=None):
+    """The standalone problem of one ensemble instance: the mesh, states and
+    tables bl_ensemble_create gives instance q (two regions of q's kappa, one
+    material of q's gamma)."""
+    nx, ny = q["nx"], q["ny"]
+    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, q["length"], 0.0, q["height"])
+    nel = nx * ny
+    if q["kind"] == abi.INSTANCE_SOD:
+        xc, _ = centroids(ielnd, ndx, ndy)
+        left = xc < 0.5 * q["length"]
+        ielreg = np.where(left, 1, 2).astype(np.int32)
+        density = np.where(left, q["rho_l"], q["rho_r"])
+        energy = np.where(left, q["e_l"], q["e_r"])
+    else:
+        ielreg = np.ones(nel, np.int32)
+        density = np.full(nel, q["rho_l"])
+        energy = np.zeros(nel)
+        energy[0] = q["e_l"] / (q["rho_l"] * element_volume(ielnd, ndx, ndy, 0))
+    itype, param = ideal_eos([q["gamma"]])
+    return Problem(
+        name="instance_%s_%dx%d" % ("sod" if q["kind"] == 0 else "sedov", nx, ny),
+        ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype, ielreg=ielreg,
+        ielmat=np.ones(nel, np.int32), density=density, energy=energy,
+        ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)), nmat=1, nreg=2, eos_itype=itype,
+        eos_param=param, kappareg=np.array([q["kappa"], q["kappa"]]),
+        config=dict(config or ENSEMBLE_CONFIG), nsteps=nsteps, dims=(nx, ny))
+
+
 def with_relaxation(p, rtype=abi.RELAX_EQUI, nsweep=1, om=1.0, tol=0.0):
     """p with ALE on every step and node relaxation as its mesh motion
     (zexist, not zeul)."""

@@ -291,6 +291,40 @@ int bl_set_relaxation(bl_handle_t* h, int32_t type, int32_t nsweep, double om, d
  * recomputed. */
 int bl_relax_mesh(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol);
 
+/* ---- ensembles: many independent problems stepped together ----------
+ * One instance: an nx x ny structured mesh on [0,length] x [0,height] with
+ * reflecting walls (SLIPY bottom and top, SLIPX left and right), one
+ * ideal-gas material of ratio gamma and two regions of anti-hourglass
+ * coefficient kappa.  kind 0 (Sod): density and energy rho_l, e_l in the
+ * elements whose corner mean has x < length/2 (region 1), rho_r, e_r
+ * elsewhere (region 2).  kind 1 (Sedov): density rho_l everywhere, energy
+ * e_l/(rho_l V) in the element at the origin (e_l is the deposited energy)
+ * and 0 elsewhere, region 1.  Every instance needs nx*ny >= 256. */
+typedef struct bl_instance {
+  int32_t nx, ny, kind, pad;
+  double length, height, gamma, kappa;
+  double rho_l, e_l, rho_r, e_r;
+} bl_instance_t;
+/* A handle holding the disjoint union of the instances' meshes, generated
+ * on the device with their initial state (setup_mesh_region's LIN1 block,
+ * inf_init's geometry, mass and EOS set-up).  Instance k owns elements
+ * eoff[k] .. eoff[k+1]-1 and nodes noff[k] .. noff[k+1]-1 of the
+ * registry; instance k has material k+1 and regions 2k+1, 2k+2.  cfg is
+ * shared by every instance (zexist must be 0). */
+int bl_ensemble_create(const bl_instance_t* inst, int32_t ninst, const bl_config_t* cfg,
+                       bl_handle_t** out);
+/* nsteps steps of every instance, one captured hipGraph per step, each
+ * instance with its own dt (time_dr_calc/end on its own CFL and DIV
+ * controls).  The histories hold nsteps x ninst entries, step-major; idt
+ * is the instance-local 1-based element.  A failure stops every instance;
+ * err_idx names the element (negative volume) or instance (dt). */
+int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* idt_hist,
+                    int32_t* ctl_hist, int32_t* steps_taken, bl_error_t* err);
+/* the offset tables (ninst + 1 entries each) */
+int bl_ensemble_offsets(bl_handle_t* h, int32_t* eoff, int32_t* noff);
+/* each instance's time, last dt and step count */
+int bl_ensemble_times(bl_handle_t* h, double* time, double* dt, int32_t* nstep);
+
 /* Kernel timing with HIP events on the handle's stream: average duration
  * (ms) of the named kernel class over the launches recorded since the last
  * bl_timing_reset. */

@@ -1,0 +1,98 @@
+"""Ensembles on the device (bl_ensemble_create / bl_ensemble_run) against each
+instance run alone: the generated mesh, tables and initial state against the
+host set-up of the same instance (bl_init_mesh, bl_build_maps, inf_init's
+drivers), and every instance's dt sequence and final state against the
+oracle running that instance by itself, bit for bit."""
+import numpy as np
+import pytest
+
+from bookleaf_f90_amd import abi
+from bookleaf_f90_amd import problem as P
+from tests.helpers import diff_fields
+from tests.test_gpu_graph import LAG_SCRATCH
+
+pytestmark = pytest.mark.gpu
+
+INIT_FIELDS = ("ielnd", "ielel", "ielfc", "indtype", "ielreg", "ielmat", "ielsort1", "ielsort2",
+               "ndx", "ndy", "ndu", "ndv", "eldensity", "elenergy", "elpressure", "elcs2",
+               "elvolume", "elmass", "cnwt", "cnmass", "cnx", "cny", "a1", "a2", "a3", "b1",
+               "b2", "b3")
+STATE_FIELDS = ("eldensity", "elenergy", "elpressure", "elcs2", "elvolume", "elmass", "elvisc",
+                "cnwt", "cnmass", "cnviscx", "cnviscy", "cnx", "cny", "a1", "a3", "b1", "b3",
+                "ndx", "ndy", "ndu", "ndv")
+
+
+# a larger first dt than the ensemble default, so that the small instances'
+# own CFL limits take over from the first steps
+CFG = dict(P.ENSEMBLE_CONFIG, dt_initial=1.0e-2)
+
+
+def _small(seed=11, n=5):
+    return P.ensemble_params(n, seed=seed, nmin=16, nmax=26)
+
+
+def test_generated_ensemble_matches_host_setup():
+    from bookleaf_f90_amd.device import BookLeaf, Ensemble
+    params = _small()
+    print("ensemble generation", [(q["kind"], q["nx"]) for q in params], flush=True)
+    ens = Ensemble(params, config=CFG)
+    full = {f: ens.get(f) for f in INIT_FIELDS}
+    for k, q in enumerate(params):
+        b = BookLeaf(P.ensemble_instance(q, config=CFG))
+        for f in INIT_FIELDS:
+            mine = ens.instance_field(f, k, full[f])
+            ref = b.get(f)[:len(mine)]
+            assert np.array_equal(mine, ref), (k, f)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_ensemble_run_matches_each_instance_alone(seed, oracle_lib):
+    from bookleaf_f90_amd.device import Ensemble
+    from oracle.pyoracle import Oracle
+    params = _small(seed=seed, n=6)
+    print("ensemble run", [(q["kind"], q["nx"]) for q in params], flush=True)
+    ens = Ensemble(params, config=CFG)
+    h1 = ens.run(15)
+    h2 = ens.run(25)
+    dt, idt, ctl = (np.concatenate([a, b]) for a, b in zip(h1, h2))
+    times, last_dt, nstep = ens.times()
+    state = {f: ens.get(f) for f in STATE_FIELDS}
+    for k, q in enumerate(params):
+        o = Oracle(P.ensemble_instance(q, config=CFG))
+        odt, oidt, octl = o.run(40)
+        assert np.array_equal(dt[:, k], odt), k
+        assert np.array_equal(idt[:, k], oidt), k
+        assert np.array_equal(ctl[:, k], octl), k
+        assert times[k] == o.ts.time and last_dt[k] == o.ts.dt and nstep[k] == 40
+        mine = {f: ens.instance_field(f, k, state[f]) for f in STATE_FIELDS}
+        assert diff_fields(mine, o.state(), [f for f in STATE_FIELDS
+                                             if f not in LAG_SCRATCH]) == [], k
+
+
+def test_ensemble_medium_spot_checks(oracle_lib):
+    """64 instances of 64^2..128^2 elements; three instances checked against
+    the oracle after 20 steps."""
+    from bookleaf_f90_amd.device import Ensemble
+    from oracle.pyoracle import Oracle
+    params = P.ensemble_params(64, seed=3, nmin=64, nmax=128)
+    print("ensemble medium nel", sum(q["nx"] * q["ny"] for q in params), flush=True)
+    ens = Ensemble(params)
+    dt, idt, ctl = ens.run(20)
+    state = {f: ens.get(f) for f in ("eldensity", "elenergy", "ndx", "ndu")}
+    for k in (0, 31, 63):
+        o = Oracle(P.ensemble_instance(params[k]))
+        odt, oidt, octl = o.run(20)
+        assert np.array_equal(dt[:, k], odt) and np.array_equal(idt[:, k], oidt)
+        for f in state:
+            assert np.array_equal(ens.instance_field(f, k, state[f]),
+                                  o.get(f)[:len(ens.instance_field(f, k, state[f]))]), (k, f)
+
+
+def test_ensemble_argument_errors():
+    from bookleaf_f90_amd.device import BookLeafError, Ensemble
+    bad = [dict(_small()[0], nx=8, ny=8)]  # 64 elements < 256
+    with pytest.raises(BookLeafError):
+        Ensemble(bad)
+    cfg = dict(P.ENSEMBLE_CONFIG, zexist=1)
+    with pytest.raises(BookLeafError):
+        Ensemble(_small(), config=cfg)
