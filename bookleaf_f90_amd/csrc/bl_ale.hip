@@ -325,8 +325,8 @@ static int fluxvolume(bl_handle_t* h, double dt, const DevTimestep* dts) {
   int nnd = h->sz.nnd, nel = h->sz.nel;
   // the relaxation's mesh velocity replaces the Lagrangian one in store4/5
   if (!h->cfg.zeul && h->relax_type != BL_RELAX_OFF &&
-      relax_velocity(h, h->R(BL_NDX), h->R(BL_NDY), h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15),
-                     dt, dts))
+      mesh_motion(h, h->R(BL_NDX), h->R(BL_NDY), h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), dt,
+                  dts))
     return BL_FAILURE;
   BL_LAUNCH(h, "ale_fluxvol_nodes", k_fluxvol_nodes, nblocks(nnd), kBlock, nnd, h->cfg.zeul,
             dt, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), h->R(BL_RSCRATCH12),

@@ -1101,7 +1101,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     AleArgs B = make_ale(h, O);
     // node relaxation: its mesh velocity overwrites ubar (B.zeul = 0)
     if (!h->cfg.zeul &&
-        relax_velocity(h, O.x, O.y, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), 0.0, h->d_ts))
+        mesh_motion(h, O.x, O.y, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), 0.0, h->d_ts))
       return BL_FAILURE;
     BL_LAUNCH(h, "r1_volflux", k_r1, nbe, kBlock, B);
     BL_LAUNCH(h, "r2_massflux", k_r2, nbe, kBlock, B);

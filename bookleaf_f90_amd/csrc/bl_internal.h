@@ -138,6 +138,15 @@ struct Handle {
   double relax_om = 1.0, relax_tol = 0.0;
   int32_t* d_ring = nullptr;      // (8, nnd) stencil nodes, -1: not relaxed
   double* relax_tmp[4] = {};      // sweep ping-pong, nnd each
+  // learned relaxation (bl_relax_learned / bl_relax_collect)
+  struct bl_mlp* mlp_infer = nullptr;
+  struct bl_mlp* mlp_collect = nullptr;
+  int32_t collect_stride = 1;     // every stride-th relaxed node per step
+  int32_t* d_rlist = nullptr;     // relaxed nodes, ascending
+  int32_t nrelaxed = 0;
+  float* d_lpred = nullptr;       // (2, nnd) predicted displacement / h
+  double* d_lh = nullptr;         // nnd: local length scale h
+  int32_t* d_lflag = nullptr;     // [0] fallback this step, [1] steps, [2] fallbacks
   // per-step diagnostics of the fused step: region table + Sod L1 records
   bool diag_on = false;
   double diag_x0 = 0.0, diag_scale = 1.0;
@@ -170,7 +179,21 @@ int ale_step_device(Handle* h, const DevTimestep* d_ts);
 // the relaxation's ALE mesh velocity (x_R - x)/dt into (wu, wv), gated by
 // ts when given; the stencil table rebuilt from the connectivity
 int relax_velocity(Handle* h, const double* x, const double* y, double* wu, double* wv,
-                   double dt, const DevTimestep* ts);
+                   double dt, const DevTimestep* ts, const int32_t* only_if = nullptr);
+// the learned relaxation's mesh velocity (bl_relax_learned), with the plain
+// relaxation as the fallback; and the training pairs of the plain relaxation
+// from x (Lagrangian) and its mesh velocity (bl_relax_collect)
+int learned_velocity(Handle* h, const double* x, const double* y, double* wu, double* wv,
+                     double dt, const DevTimestep* ts);
+int collect_pairs(Handle* h, const double* x, const double* y, const double* wu,
+                  const double* wv, double dt, const DevTimestep* ts);
+// zero the learned motion's per-node proposal and length scale (nodes that
+// are not relaxed keep zeros: their proposal is their position)
+int learned_reset(Handle* h);
+// the ALE mesh velocity of the relaxation in use (learned or plain), and
+// the training pairs when collecting
+int mesh_motion(Handle* h, const double* x, const double* y, double* wu, double* wv, double dt,
+                const DevTimestep* ts);
 int relax_rebuild(Handle* h);
 // the per-step diagnostics record from the node set a step wrote
 int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* ndu,

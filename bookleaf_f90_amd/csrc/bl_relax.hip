@@ -25,6 +25,7 @@ struct RelaxArgs {
   int nnd, type;
   double om, tol, zerocut;
   const int32_t* ring;
+  const int32_t* only_if;  // non-null: the sweeps run only when *only_if != 0
 };
 
 // one node of one sweep (setup_mesh.F90:797-863); (xo, yo) the position
@@ -86,7 +87,7 @@ __device__ __forceinline__ void relax_node(const RelaxArgs& R, int n, const doub
 }
 
 #define RELAX_GATE(ts) \
-  if ((ts) && ((ts)->skip || !(ts)->ale_on)) return
+  if (((ts) && ((ts)->skip || !(ts)->ale_on)) || (R.only_if && *R.only_if == 0)) return
 
 __global__ void k_relax_sweep(RelaxArgs R, const double* __restrict__ xi,
                               const double* __restrict__ yi, double* __restrict__ xo,
@@ -181,6 +182,17 @@ int build_ring(bl_handle_t* h) {
   h->d_ring = nullptr;
   BL_HIP(hipMalloc(&h->d_ring, sizeof(int32_t) * ring.size()));
   BL_HIP(hipMemcpy(h->d_ring, ring.data(), sizeof(int32_t) * ring.size(), hipMemcpyHostToDevice));
+  // the relaxed nodes in ascending order (the learned relaxation's work list)
+  std::vector<int32_t> rlist;
+  for (int n = 0; n < nnd; ++n)
+    if (ring[8 * (size_t)n] >= 0) rlist.push_back(n);
+  if (h->d_rlist) hipFree(h->d_rlist);
+  h->d_rlist = nullptr;
+  h->nrelaxed = (int32_t)rlist.size();
+  BL_HIP(hipMalloc(&h->d_rlist, sizeof(int32_t) * std::max<size_t>(rlist.size(), 1)));
+  if (!rlist.empty())
+    BL_HIP(hipMemcpy(h->d_rlist, rlist.data(), sizeof(int32_t) * rlist.size(),
+                     hipMemcpyHostToDevice));
   return BL_SUCCESS;
 }
 
@@ -203,6 +215,7 @@ RelaxArgs make_relax(bl_handle_t* h, int type, double om, double tol) {
   R.tol = tol;
   R.zerocut = h->cfg.zerocut;
   R.ring = h->d_ring;
+  R.only_if = nullptr;
   return R;
 }
 
@@ -212,14 +225,15 @@ namespace bl {
 
 int relax_rebuild(Handle* h) {
   if (h->relax_type == BL_RELAX_OFF && !h->d_ring) return BL_SUCCESS;
-  return build_ring(static_cast<bl_handle_t*>(h));
+  if (build_ring(static_cast<bl_handle_t*>(h))) return BL_FAILURE;
+  return learned_reset(h);
 }
 
 // the ALE mesh velocity of the relaxation into (wu, wv) from the positions
 // (x, y); with ts the kernels are gated like the other ALE kernels and dt
 // is the device timestep's
 int relax_velocity(Handle* hh, const double* x, const double* y, double* wu, double* wv,
-                   double dt, const DevTimestep* ts) {
+                   double dt, const DevTimestep* ts, const int32_t* only_if) {
   bl_handle_t* h = static_cast<bl_handle_t*>(hh);
   const int nnd = h->sz.nnd;
   if (!h->d_ring || !h->relax_tmp[0]) {
@@ -227,6 +241,7 @@ int relax_velocity(Handle* hh, const double* x, const double* y, double* wu, dou
     return BL_FAILURE;
   }
   RelaxArgs R = make_relax(h, h->relax_type, h->relax_om, h->relax_tol);
+  R.only_if = only_if;
   const double* xi = x;
   const double* yi = y;
   for (int k = 0; k + 1 < h->relax_nsweep; ++k) {

@@ -56,6 +56,19 @@ def load_library(path=LIBPATH):
                                         P(C.c_int32), P(C.c_int32), P(abi.Error)]
         lib.bl_ensemble_offsets.argtypes = [vp, P(C.c_int32), P(C.c_int32)]
         lib.bl_ensemble_times.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_int32)]
+        fp = P(C.c_float)
+        lib.bl_mlp_create.argtypes = [C.c_uint64, C.c_int32, P(vp)]
+        lib.bl_mlp_destroy.argtypes = [vp]
+        lib.bl_mlp_weights.argtypes = [vp, fp, fp]
+        lib.bl_mlp_forward.argtypes = [vp, fp, C.c_int32, fp]
+        lib.bl_mlp_add_pairs.argtypes = [vp, fp, fp, C.c_int32]
+        lib.bl_mlp_pairs.argtypes = [vp, P(C.c_int32), fp, fp]
+        lib.bl_mlp_gradient.argtypes = [vp, C.c_int32, fp, fp]
+        lib.bl_mlp_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
+        lib.bl_relax_collect.argtypes = [vp, vp, C.c_int32]
+        lib.bl_relax_learned.argtypes = [vp, vp]
+        lib.bl_relax_learned_stats.argtypes = [vp, P(C.c_int32), P(C.c_int32)]
+        lib.bl_relax_features.argtypes = [vp, fp]
         lib.bl_timing_enable.argtypes = [vp, C.c_int32]
         lib.bl_timing_reset.argtypes = [vp]
         lib.bl_timing_query.argtypes = [vp, C.c_char_p, P(C.c_double), P(C.c_int64)]
@@ -71,6 +84,10 @@ def _dptr(a):
 
 def _iptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
 
 
 class DriverState:
@@ -364,6 +381,32 @@ class BookLeaf(DriverState):
         return {"nstep": r[:, 0].astype(np.int64), "time": r[:, 1],
                 "region": r[:, 2:2 + 9 * nreg].reshape(n.value, nreg, 9), "l1": r[:, -2:]}
 
+    def relax_collect(self, net, stride=1):
+        """Append (stencil, displacement) pairs of the plain relaxation to net
+        on every ALE step (every stride-th relaxed node); net None stops."""
+        self._check(self.lib.bl_relax_collect(self.h, net.m if net else None, stride),
+                    "relax_collect")
+        self._collect_net = net
+
+    def relax_learned(self, net):
+        """The network's node motion as the ALE mesh motion (the plain
+        relaxation as its fallback); net None goes back to the plain one."""
+        self._check(self.lib.bl_relax_learned(self.h, net.m if net else None), "relax_learned")
+        self._learned_net = net
+
+    def learned_stats(self):
+        """(ALE steps run with the network, steps that fell back)"""
+        a, b = C.c_int32(), C.c_int32()
+        self._check(self.lib.bl_relax_learned_stats(self.h, C.byref(a), C.byref(b)),
+                    "relax_learned_stats")
+        return a.value, b.value
+
+    def relax_features(self):
+        """The network's input features of every node (nnd, 26)."""
+        out = np.zeros((self.sizes.nnd, abi.MLP_NIN), np.float32)
+        self._check(self.lib.bl_relax_features(self.h, _fptr(out)), "relax_features")
+        return out
+
     def timing(self, on=True):
         self._check(self.lib.bl_timing_enable(self.h, int(on)), "timing_enable")
 
@@ -385,6 +428,85 @@ class BookLeaf(DriverState):
         if getattr(self, "h", None):
             self.lib.bl_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RelaxNet:
+    """The node-relaxation network (bl_mlp_*) on the current HIP device."""
+
+    def __init__(self, seed=0, capacity=0):
+        self.lib = load_library()
+        m = C.c_void_p()
+        if self.lib.bl_mlp_create(seed, capacity, C.byref(m)) != abi.SUCCESS:
+            raise BookLeafError("bl_mlp_create: " + self.lib.bl_last_error().decode())
+        self.m = m
+        self.capacity = capacity
+
+    def _check(self, rc, what):
+        if rc != abi.SUCCESS:
+            raise BookLeafError(what + ": " + self.lib.bl_last_error().decode())
+
+    def weights(self):
+        w = np.empty(abi.MLP_NPARAM, np.float32)
+        self._check(self.lib.bl_mlp_weights(self.m, _fptr(w), None), "mlp_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (abi.MLP_NPARAM,)
+        self._check(self.lib.bl_mlp_weights(self.m, None, _fptr(w)), "mlp_weights")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, abi.MLP_NIN)
+        y = np.empty((len(x), abi.MLP_NOUT), np.float32)
+        self._check(self.lib.bl_mlp_forward(self.m, _fptr(x), len(x), _fptr(y)), "mlp_forward")
+        return y
+
+    def add_pairs(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, abi.MLP_NIN)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, abi.MLP_NOUT)
+        assert len(x) == len(t)
+        self._check(self.lib.bl_mlp_add_pairs(self.m, _fptr(x), _fptr(t), len(x)),
+                    "mlp_add_pairs")
+
+    def npairs(self):
+        n = C.c_int32()
+        self._check(self.lib.bl_mlp_pairs(self.m, C.byref(n), None, None), "mlp_pairs")
+        return n.value
+
+    def pairs(self):
+        n = self.npairs()
+        x = np.empty((n, abi.MLP_NIN), np.float32)
+        t = np.empty((n, abi.MLP_NOUT), np.float32)
+        c = C.c_int32()
+        self._check(self.lib.bl_mlp_pairs(self.m, C.byref(c), _fptr(x), _fptr(t)), "mlp_pairs")
+        return x, t
+
+    def gradient(self, n):
+        """(gradient, loss) of the loss over pairs 0..n-1"""
+        g = np.empty(abi.MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        self._check(self.lib.bl_mlp_gradient(self.m, n, _fptr(g),
+                                             C.cast(C.byref(loss), C.POINTER(C.c_float))),
+                    "mlp_gradient")
+        return g, loss.value
+
+    def train(self, iters, batch=4096, lr=1e-3, seed=0):
+        """iters Adam steps; the loss of each batch"""
+        hist = np.empty(max(iters, 1), np.float32)
+        self._check(self.lib.bl_mlp_train(self.m, iters, batch, lr, seed, _fptr(hist)),
+                    "mlp_train")
+        return hist[:iters]
+
+    def close(self):
+        if getattr(self, "m", None):
+            self.lib.bl_mlp_destroy(self.m)
+            self.m = None
 
     def __del__(self):
         try:

@@ -10,7 +10,7 @@
 ! (dataAPI_params.f90:31-34).
 MODULE bookleaf_hip_mod
 
-  USE, INTRINSIC :: iso_c_binding, ONLY: C_INT32_T,C_INT64_T,C_DOUBLE,C_CHAR,  &
+  USE, INTRINSIC :: iso_c_binding, ONLY: C_INT32_T,C_INT64_T,C_DOUBLE,C_FLOAT,C_CHAR,  &
 &                                        C_PTR,C_NULL_CHAR,C_ASSOCIATED,       &
 &                                        C_F_POINTER
 
@@ -380,6 +380,84 @@ MODULE bookleaf_hip_mod
       INTEGER(KIND=C_INT32_T),VALUE :: rtype,nsweep
       REAL(KIND=C_DOUBLE),VALUE     :: om,tol
     END FUNCTION bl_relax_mesh
+
+    ! ---- learned node relaxation (no counterpart in the reference) ---------
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_create(seed,capacity,m)             &
+&                                    BIND(C,NAME="bl_mlp_create")
+      IMPORT :: C_INT32_T,C_INT64_T,C_PTR
+      INTEGER(KIND=C_INT64_T),VALUE :: seed
+      INTEGER(KIND=C_INT32_T),VALUE :: capacity
+      TYPE(C_PTR),INTENT(OUT)       :: m
+    END FUNCTION bl_mlp_create
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_destroy(m) BIND(C,NAME="bl_mlp_destroy")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE :: m
+    END FUNCTION bl_mlp_destroy
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_weights(m,wget,wset)                &
+&                                    BIND(C,NAME="bl_mlp_weights")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE :: m,wget,wset
+    END FUNCTION bl_mlp_weights
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_forward(m,x,n,y)                    &
+&                                    BIND(C,NAME="bl_mlp_forward")
+      IMPORT :: C_INT32_T,C_PTR,C_FLOAT
+      TYPE(C_PTR),VALUE                           :: m
+      REAL(KIND=C_FLOAT),DIMENSION(*),INTENT(IN)  :: x
+      INTEGER(KIND=C_INT32_T),VALUE               :: n
+      REAL(KIND=C_FLOAT),DIMENSION(*),INTENT(OUT) :: y
+    END FUNCTION bl_mlp_forward
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_add_pairs(m,x,t,n)                  &
+&                                    BIND(C,NAME="bl_mlp_add_pairs")
+      IMPORT :: C_INT32_T,C_PTR,C_FLOAT
+      TYPE(C_PTR),VALUE                          :: m
+      REAL(KIND=C_FLOAT),DIMENSION(*),INTENT(IN) :: x,t
+      INTEGER(KIND=C_INT32_T),VALUE              :: n
+    END FUNCTION bl_mlp_add_pairs
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_pairs(m,n,x,t)                      &
+&                                    BIND(C,NAME="bl_mlp_pairs")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE                   :: m,x,t
+      INTEGER(KIND=C_INT32_T),INTENT(OUT) :: n
+    END FUNCTION bl_mlp_pairs
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_gradient(m,n,grad,loss)             &
+&                                    BIND(C,NAME="bl_mlp_gradient")
+      IMPORT :: C_INT32_T,C_PTR,C_FLOAT
+      TYPE(C_PTR),VALUE                           :: m
+      INTEGER(KIND=C_INT32_T),VALUE               :: n
+      REAL(KIND=C_FLOAT),DIMENSION(*),INTENT(OUT) :: grad
+      REAL(KIND=C_FLOAT),INTENT(OUT)              :: loss
+    END FUNCTION bl_mlp_gradient
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_train(m,iters,batch,lr,seed,hist)   &
+&                                    BIND(C,NAME="bl_mlp_train")
+      IMPORT :: C_INT32_T,C_INT64_T,C_PTR,C_FLOAT
+      TYPE(C_PTR),VALUE             :: m,hist
+      INTEGER(KIND=C_INT32_T),VALUE :: iters,batch
+      REAL(KIND=C_FLOAT),VALUE      :: lr
+      INTEGER(KIND=C_INT64_T),VALUE :: seed
+    END FUNCTION bl_mlp_train
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_collect(h,m,stride)               &
+&                                    BIND(C,NAME="bl_relax_collect")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE             :: h,m
+      INTEGER(KIND=C_INT32_T),VALUE :: stride
+    END FUNCTION bl_relax_collect
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_learned(h,m)                      &
+&                                    BIND(C,NAME="bl_relax_learned")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE :: h,m
+    END FUNCTION bl_relax_learned
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_learned_stats(h,steps,fallbacks)  &
+&                                    BIND(C,NAME="bl_relax_learned_stats")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE                   :: h
+      INTEGER(KIND=C_INT32_T),INTENT(OUT) :: steps,fallbacks
+    END FUNCTION bl_relax_learned_stats
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_features(h,feat)                  &
+&                                    BIND(C,NAME="bl_relax_features")
+      IMPORT :: C_INT32_T,C_PTR,C_FLOAT
+      TYPE(C_PTR),VALUE                           :: h
+      REAL(KIND=C_FLOAT),DIMENSION(*),INTENT(OUT) :: feat
+    END FUNCTION bl_relax_features
 
     ! ---- kernel timing and diagnostics -----------------------------------
     INTEGER(KIND=C_INT32_T) FUNCTION bl_timing_enable(h,on) BIND(C,NAME="bl_timing_enable")

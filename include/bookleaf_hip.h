@@ -325,6 +325,61 @@ int bl_ensemble_offsets(bl_handle_t* h, int32_t* eoff, int32_t* noff);
 /* each instance's time, last dt and step count */
 int bl_ensemble_times(bl_handle_t* h, double* time, double* dt, int32_t* nstep);
 
+/* ---- learned node relaxation ------------------------------------------
+ * No counterpart in the reference: a multilayer perceptron 26-64-64-2
+ * (ReLU) that proposes each relaxed node's ALE displacement from its
+ * stencil, run on the matrix cores with BF16 weights and activations and
+ * FP32 accumulation, trained on (stencil, displacement) pairs the plain
+ * relaxation (bl_set_relaxation) produces.
+ *
+ * Features of a relaxed node n with stencil nodes nb[0..7] (the
+ * relaxation's table order) and h^2 the mean area of its four elements:
+ * [0..15] (x_nb - x_n, y_nb - y_n)/h; [16..19] the element areas / h^2 and
+ * [20..23] their corner Jacobians at n / h^2, elements counter-clockwise
+ * from the one between nb[4] and nb[2]; [24..25] the mean of [0..15]'s x
+ * and y.  Target: the displacement / h.
+ *
+ * Parameters (FP32 master copy, 6018 floats): W1 (64,26) row-major, b1
+ * (64), W2 (64,64), b2 (64), W3 (2,64), b3 (2); y = W3 r(W2 r(W1 x + b1)
+ * + b2) + b3.  The network's own device stream orders its calls. */
+typedef struct bl_mlp bl_mlp_t;
+enum { BL_MLP_NPARAM = 6018, BL_MLP_NIN = 26, BL_MLP_NOUT = 2 };
+/* He-uniform weights from seed, zero biases; room for capacity pairs */
+int bl_mlp_create(uint64_t seed, int32_t capacity, bl_mlp_t** out);
+int bl_mlp_destroy(bl_mlp_t* m);
+/* copy the master parameters out (get) and/or in (set), either may be NULL */
+int bl_mlp_weights(bl_mlp_t* m, float* get, const float* set);
+/* y (n x 2) = the network on x (n x 26) as the matrix cores evaluate it */
+int bl_mlp_forward(bl_mlp_t* m, const float* x, int32_t n, float* y);
+/* append n pairs (x n x 26, t n x 2) from the host */
+int bl_mlp_add_pairs(bl_mlp_t* m, const float* x, const float* t, int32_t n);
+/* the pair count, and the pairs when x, t are not NULL */
+int bl_mlp_pairs(bl_mlp_t* m, int32_t* n, float* x, float* t);
+/* gradient of the loss mean((y - t)^2) over pairs 0..n-1 with respect to
+ * the parameters (the forward pass rounds weights, inputs and hidden
+ * activations to BF16 as the matrix cores see them), and the loss */
+int bl_mlp_gradient(bl_mlp_t* m, int32_t n, float* grad, float* loss);
+/* iters Adam steps (beta 0.9, 0.999, eps 1e-8, rate lr) on batches of
+ * batch pairs drawn from the stored pairs by seed; loss_hist (iters) may be
+ * NULL.  Deterministic for a given seed. */
+int bl_mlp_train(bl_mlp_t* m, int32_t iters, int32_t batch, float lr, uint64_t seed,
+                 float* loss_hist);
+/* With m, every ALE step of h appends a pair for every stride-th relaxed
+ * node (the plain relaxation's mesh motion) to m while they fit; NULL
+ * stops.  Needs bl_set_relaxation; not with bl_relax_learned. */
+int bl_relax_collect(bl_handle_t* h, bl_mlp_t* m, int32_t stride);
+/* With m, the ALE mesh motion of h is the network's: x + h*y per relaxed
+ * node, velocity (x_new - x)/dt.  When any element of the proposed mesh has
+ * a corner Jacobian <= 0 or a longest/shortest side ratio > 10, the step
+ * uses the plain relaxation set by bl_set_relaxation instead.  NULL goes
+ * back to the plain relaxation.  m must outlive its use by h. */
+int bl_relax_learned(bl_handle_t* h, bl_mlp_t* m);
+/* ALE steps run with the network, and those that fell back */
+int bl_relax_learned_stats(bl_handle_t* h, int32_t* steps, int32_t* fallbacks);
+/* the features of every node at the current positions (nnd x 26, zero
+ * rows for nodes that are not relaxed) */
+int bl_relax_features(bl_handle_t* h, float* feat);
+
 /* Kernel timing with HIP events on the handle's stream: average duration
  * (ms) of the named kernel class over the launches recorded since the last
  * bl_timing_reset. */

@@ -1,0 +1,236 @@
+"""The learned node relaxation on the device against its numpy oracle
+(oracle/mlp.py) and against the plain relaxation: the matrix-core forward
+pass (operand layouts), the training gradient, the features, the collected
+training pairs, the fallback (bit for bit the plain-relaxation run), the
+applied motion, and training itself (loss falls, runs repeat exactly)."""
+import numpy as np
+import pytest
+
+from bookleaf_f90_amd import abi
+from bookleaf_f90_amd import problem as P
+from oracle import mlp as M
+from tests.helpers import diff_fields
+from tests.test_gpu_graph import compared
+
+pytestmark = pytest.mark.gpu
+
+
+def _relaxed(nx=32, ny=8, nsweep=3, om=0.6, jitter=0.2, seed=2024, nsteps=20):
+    return P.with_relaxation(
+        P.sod2mat_mixed(nx=nx, ny=ny, nsteps=nsteps, eulerian=False, jitter=jitter, seed=seed),
+        abi.RELAX_EQUI, nsweep=nsweep, om=om)
+
+
+def _random_weights(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    W1 = rng.uniform(-1, 1, (64, 26)) * np.sqrt(6 / 26) * scale
+    W2 = rng.uniform(-1, 1, (64, 64)) * np.sqrt(6 / 64) * scale
+    W3 = rng.uniform(-1, 1, (2, 64)) * np.sqrt(6 / 64) * scale
+    return M.pack(W1, rng.normal(0, 0.1, 64), W2, rng.normal(0, 0.1, 64), W3,
+                  rng.normal(0, 0.1, 2))
+
+
+@pytest.mark.parametrize("n", [1, 16, 1000])
+def test_forward_matches_numpy(n):
+    """Every output of the matrix-core forward pass against the oracle; a
+    wrong operand or result layout moves outputs by O(1)."""
+    from bookleaf_f90_amd.device import RelaxNet
+    print("mlp forward n", n, flush=True)
+    net = RelaxNet(seed=1)
+    w = _random_weights(5)
+    net.set_weights(w)
+    assert np.array_equal(net.weights(), w)
+    x = np.random.default_rng(n).normal(0, 1, (n, 26)).astype(np.float32)
+    y = net.forward(x)
+    yn, _ = M.forward(w, x)
+    # FP32 sums in another order may move a hidden activation by one BF16
+    # step (2^-8 relative) where it sits at a rounding boundary
+    err = np.abs(y - yn)
+    assert np.all(err <= 1e-2 * (1.0 + np.abs(yn))), err.max()
+    assert np.median(err) <= 1e-4 * (1.0 + np.abs(yn).max())
+
+
+def test_forward_each_input_reaches_its_column():
+    """W1 with a single nonzero (i, k) and identity-like later layers: output
+    0 is x[k] routed through hidden unit i, for every k (the A operand's
+    k-steps and lane groups) and several i (the B operand's tiles)."""
+    from bookleaf_f90_amd.device import RelaxNet
+    net = RelaxNet(seed=1)
+    x = np.random.default_rng(0).uniform(0.5, 2.0, (64, 26)).astype(np.float32)
+    x = M.bf16(x)
+    for k in range(26):
+        for i in (0, 17, 38, 63):
+            W1 = np.zeros((64, 26)); W1[i, k] = 1.0
+            W2 = np.zeros((64, 64)); W2[(i * 5) % 64, i] = 1.0
+            W3 = np.zeros((2, 64)); W3[0, (i * 5) % 64] = 1.0; W3[1, (i * 5) % 64] = -2.0
+            net.set_weights(M.pack(W1, np.zeros(64), W2, np.zeros(64), W3, np.zeros(2)))
+            y = net.forward(x)
+            assert np.array_equal(y[:, 0], x[:, k]), (k, i)
+            assert np.array_equal(y[:, 1], -2.0 * x[:, k]), (k, i)
+
+
+@pytest.mark.parametrize("n", [64, 300])
+def test_gradient_matches_numpy(n):
+    from bookleaf_f90_amd.device import RelaxNet
+    print("mlp gradient n", n, flush=True)
+    rng = np.random.default_rng(n)
+    x = rng.normal(0, 1, (n, 26)).astype(np.float32)
+    t = rng.normal(0, 0.5, (n, 2)).astype(np.float32)
+    net = RelaxNet(seed=3, capacity=n)
+    net.add_pairs(x, t)
+    assert net.npairs() == n
+    xs, ts = net.pairs()
+    assert np.array_equal(xs, x) and np.array_equal(ts, t)
+    w = net.weights()
+    g, loss = net.gradient(n)
+    ln, gn = M.loss_and_gradient(w, x, t)
+    assert loss == pytest.approx(ln, rel=1e-4)
+    scale = np.abs(gn).max()
+    assert np.all(np.abs(g - gn) <= 2e-3 * scale), np.abs(g - gn).max() / scale
+    for name, lo, hi in (("W1", 0, 1664), ("b1", 1664, 1728), ("W2", 1728, 5824),
+                         ("b2", 5824, 5888), ("W3", 5888, 6016), ("b3", 6016, 6018)):
+        a, b = g[lo:hi], gn[lo:hi]
+        assert np.dot(a, b) > 0.99 * np.linalg.norm(a) * np.linalg.norm(b), name
+
+
+def test_features_match_numpy(oracle_lib):
+    from bookleaf_f90_amd.device import BookLeaf
+    from oracle.pyoracle import Oracle
+    p = _relaxed(40, 24, jitter=0.3)
+    g, o = BookLeaf(p), Oracle(p)
+    ring = o.relax_ring()
+    f = g.relax_features()
+    fn, _ = M.features(ring, p.ndx, p.ndy, np.arange(p.nnd))
+    relaxed = ring[:, 0] >= 0
+    assert relaxed.sum() == (40 - 1) * (24 - 1)
+    assert np.array_equal(f[relaxed], fn[relaxed])
+    assert not np.any(f[~relaxed])
+
+
+def test_collected_pairs_are_the_plain_relaxation(oracle_lib):
+    """One ALE step collecting every relaxed node: features at the Lagrangian
+    positions and targets (x_R - x_L)/h, x_R the plain relaxation of x_L."""
+    from bookleaf_f90_amd.device import BookLeaf, RelaxNet
+    from oracle.pyoracle import Oracle
+    p = _relaxed(40, 24, nsweep=3, om=0.6, jitter=0.3)
+    lag = P.sod2mat_mixed(nx=40, ny=24, nsteps=1, eulerian=False, jitter=0.3)
+    g = BookLeaf(p)
+    net = RelaxNet(seed=0, capacity=10000)
+    g.relax_collect(net, stride=1)
+    g.run(1)
+    x, t = net.pairs()
+    o = Oracle(lag)
+    o.run(1)
+    xl, yl = o.get("ndx")[:p.nnd], o.get("ndy")[:p.nnd]
+    o.relax_mesh(abi.RELAX_EQUI, 3, 0.6, 0.0)
+    xr, yr = o.get("ndx")[:p.nnd], o.get("ndy")[:p.nnd]
+    ring = Oracle(p).relax_ring()
+    nodes = np.nonzero(ring[:, 0] >= 0)[0]
+    fn, h = M.features(ring, xl, yl, nodes)
+    assert len(x) == len(nodes)
+    assert np.array_equal(x, fn)
+    tn = np.stack([(xr - xl)[nodes] / h, (yr - yl)[nodes] / h], axis=1).astype(np.float32)
+    assert np.allclose(t, tn, rtol=1e-6, atol=1e-12)
+    assert np.abs(tn).max() > 1e-3
+    # stride 4 on the next steps: a quarter of the nodes per step
+    net2 = RelaxNet(seed=0, capacity=10000)
+    g.relax_collect(net2, stride=4)
+    g.run(3)
+    assert 3 * (len(nodes) // 4) <= net2.npairs() <= 3 * (-(-len(nodes) // 4))
+    g.relax_collect(None)
+
+
+def _far_net():
+    """every node proposed 50 h to the right: every step must fall back"""
+    from bookleaf_f90_amd.device import RelaxNet
+    net = RelaxNet(seed=0)
+    net.set_weights(M.pack(np.zeros((64, 26)), np.zeros(64), np.zeros((64, 64)), np.zeros(64),
+                           np.zeros((2, 64)), np.array([50.0, 0.0])))
+    return net
+
+
+@pytest.mark.parametrize("runner", ["run_graph", "run"])
+def test_fallback_is_the_plain_relaxation_bitexact(runner, oracle_lib):
+    from bookleaf_f90_amd.device import BookLeaf
+    from oracle.pyoracle import Oracle
+    p = _relaxed(32, 8, nsweep=2, om=0.6, nsteps=20)
+    print("mlp fallback", runner, flush=True)
+    g, o = BookLeaf(p), Oracle(p)
+    net = _far_net()
+    g.relax_learned(net)
+    hg = g.run(20, runner=runner)
+    ho = o.run(20)
+    for a, b in zip(hg, ho):
+        assert np.array_equal(a, b)
+    assert g.learned_stats() == (20, 20)
+    names = compared(p) if runner == "run_graph" else None
+    assert diff_fields(g.state(), o.state(), names, ncp=p.ncp or None) == []
+
+
+def test_learned_motion_is_applied(oracle_lib):
+    """A network proposing +0.01 h in x everywhere: after one ALE step every
+    relaxed node sits 0.01 h right of its Lagrangian position."""
+    from bookleaf_f90_amd.device import BookLeaf, RelaxNet
+    p = _relaxed(32, 16, jitter=0.2, nsteps=1)
+    lag = P.sod2mat_mixed(nx=32, ny=16, nsteps=1, eulerian=False, jitter=0.2)
+    g, gl = BookLeaf(p), BookLeaf(lag)
+    net = RelaxNet(seed=0)
+    d = np.float32(0.01)
+    net.set_weights(M.pack(np.zeros((64, 26)), np.zeros(64), np.zeros((64, 64)), np.zeros(64),
+                           np.zeros((2, 64)), np.array([d, 0.0], np.float32)))
+    g.relax_learned(net)
+    g.run(1)
+    gl.run(1)
+    assert g.learned_stats() == (1, 0)
+    from oracle.pyoracle import Oracle
+    ring = Oracle(p).relax_ring()
+    nodes = np.nonzero(ring[:, 0] >= 0)[0]
+    xl, yl = gl.get("ndx")[:p.nnd], gl.get("ndy")[:p.nnd]
+    _, h = M.features(ring, xl, yl, nodes)
+    x, y = g.get("ndx")[:p.nnd], g.get("ndy")[:p.nnd]
+    assert np.allclose(x[nodes] - xl[nodes], float(d) * h, rtol=1e-9, atol=0)
+    assert np.allclose(y[nodes], yl[nodes], rtol=0, atol=1e-15)
+    other = np.setdiff1d(np.arange(p.nnd), nodes)
+    assert np.array_equal(x[other], xl[other]) and np.array_equal(y[other], yl[other])
+
+
+def test_training_lowers_loss_and_repeats_exactly():
+    from bookleaf_f90_amd.device import BookLeaf, RelaxNet
+    p = _relaxed(64, 32, nsweep=4, om=0.5, jitter=0.2, nsteps=30)
+    g = BookLeaf(p)
+    net = RelaxNet(seed=9, capacity=200000)
+    g.relax_collect(net, stride=1)
+    g.run(30)
+    g.relax_collect(None)
+    x, t = net.pairs()
+    assert len(x) == 30 * 63 * 31
+    w0 = net.weights()
+    l0, _ = M.loss_and_gradient(w0, x[:4096], t[:4096])
+    hist = net.train(300, batch=1024, lr=2e-3, seed=4)
+    w1 = net.weights()
+    l1, _ = M.loss_and_gradient(w1, x[:4096], t[:4096])
+    print("mlp training loss %.3e -> %.3e (batch %.3e -> %.3e)" % (l0, l1, hist[0], hist[-1]),
+          flush=True)
+    assert l1 < 0.2 * l0
+    net2 = RelaxNet(seed=9, capacity=200000)
+    net2.add_pairs(x, t)
+    net2.train(300, batch=1024, lr=2e-3, seed=4)
+    assert np.array_equal(net2.weights(), w1)
+
+
+def test_argument_errors():
+    from bookleaf_f90_amd.device import BookLeaf, BookLeafError, RelaxNet
+    lag = P.sod2mat_mixed(nx=32, ny=8, nsteps=1, eulerian=False)
+    g = BookLeaf(lag)
+    net = RelaxNet(seed=0, capacity=10)
+    with pytest.raises(BookLeafError):
+        g.relax_learned(net)  # no plain relaxation set
+    with pytest.raises(BookLeafError):
+        net.add_pairs(np.zeros((11, 26)), np.zeros((11, 2)))
+    with pytest.raises(BookLeafError):
+        net.train(1)  # no pairs
+    p = _relaxed(32, 8)
+    g2 = BookLeaf(p)
+    g2.relax_collect(net, 1)
+    with pytest.raises(BookLeafError):
+        g2.relax_learned(net)

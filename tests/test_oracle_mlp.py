@@ -1,0 +1,104 @@
+"""Pins of the learned relaxation's numpy oracle (oracle/mlp.py): BF16
+rounding on known values, features in closed form on a square grid and their
+invariances, the forward pass on a hand-made network, and the gradient
+against central finite differences."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+
+def test_bf16_rounding_known_values():
+    one = np.float32(1.0)
+    ulp = 2.0 ** -7  # BF16 spacing at 1
+    vals = np.array([1.0, 1.0 + ulp / 2, 1.0 + 3 * ulp / 2, 1.0 + ulp / 2 + 2 ** -20,
+                     -2.5, 3.0e38, 1.0e-40], np.float32)
+    got = M.bf16(vals)
+    # ties go to the even mantissa: 1 + ulp/2 -> 1, 1 + 3ulp/2 -> 1 + 2ulp
+    assert got[0] == one and got[1] == one and got[2] == np.float32(1.0 + 2 * ulp)
+    assert got[3] == np.float32(1.0 + ulp)
+    assert got[4] == np.float32(-2.5)
+    assert np.isfinite(got[5])
+    # every result has a zero low half
+    assert np.all(got.view(np.uint32) & 0xFFFF == 0)
+    assert np.isnan(M.bf16(np.array([np.nan], np.float32))[0])
+
+
+def _grid(n, dx=1.0):
+    """(n+1)^2 nodes of a square grid and the stencil table of its interior
+    nodes in table order (lp,km) (lp,kk) (lp,kp) (ll,km) (ll,kp) (lm,km)
+    (lm,kk) (lm,kp), k along x and l along y"""
+    m = n + 1
+    idx = np.arange(m * m).reshape(m, m)  # [l, k]
+    x = (np.tile(np.arange(m), m) * dx).astype(np.float64)
+    y = (np.repeat(np.arange(m), m) * dx).astype(np.float64)
+    ring = -np.ones((m * m, 8), np.int32)
+    for l in range(1, n):
+        for k in range(1, n):
+            ring[idx[l, k]] = [idx[l + 1, k - 1], idx[l + 1, k], idx[l + 1, k + 1],
+                               idx[l, k - 1], idx[l, k + 1],
+                               idx[l - 1, k - 1], idx[l - 1, k], idx[l - 1, k + 1]]
+    return ring, x, y
+
+
+def test_features_closed_form_on_square_grid():
+    ring, x, y = _grid(4, dx=0.25)
+    f, h = M.features(ring, x, y)
+    assert f.shape == (9, 26) and np.allclose(h, 0.25)
+    offs = np.array([(-1, 1), (0, 1), (1, 1), (-1, 0), (1, 0), (-1, -1), (0, -1), (1, -1)],
+                    np.float32).ravel()
+    assert np.array_equal(f[:, :16], np.tile(offs, (9, 1)))
+    assert np.array_equal(f[:, 16:24], np.ones((9, 8), np.float32))
+    assert np.array_equal(f[:, 24:], np.zeros((9, 2), np.float32))
+
+
+def test_features_invariant_under_translation_and_scaling():
+    ring, x, y = _grid(6)
+    rng = np.random.default_rng(3)
+    x = x + rng.uniform(-0.2, 0.2, x.shape)
+    y = y + rng.uniform(-0.2, 0.2, y.shape)
+    f0, h0 = M.features(ring, x, y)
+    # scaling by a power of two and shifting by a dyadic amount are exact
+    f1, h1 = M.features(ring, 4.0 * x + 8.0, 4.0 * y - 16.0)
+    assert np.allclose(f1, f0, rtol=1e-6, atol=1e-6) and np.allclose(h1, 4.0 * h0)
+    # a skewed element changes its own area and Jacobian features only
+    assert np.all(f0[:, 16:20] > 0) and np.all(f0[:, 20:24] > 0)
+
+
+def test_forward_hand_made_network():
+    W1 = np.zeros((64, 26)); W2 = np.zeros((64, 64)); W3 = np.zeros((2, 64))
+    b1 = np.zeros(64); b2 = np.zeros(64); b3 = np.array([0.5, -0.25])
+    # hidden 0 = relu(x0), hidden 1 = relu(-x1); layer 2 copies; output 0 =
+    # h0 + b3[0], output 1 = 2 h1 + b3[1]
+    W1[0, 0] = 1.0; W1[1, 1] = -1.0
+    W2[0, 0] = 1.0; W2[1, 1] = 1.0
+    W3[0, 0] = 1.0; W3[1, 1] = 2.0
+    w = M.pack(W1, b1, W2, b2, W3, b3)
+    x = np.zeros((3, 26), np.float32)
+    x[:, 0] = [1.5, -2.0, 0.25]
+    x[:, 1] = [-3.0, 4.0, -0.5]
+    for rounding in (False, True):
+        y, _ = M.forward(w, x, rounding)
+        assert np.allclose(y[:, 0], np.maximum(x[:, 0], 0) + 0.5)
+        assert np.allclose(y[:, 1], 2 * np.maximum(-x[:, 1], 0) - 0.25)
+
+
+def test_gradient_against_finite_differences():
+    rng = np.random.default_rng(7)
+    w = rng.normal(0, 0.3, M.NPARAM)
+    x = rng.normal(0, 1, (9, 26))
+    t = rng.normal(0, 1, (9, 2))
+    loss, g = M.loss_and_gradient(w, x, t, rounding=False)
+    assert loss > 0
+    eps = 1e-6
+    picks = []
+    for name, lo, hi in (("W1", 0, 1664), ("b1", 1664, 1728), ("W2", 1728, 5824),
+                         ("b2", 5824, 5888), ("W3", 5888, 6016), ("b3", 6016, 6018)):
+        picks += list(rng.choice(np.arange(lo, hi), size=min(12, hi - lo), replace=False))
+    for p in picks:
+        wp = w.copy(); wp[p] += eps
+        wm = w.copy(); wm[p] -= eps
+        lp, _ = M.loss_and_gradient(wp, x, t, rounding=False)
+        lm, _ = M.loss_and_gradient(wm, x, t, rounding=False)
+        fd = (lp - lm) / (2 * eps)
+        assert fd == pytest.approx(g[p], rel=1e-5, abs=1e-8), p
