@@ -90,7 +90,7 @@ RELAX_OFF, RELAX_LIN1, RELAX_LIN2, RELAX_EQUI = 0, 1, 2, 3
 
 # the node-relaxation network: parameter count, inputs, outputs, and the
 # offsets of W1 (64,26), b1, W2 (64,64), b2, W3 (2,64), b3 in the flat vector
-MLP_NPARAM, MLP_NIN, MLP_NHID, MLP_NOUT = 6018, 26, 64, 2
+MLP_NPARAM, MLP_NIN, MLP_NHID, MLP_NOUT, MLP_NNORM = 6018, 26, 64, 2, 56
 MLP_OFF = {"W1": 0, "b1": 1664, "W2": 1728, "b2": 5824, "W3": 5888, "b3": 6016}
 
 FLAG_TOTALS = ("volume", "mass", "ie", "ke", "pressure", "pressure_min", "pressure_max",

@@ -11,6 +11,9 @@
 namespace bl {
 
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
+// workgroups of a grid-stride launch that usually exits at once (a gated
+// fallback): one per CU
+constexpr unsigned kGatedBlocks = 256;
 
 void set_error(const std::string& msg);
 
@@ -142,11 +145,16 @@ struct Handle {
   struct bl_mlp* mlp_infer = nullptr;
   struct bl_mlp* mlp_collect = nullptr;
   int32_t collect_stride = 1;     // every stride-th relaxed node per step
+  double* d_cw[2] = {};           // plain motion collected beside the learned one
+  double learned_max_aspect = 10.0;  // the check of the proposed mesh
+  int32_t learned_relative = 1;
   int32_t* d_rlist = nullptr;     // relaxed nodes, ascending
   int32_t nrelaxed = 0;
-  float* d_lpred = nullptr;       // (2, nnd) predicted displacement / h
-  double* d_lh = nullptr;         // nnd: local length scale h
-  int32_t* d_lflag = nullptr;     // [0] fallback this step, [1] steps, [2] fallbacks
+  int32_t* d_zlist = nullptr;     // nodes that are not relaxed, ascending
+  int32_t nzero = 0;
+  int32_t* d_lflag = nullptr;     // learned-motion flags and counters
+  int32_t* d_lmark = nullptr;     // nnd: node marked by the check for the guard
+  float* d_ld1 = nullptr;         // (nnd, 2): one-sweep displacement of the node
   // per-step diagnostics of the fused step: region table + Sod L1 records
   bool diag_on = false;
   double diag_x0 = 0.0, diag_scale = 1.0;
@@ -187,9 +195,6 @@ int learned_velocity(Handle* h, const double* x, const double* y, double* wu, do
                      double dt, const DevTimestep* ts);
 int collect_pairs(Handle* h, const double* x, const double* y, const double* wu,
                   const double* wv, double dt, const DevTimestep* ts);
-// zero the learned motion's per-node proposal and length scale (nodes that
-// are not relaxed keep zeros: their proposal is their position)
-int learned_reset(Handle* h);
 // the ALE mesh velocity of the relaxation in use (learned or plain), and
 // the training pairs when collecting
 int mesh_motion(Handle* h, const double* x, const double* y, double* wu, double* wv, double dt,

@@ -18,73 +18,9 @@
 
 #include "bl_internal.h"
 #include "bl_math.h"
+#include "bl_relax.h"
 
 namespace bl {
-
-struct RelaxArgs {
-  int nnd, type;
-  double om, tol, zerocut;
-  const int32_t* ring;
-  const int32_t* only_if;  // non-null: the sweeps run only when *only_if != 0
-};
-
-// one node of one sweep (setup_mesh.F90:797-863); (xo, yo) the position
-// after it.  rr of the generator is y, ss is x.
-__device__ __forceinline__ void relax_node(const RelaxArgs& R, int n, const double* __restrict__ xi,
-                                           const double* __restrict__ yi, double& xo,
-                                           double& yo) {
-  const double xn = xi[n], yn = yi[n];
-  xo = xn;
-  yo = yn;
-  const int4 qa = reinterpret_cast<const int4*>(R.ring)[2 * n];
-  const int4 qb = reinterpret_cast<const int4*>(R.ring)[2 * n + 1];
-  if (qa.x < 0) return;
-  const int nb[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-  double x[8], y[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    x[i] = xi[nb[i]];
-    y[i] = yi[nb[i]];
-  }
-  double r1, r2, r3, r4, r5, r6, r7, r8;
-  if (R.type == BL_RELAX_EQUI) {
-    // (ll,kp) = 4, (ll,km) = 3, (lp,kk) = 1, (lm,kk) = 6
-    const double r_psi = 0.5 * (y[4] - y[3]);
-    const double s_psi = 0.5 * (x[4] - x[3]);
-    const double r_phi = 0.5 * (y[1] - y[6]);
-    const double s_phi = 0.5 * (x[1] - x[6]);
-    const double zeta = s_psi * r_phi - s_phi * r_psi;
-    if (fabs(zeta) < R.zerocut) return;
-    const double alph = s_psi * s_psi + r_psi * r_psi;
-    const double beta = s_phi * s_psi + r_phi * r_psi;
-    const double gamm = s_phi * s_phi + r_phi * r_phi;
-    r1 = 0.5 * beta; r2 = alph; r3 = -r1; r4 = gamm;
-    r5 = r4; r6 = r3; r7 = r2; r8 = r1;
-    double fac = r1 + r2 + r3 + r4 + r5 + r6 + r7 + r8;
-    if (fac < 1.0e-24) {
-      fac = fac * 1.0e14;
-      r1 = r1 * 1.0e14; r2 = r2 * 1.0e14; r3 = r3 * 1.0e14; r4 = r4 * 1.0e14;
-      r5 = r5 * 1.0e14; r6 = r6 * 1.0e14; r7 = r7 * 1.0e14; r8 = r8 * 1.0e14;
-    }
-    r1 = r1 / fac; r2 = r2 / fac; r3 = r3 / fac; r4 = r4 / fac;
-    r5 = r5 / fac; r6 = r6 / fac; r7 = r7 / fac; r8 = r8 / fac;
-  } else {
-    // LIN1: r_wgt(2) = r_wgt(7) = 1; LIN2: r_wgt(4) = r_wgt(5) = 1; each
-    // divided by their sum (setup_mesh.F90:626-660)
-    const double w = 1.0 / 2.0;
-    r1 = 0.0; r2 = 0.0; r3 = 0.0; r4 = 0.0; r5 = 0.0; r6 = 0.0; r7 = 0.0; r8 = 0.0;
-    if (R.type == BL_RELAX_LIN1) { r2 = w; r7 = w; } else { r4 = w; r5 = w; }
-  }
-  // s_i = r_i for every type here
-  const double dr = r1 * y[0] + r2 * y[1] + r3 * y[2] + r4 * y[3] + r5 * y[4] + r6 * y[5] +
-                    r7 * y[6] + r8 * y[7] - yn;
-  const double ds = r1 * x[0] + r2 * x[1] + r3 * x[2] + r4 * x[3] + r5 * x[4] + r6 * x[5] +
-                    r7 * x[6] + r8 * x[7] - xn;
-  if ((fabs(dr) > R.tol) || (fabs(ds) > R.tol)) {
-    yo = yn + R.om * dr;
-    xo = xn + R.om * ds;
-  }
-}
 
 #define RELAX_GATE(ts) \
   if (((ts) && ((ts)->skip || !(ts)->ale_on)) || (R.only_if && *R.only_if == 0)) return
@@ -93,12 +29,13 @@ __global__ void k_relax_sweep(RelaxArgs R, const double* __restrict__ xi,
                               const double* __restrict__ yi, double* __restrict__ xo,
                               double* __restrict__ yo, const DevTimestep* ts) {
   RELAX_GATE(ts);
-  const int n = xcd_index();
-  if (n >= R.nnd) return;
-  double a, b;
-  relax_node(R, n, xi, yi, a, b);
-  xo[n] = a;
-  yo[n] = b;
+  const int stride = gridDim.x * blockDim.x;
+  for (int n = xcd_index(); n < R.nnd; n += stride) {
+    double a, b;
+    relax_node(R, n, xi, yi, a, b);
+    xo[n] = a;
+    yo[n] = b;
+  }
 }
 
 // the last sweep of the ALE mesh motion: w = (x_R - x)/dt, x the Lagrangian
@@ -109,12 +46,13 @@ __global__ void k_relax_velocity(RelaxArgs R, const double* __restrict__ xl,
                                  double* __restrict__ wv, double dt, const DevTimestep* ts) {
   RELAX_GATE(ts);
   if (ts) dt = ts->dt;
-  const int n = xcd_index();
-  if (n >= R.nnd) return;
-  double a, b;
-  relax_node(R, n, xi, yi, a, b);
-  wu[n] = (a - xl[n]) / dt;
-  wv[n] = (b - yl[n]) / dt;
+  const int stride = gridDim.x * blockDim.x;
+  for (int n = xcd_index(); n < R.nnd; n += stride) {
+    double a, b;
+    relax_node(R, n, xi, yi, a, b);
+    wu[n] = (a - xl[n]) / dt;
+    wv[n] = (b - yl[n]) / dt;
+  }
 }
 
 }  // namespace bl
@@ -182,17 +120,21 @@ int build_ring(bl_handle_t* h) {
   h->d_ring = nullptr;
   BL_HIP(hipMalloc(&h->d_ring, sizeof(int32_t) * ring.size()));
   BL_HIP(hipMemcpy(h->d_ring, ring.data(), sizeof(int32_t) * ring.size(), hipMemcpyHostToDevice));
-  // the relaxed nodes in ascending order (the learned relaxation's work list)
-  std::vector<int32_t> rlist;
-  for (int n = 0; n < nnd; ++n)
-    if (ring[8 * (size_t)n] >= 0) rlist.push_back(n);
-  if (h->d_rlist) hipFree(h->d_rlist);
-  h->d_rlist = nullptr;
+  // the relaxed nodes and the others, each ascending (the learned
+  // relaxation's work lists)
+  std::vector<int32_t> rlist, zlist;
+  for (int n = 0; n < nnd; ++n) (ring[8 * (size_t)n] >= 0 ? rlist : zlist).push_back(n);
+  for (int k = 0; k < 2; ++k) {
+    int32_t*& d = k == 0 ? h->d_rlist : h->d_zlist;
+    const std::vector<int32_t>& v = k == 0 ? rlist : zlist;
+    if (d) hipFree(d);
+    d = nullptr;
+    BL_HIP(hipMalloc(&d, sizeof(int32_t) * std::max<size_t>(v.size(), 1)));
+    if (!v.empty())
+      BL_HIP(hipMemcpy(d, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice));
+  }
   h->nrelaxed = (int32_t)rlist.size();
-  BL_HIP(hipMalloc(&h->d_rlist, sizeof(int32_t) * std::max<size_t>(rlist.size(), 1)));
-  if (!rlist.empty())
-    BL_HIP(hipMemcpy(h->d_rlist, rlist.data(), sizeof(int32_t) * rlist.size(),
-                     hipMemcpyHostToDevice));
+  h->nzero = (int32_t)zlist.size();
   return BL_SUCCESS;
 }
 
@@ -223,10 +165,13 @@ RelaxArgs make_relax(bl_handle_t* h, int type, double om, double tol) {
 
 namespace bl {
 
+RelaxArgs relax_args(Handle* h) {
+  return make_relax(static_cast<bl_handle_t*>(h), h->relax_type, h->relax_om, h->relax_tol);
+}
+
 int relax_rebuild(Handle* h) {
   if (h->relax_type == BL_RELAX_OFF && !h->d_ring) return BL_SUCCESS;
-  if (build_ring(static_cast<bl_handle_t*>(h))) return BL_FAILURE;
-  return learned_reset(h);
+  return build_ring(static_cast<bl_handle_t*>(h));
 }
 
 // the ALE mesh velocity of the relaxation into (wu, wv) from the positions
@@ -242,17 +187,19 @@ int relax_velocity(Handle* hh, const double* x, const double* y, double* wu, dou
   }
   RelaxArgs R = make_relax(h, h->relax_type, h->relax_om, h->relax_tol);
   R.only_if = only_if;
+  // a gated fallback mostly does nothing: a small grid (each thread a few
+  // nodes when it runs) keeps the empty launches cheap
+  const unsigned grid = only_if ? std::min(nblocks(nnd), kGatedBlocks) : nblocks(nnd);
   const double* xi = x;
   const double* yi = y;
   for (int k = 0; k + 1 < h->relax_nsweep; ++k) {
     double* xo = h->relax_tmp[2 * (k & 1)];
     double* yo = h->relax_tmp[2 * (k & 1) + 1];
-    BL_LAUNCH(h, "relax_sweep", k_relax_sweep, nblocks(nnd), kBlock, R, xi, yi, xo, yo, ts);
+    BL_LAUNCH(h, "relax_sweep", k_relax_sweep, grid, kBlock, R, xi, yi, xo, yo, ts);
     xi = xo;
     yi = yo;
   }
-  BL_LAUNCH(h, "relax_velocity", k_relax_velocity, nblocks(nnd), kBlock, R, x, y, xi, yi, wu,
-            wv, dt, ts);
+  BL_LAUNCH(h, "relax_velocity", k_relax_velocity, grid, kBlock, R, x, y, xi, yi, wu, wv, dt, ts);
   return BL_SUCCESS;
 }
 

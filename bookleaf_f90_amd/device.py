@@ -61,13 +61,17 @@ def load_library(path=LIBPATH):
         lib.bl_mlp_destroy.argtypes = [vp]
         lib.bl_mlp_weights.argtypes = [vp, fp, fp]
         lib.bl_mlp_forward.argtypes = [vp, fp, C.c_int32, fp]
+        lib.bl_mlp_normalize.argtypes = [vp]
+        lib.bl_mlp_norm.argtypes = [vp, fp, fp]
         lib.bl_mlp_add_pairs.argtypes = [vp, fp, fp, C.c_int32]
         lib.bl_mlp_pairs.argtypes = [vp, P(C.c_int32), fp, fp]
         lib.bl_mlp_gradient.argtypes = [vp, C.c_int32, fp, fp]
         lib.bl_mlp_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
         lib.bl_relax_collect.argtypes = [vp, vp, C.c_int32]
         lib.bl_relax_learned.argtypes = [vp, vp]
-        lib.bl_relax_learned_stats.argtypes = [vp, P(C.c_int32), P(C.c_int32)]
+        lib.bl_set_learned_check.argtypes = [vp, C.c_double, C.c_int32]
+        lib.bl_relax_learned_stats.argtypes = [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                               P(C.c_int32)]
         lib.bl_relax_features.argtypes = [vp, fp]
         lib.bl_timing_enable.argtypes = [vp, C.c_int32]
         lib.bl_timing_reset.argtypes = [vp]
@@ -394,12 +398,20 @@ class BookLeaf(DriverState):
         self._check(self.lib.bl_relax_learned(self.h, net.m if net else None), "relax_learned")
         self._learned_net = net
 
-    def learned_stats(self):
-        """(ALE steps run with the network, steps that fell back)"""
-        a, b = C.c_int32(), C.c_int32()
-        self._check(self.lib.bl_relax_learned_stats(self.h, C.byref(a), C.byref(b)),
-                    "relax_learned_stats")
-        return a.value, b.value
+    def set_learned_check(self, max_aspect=10.0, relative=True):
+        """the check of the network's mesh (relative: only elements it makes
+        worse than the Lagrangian mesh count)"""
+        self._check(self.lib.bl_set_learned_check(self.h, max_aspect, int(bool(relative))),
+                    "set_learned_check")
+
+    def learned_stats(self, full=False):
+        """(ALE steps run with the network, steps that fell back); with full
+        (steps, guarded steps, fallbacks, reason bits: 1 a corner Jacobian
+        <= 0, 2 an aspect ratio > max_aspect)"""
+        a, g, b, c = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.bl_relax_learned_stats(self.h, C.byref(a), C.byref(g), C.byref(b),
+                                                    C.byref(c)), "relax_learned_stats")
+        return (a.value, g.value, b.value, c.value) if full else (a.value, b.value)
 
     def relax_features(self):
         """The network's input features of every node (nnd, 26)."""
@@ -460,6 +472,21 @@ class RelaxNet:
         w = np.ascontiguousarray(w, np.float32)
         assert w.shape == (abi.MLP_NPARAM,)
         self._check(self.lib.bl_mlp_weights(self.m, None, _fptr(w)), "mlp_weights")
+
+    def normalize(self):
+        """standardisation from the stored pairs"""
+        self._check(self.lib.bl_mlp_normalize(self.m), "mlp_normalize")
+
+    def norm(self):
+        """(mu, inverse sigma, output mu, output sigma)"""
+        v = np.empty(abi.MLP_NNORM, np.float32)
+        self._check(self.lib.bl_mlp_norm(self.m, _fptr(v), None), "mlp_norm")
+        return v[:26], v[26:52], v[52:54], v[54:56]
+
+    def set_norm(self, mu, isig, mu_t, sig_t):
+        v = np.concatenate([mu, isig, mu_t, sig_t]).astype(np.float32)
+        assert v.shape == (abi.MLP_NNORM,)
+        self._check(self.lib.bl_mlp_norm(self.m, None, _fptr(v)), "mlp_norm")
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, abi.MLP_NIN)

@@ -398,6 +398,15 @@ MODULE bookleaf_hip_mod
       IMPORT :: C_INT32_T,C_PTR
       TYPE(C_PTR),VALUE :: m,wget,wset
     END FUNCTION bl_mlp_weights
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_normalize(m) BIND(C,NAME="bl_mlp_normalize")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE :: m
+    END FUNCTION bl_mlp_normalize
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_norm(m,nget,nset)                   &
+&                                    BIND(C,NAME="bl_mlp_norm")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE :: m,nget,nset
+    END FUNCTION bl_mlp_norm
     INTEGER(KIND=C_INT32_T) FUNCTION bl_mlp_forward(m,x,n,y)                    &
 &                                    BIND(C,NAME="bl_mlp_forward")
       IMPORT :: C_INT32_T,C_PTR,C_FLOAT
@@ -446,11 +455,18 @@ MODULE bookleaf_hip_mod
       IMPORT :: C_INT32_T,C_PTR
       TYPE(C_PTR),VALUE :: h,m
     END FUNCTION bl_relax_learned
-    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_learned_stats(h,steps,fallbacks)  &
-&                                    BIND(C,NAME="bl_relax_learned_stats")
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_set_learned_check(h,max_aspect,relative) &
+&                                    BIND(C,NAME="bl_set_learned_check")
+      IMPORT :: C_INT32_T,C_PTR,C_DOUBLE
+      TYPE(C_PTR),VALUE             :: h
+      REAL(KIND=C_DOUBLE),VALUE     :: max_aspect
+      INTEGER(KIND=C_INT32_T),VALUE :: relative
+    END FUNCTION bl_set_learned_check
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_learned_stats(h,steps,guarded,    &
+&                                    fallbacks,reasons) BIND(C,NAME="bl_relax_learned_stats")
       IMPORT :: C_INT32_T,C_PTR
       TYPE(C_PTR),VALUE                   :: h
-      INTEGER(KIND=C_INT32_T),INTENT(OUT) :: steps,fallbacks
+      INTEGER(KIND=C_INT32_T),INTENT(OUT) :: steps,guarded,fallbacks,reasons
     END FUNCTION bl_relax_learned_stats
     INTEGER(KIND=C_INT32_T) FUNCTION bl_relax_features(h,feat)                  &
 &                                    BIND(C,NAME="bl_relax_features")

@@ -336,28 +336,45 @@ int bl_ensemble_times(bl_handle_t* h, double* time, double* dt, int32_t* nstep);
  * relaxation's table order) and h^2 the mean area of its four elements:
  * [0..15] (x_nb - x_n, y_nb - y_n)/h; [16..19] the element areas / h^2 and
  * [20..23] their corner Jacobians at n / h^2, elements counter-clockwise
- * from the one between nb[4] and nb[2]; [24..25] the mean of [0..15]'s x
- * and y.  Target: the displacement / h.
+ * from the one between nb[4] and nb[2]; [24..25] the displacement / h of
+ * one sweep of the relaxation set by bl_set_relaxation.  Target: the
+ * displacement / h after that relaxation's nsweep sweeps.  The network
+ * learns the target's difference from [24..25]: a network that outputs zero
+ * moves the mesh by one plain sweep.
  *
  * Parameters (FP32 master copy, 6018 floats): W1 (64,26) row-major, b1
- * (64), W2 (64,64), b2 (64), W3 (2,64), b3 (2); y = W3 r(W2 r(W1 x + b1)
- * + b2) + b3.  The network's own device stream orders its calls. */
+ * (64), W2 (64,64), b2 (64), W3 (2,64), b3 (2); N(x) = W3 r(W2 r(W1 x +
+ * b1) + b2) + b3.  With the standardisation (56 floats: input means mu[26],
+ * inverse input standard deviations is[26], output means mt[2], output
+ * standard deviations st[2]) the proposed displacement / h is
+ * f[24..25] + mt + st * N((f - mu) * is), the standardised input rounded to
+ * BF16.  The network's own device stream orders its calls. */
 typedef struct bl_mlp bl_mlp_t;
-enum { BL_MLP_NPARAM = 6018, BL_MLP_NIN = 26, BL_MLP_NOUT = 2 };
+enum { BL_MLP_NPARAM = 6018, BL_MLP_NIN = 26, BL_MLP_NOUT = 2, BL_MLP_NNORM = 56 };
 /* He-uniform weights from seed, zero biases; room for capacity pairs */
 int bl_mlp_create(uint64_t seed, int32_t capacity, bl_mlp_t** out);
 int bl_mlp_destroy(bl_mlp_t* m);
 /* copy the master parameters out (get) and/or in (set), either may be NULL */
 int bl_mlp_weights(bl_mlp_t* m, float* get, const float* set);
-/* y (n x 2) = the network on x (n x 26) as the matrix cores evaluate it */
+/* the standardisation from the stored pairs' per-column mean and standard
+ * deviation, of the features and of t - x[24..25] (a column whose deviation
+ * is ~0 keeps scale 1); bl_mlp_train
+ * calls it on its first use when no standardisation has been set */
+int bl_mlp_normalize(bl_mlp_t* m);
+/* copy the standardisation out (get) and/or in (set), either may be NULL */
+int bl_mlp_norm(bl_mlp_t* m, float* get, const float* set);
+/* y (n x 2) = the proposed displacement / h for features x (n x 26), as
+ * the matrix cores evaluate the network */
 int bl_mlp_forward(bl_mlp_t* m, const float* x, int32_t n, float* y);
 /* append n pairs (x n x 26, t n x 2) from the host */
 int bl_mlp_add_pairs(bl_mlp_t* m, const float* x, const float* t, int32_t n);
 /* the pair count, and the pairs when x, t are not NULL */
 int bl_mlp_pairs(bl_mlp_t* m, int32_t* n, float* x, float* t);
-/* gradient of the loss mean((y - t)^2) over pairs 0..n-1 with respect to
- * the parameters (the forward pass rounds weights, inputs and hidden
- * activations to BF16 as the matrix cores see them), and the loss */
+/* gradient of the loss mean((N(x_s) - t_s)^2) over pairs 0..n-1, x_s the
+ * standardised inputs and t_s the standardised differences t - x[24..25],
+ * with respect to the parameters
+ * (the forward pass rounds weights, inputs and hidden activations to BF16
+ * as the matrix cores see them), and the loss */
 int bl_mlp_gradient(bl_mlp_t* m, int32_t n, float* grad, float* loss);
 /* iters Adam steps (beta 0.9, 0.999, eps 1e-8, rate lr) on batches of
  * batch pairs drawn from the stored pairs by seed; loss_hist (iters) may be
@@ -365,17 +382,35 @@ int bl_mlp_gradient(bl_mlp_t* m, int32_t n, float* grad, float* loss);
 int bl_mlp_train(bl_mlp_t* m, int32_t iters, int32_t batch, float lr, uint64_t seed,
                  float* loss_hist);
 /* With m, every ALE step of h appends a pair for every stride-th relaxed
- * node (the plain relaxation's mesh motion) to m while they fit; NULL
- * stops.  Needs bl_set_relaxation; not with bl_relax_learned. */
+ * node (the plain relaxation's mesh motion of the step's Lagrangian mesh)
+ * to m while they fit; NULL stops.  Needs bl_set_relaxation.  With
+ * bl_relax_learned on as well, the network moves the mesh and the plain
+ * motion is computed beside it for the pairs only (training on the states
+ * the network leads to). */
 int bl_relax_collect(bl_handle_t* h, bl_mlp_t* m, int32_t stride);
 /* With m, the ALE mesh motion of h is the network's: x + h*y per relaxed
- * node, velocity (x_new - x)/dt.  When any element of the proposed mesh has
- * a corner Jacobian <= 0 or a longest/shortest side ratio > 10, the step
- * uses the plain relaxation set by bl_set_relaxation instead.  NULL goes
- * back to the plain relaxation.  m must outlive its use by h. */
+ * node, velocity h*y/dt.  The proposed mesh is checked element by element
+ * (bl_set_learned_check); the four nodes of every failing element move by
+ * one plain sweep instead (the guard) and the mesh is checked again; if it
+ * still fails, the step uses the plain relaxation set by bl_set_relaxation
+ * instead (the fallback).  NULL goes back to the plain relaxation.  m
+ * must outlive its use by h.  The step counters of bl_relax_learned_stats
+ * restart. */
 int bl_relax_learned(bl_handle_t* h, bl_mlp_t* m);
-/* ALE steps run with the network, and those that fell back */
-int bl_relax_learned_stats(bl_handle_t* h, int32_t* steps, int32_t* fallbacks);
+/* The check of the proposed mesh: an element fails with a corner Jacobian
+ * <= 0 or a longest/shortest side ratio > max_aspect.  relative 0: any
+ * failing element makes the step fall back; relative 1 (the default, with
+ * max_aspect 10): only an element the proposal makes worse than the same
+ * element of the step's Lagrangian mesh (smaller corner Jacobian, larger
+ * ratio), so that a poor element the Lagrangian motion made, and which the
+ * plain relaxation would keep as well, does not force the fallback. */
+int bl_set_learned_check(bl_handle_t* h, double max_aspect, int32_t relative);
+/* since the network was attached: ALE steps run with it, the steps that
+ * used the guard, the steps that fell back, and the reasons seen (bit 0: a
+ * corner Jacobian <= 0, bit 1: an aspect ratio > max_aspect); any output
+ * may be NULL */
+int bl_relax_learned_stats(bl_handle_t* h, int32_t* steps, int32_t* guarded, int32_t* fallbacks,
+                           int32_t* reasons);
 /* the features of every node at the current positions (nnd x 26, zero
  * rows for nodes that are not relaxed) */
 int bl_relax_features(bl_handle_t* h, float* feat);

@@ -43,7 +43,7 @@ def _grid(n, dx=1.0):
 
 def test_features_closed_form_on_square_grid():
     ring, x, y = _grid(4, dx=0.25)
-    f, h = M.features(ring, x, y)
+    f, h = M.features(ring, x, y)  # one sweep moves nothing
     assert f.shape == (9, 26) and np.allclose(h, 0.25)
     offs = np.array([(-1, 1), (0, 1), (1, 1), (-1, 0), (1, 0), (-1, -1), (0, -1), (1, -1)],
                     np.float32).ravel()
@@ -60,7 +60,8 @@ def test_features_invariant_under_translation_and_scaling():
     f0, h0 = M.features(ring, x, y)
     # scaling by a power of two and shifting by a dyadic amount are exact
     f1, h1 = M.features(ring, 4.0 * x + 8.0, 4.0 * y - 16.0)
-    assert np.allclose(f1, f0, rtol=1e-6, atol=1e-6) and np.allclose(h1, 4.0 * h0)
+    assert np.allclose(f1, f0, rtol=1e-5, atol=1e-5) and np.allclose(h1, 4.0 * h0)
+    assert np.abs(f0[:, 24:]).max() > 1e-2
     # a skewed element changes its own area and Jacobian features only
     assert np.all(f0[:, 16:20] > 0) and np.all(f0[:, 20:24] > 0)
 
@@ -102,3 +103,46 @@ def test_gradient_against_finite_differences():
         lm, _ = M.loss_and_gradient(wm, x, t, rounding=False)
         fd = (lp - lm) / (2 * eps)
         assert fd == pytest.approx(g[p], rel=1e-5, abs=1e-8), p
+
+
+def test_standardisation():
+    rng = np.random.default_rng(5)
+    x = rng.normal(3.0, 0.5, (1000, 26)).astype(np.float32)
+    x[:, 7] = 1.0  # a constant column keeps scale 1
+    t = rng.normal(-0.2, 0.01, (1000, 2)).astype(np.float32)
+    mu, isig, mt, st = M.stats(x, t)
+    assert np.allclose(mu[:7], 3.0, atol=0.1) and mu[7] == 1.0 and isig[7] == 1.0
+    assert np.allclose(1.0 / isig[:7], 0.5, rtol=0.1)
+    # the outputs are standardised as differences from x[24..25]
+    r = t.astype(np.float64) - x[:, 24:26]
+    assert np.allclose(mt, r.mean(0), rtol=1e-6) and np.allclose(st, r.std(0), rtol=1e-5)
+    xs = M.standardise(x, (mu, isig, mt, st))
+    assert np.allclose(xs[:, :7].mean(0), 0.0, atol=1e-5)
+    assert np.allclose(xs[:, :7].std(0), 1.0, rtol=1e-4)
+    # the loss with standardisation is the loss of the standardised problem
+    w = rng.normal(0, 0.2, M.NPARAM)
+    ts = (M.residual(x, t) - mt) / st + xs[:, 24:26]
+    l1, g1 = M.loss_and_gradient(w, x, t, norm=(mu, isig, mt, st))
+    l2, g2 = M.loss_and_gradient(w, xs, ts)
+    # (the float32 round trip through xs[24..25] costs a few ulps)
+    assert l1 == pytest.approx(l2, rel=1e-5)
+    assert np.allclose(g1, g2, rtol=1e-4, atol=1e-6 * np.abs(g1).max())
+
+
+@pytest.mark.parametrize("rtype", [1, 2, 3])
+def test_one_sweep_feature_is_the_relaxation_sweep(rtype, oracle_lib):
+    """[24..25] against one sweep of the C oracle's relaxation (the
+    generator's stencil, pinned in test_oracle_relax) on a jittered mesh"""
+    from bookleaf_f90_amd import problem as P
+    from oracle.pyoracle import Oracle
+    ielnd, ndx, ndy, indtype = P.structured_mesh(24, 16, 0.0, 1.0, 0.0, 16 / 24, jitter=0.3)
+    p = P.mesh_problem("jit", ielnd, ndx, ndy, indtype)
+    o = Oracle(p)
+    o.relax_mesh(rtype, 1, 0.7, 0.0)
+    ring = o.relax_ring()
+    nodes = np.nonzero(ring[:, 0] >= 0)[0]
+    x1, y1 = o.get("ndx")[:p.nnd], o.get("ndy")[:p.nnd]
+    f, h = M.features(ring, ndx, ndy, nodes, rtype=rtype, om=0.7)
+    d = np.stack([(x1 - ndx)[nodes], (y1 - ndy)[nodes]], 1) / h[:, None].astype(np.float64)
+    assert np.abs(d).max() > 1e-2
+    assert np.allclose(f[:, 24:], d, rtol=1e-4, atol=1e-5)
