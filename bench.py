@@ -90,20 +90,12 @@ KERNEL_BYTES = {
     # the check: ielnd 16 | x, y, mesh velocity 32
     "learned_check": (16, 32),
 }
-# the learned-relaxation workload: EQUI relaxation with 8 sweeps of om 0.25
-# as the plain method the network imitates; training pairs from three
-# 512x256 runs of the same two-material problem with other interface seeds
-LEARNED_RELAX = (8, 0.25)
-LEARNED_TRAIN_SEEDS = (1, 2, 3)
-LEARNED_T_END = 0.015  # physical time of the equal-time L1 comparison
 # per Lagrangian step at the driver boundary: 1664 B/element + 272 B/node,
 # +84 B/element with the anti-hourglass force on
 DRIVER_BOUNDARY = (1664 + 84, 272)
 # Eulerian: the Lagrangian step without the hourglass force plus the remap's
 # 504 B/element + 108 B/node
 DRIVER_BOUNDARY_EUL = (1664 + 504, 272 + 108)
-
-
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,83 +188,6 @@ def cpu_baseline_ensemble(params, steps):
                       "element-steps/s" % (nproc, nproc, nels, steps)}
 
 
-def learned_problem(nsteps, seed=2024, nx=1024, ny=512, tend=1.0e30):
-    from bookleaf_f90_amd import abi
-    from bookleaf_f90_amd import problem as P
-    p = P.with_relaxation(P.sod2mat_mixed(nx=nx, ny=ny, nsteps=nsteps, eulerian=False,
-                                          jitter=0.1, seed=seed), abi.RELAX_EQUI,
-                          nsweep=LEARNED_RELAX[0], om=LEARNED_RELAX[1])
-    p.config["time_end"] = tend
-    return p
-
-
-def train_learned(pairs=1_000_000, iters=1500):
-    """Pairs of the plain relaxation from three 512x256 runs (every 40th
-    relaxed node per step, 120 steps each, up to `pairs`), then Adam on the
-    device: iters steps at rate 1e-3 and iters at 2e-4, batches of 8192."""
-    from bookleaf_f90_amd.device import BookLeaf, RelaxNet
-    t0 = time.perf_counter()
-    net = RelaxNet(seed=1, capacity=pairs)
-    for seed in LEARNED_TRAIN_SEEDS:
-        g = BookLeaf(learned_problem(120, seed=seed, nx=512, ny=256))
-        g.relax_collect(net, stride=40)
-        g.run(120, runner="run_graph")
-        g.relax_collect(None)
-        g.close()
-    t1 = time.perf_counter()
-    h1 = net.train(iters, batch=8192, lr=1e-3, seed=7)
-    h2 = net.train(iters, batch=8192, lr=2e-4, seed=8)
-    t2 = time.perf_counter()
-    return net, {"pairs": net.npairs(), "collect_s": t1 - t0, "train_s": t2 - t1,
-                 "adam_steps": 2 * iters, "batch": 8192, "loss_first": float(h1[0]),
-                 "loss_last": float(h2[-1])}
-
-
-def learned_vs_plain(net):
-    """The plain relaxation, the network and one plain sweep (the method at
-    the network's cost without the network), each run from t=0 to
-    LEARNED_T_END on the 1024x512 problem: wall time, steps, and the
-    volume- (mass-) weighted relative L1 of density (energy) against the
-    plain run."""
-    import numpy as np
-    from bookleaf_f90_amd import abi
-    from bookleaf_f90_amd.device import BookLeaf
-
-    def run(mode):
-        p = learned_problem(100000, tend=LEARNED_T_END)
-        if mode == "one_sweep":
-            p.relax = (abi.RELAX_EQUI, 1, LEARNED_RELAX[1], 0.0)
-        g = BookLeaf(p)
-        if mode == "learned":
-            g.relax_learned(net)
-        g.run(1, runner="run_graph")
-        g.synchronize()
-        t0 = time.perf_counter()
-        while g.ts.time < LEARNED_T_END:
-            g.run(20, stop_at_time_end=True, runner="run_graph")
-        g.synchronize()
-        sec = time.perf_counter() - t0
-        st = {k: g.get(k)[:p.nel] for k in ("eldensity", "elenergy", "elvolume", "elmass")}
-        info = {"seconds": sec, "steps": int(g.ts.nstep), "time": g.ts.time}
-        if mode == "learned":
-            s_, gd, fb, rs = g.learned_stats(full=True)
-            info.update(guarded_steps=gd, fallback_steps=fb)
-        g.close()
-        return st, info
-
-    ref, plain = run("plain")
-    out = {"t_end": LEARNED_T_END, "plain": plain}
-    for mode in ("learned", "one_sweep"):
-        st, info = run(mode)
-        for k, wk in (("eldensity", "elvolume"), ("elenergy", "elmass")):
-            w = ref[wk]
-            info["L1_" + k[2:]] = float(np.sum(np.abs(st[k] - ref[k]) * w) /
-                                        np.sum(np.abs(ref[k]) * w))
-        info["speedup_vs_plain"] = plain["seconds"] / info["seconds"]
-        out[mode] = info
-    return out
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -296,10 +211,12 @@ def main():
         wname = "sod2mat_eulerian_2048x256_mixed_interface_seed2024_%dsteps" % (
             args.warmup + args.steps)
     elif args.workload == "learned":
-        net, train_info = train_learned()
-        prob = learned_problem(args.warmup + args.steps)
+        from bookleaf_f90_amd import learned as L
+        net, train_info = L.train()
+        prob = L.relax_problem(args.warmup + args.steps)
         wname = ("sod2mat_ale_learned_equi_relaxation_1024x512_mixed_interface_jitter0.1_"
-                 "seed2024_nsweep%d_om%g_%dsteps" % (LEARNED_RELAX + (args.warmup + args.steps,)))
+                 "seed2024_nsweep%d_om%g_%dsteps" % (L.RELAX[1], L.RELAX[2],
+                                                     args.warmup + args.steps))
     else:
         from bookleaf_f90_amd import abi
         prob = P.with_relaxation(
@@ -403,7 +320,7 @@ def main():
         s_, gd, fb, rs = g.learned_stats(full=True)
         out["learned"] = {"training": train_info, "timed_run": {"guarded_steps": gd,
                                                               "fallback_steps": fb},
-                          "equal_time": learned_vs_plain(net)}
+                          "equal_time": L.compare(net)}
     print(json.dumps(out))
 
 

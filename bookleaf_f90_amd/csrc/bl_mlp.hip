@@ -352,10 +352,10 @@ __global__ void __launch_bounds__(256) k_mlp_nodes(MlpPacked W, const float* __r
   }
 }
 
-// smallest corner Jacobian and squared longest/shortest side ratio of the
-// quadrilateral (px, py), corners counter-clockwise
+// smallest corner Jacobian, squared longest/shortest side ratio and area
+// of the quadrilateral (px, py), corners counter-clockwise
 __device__ __forceinline__ void quad_quality(const double px[4], const double py[4], double& jmin,
-                                             double& asp2) {
+                                             double& asp2, double& area) {
   double lmin = DBL_MAX, lmax = 0.0;
   jmin = DBL_MAX;
 #pragma unroll
@@ -369,6 +369,7 @@ __device__ __forceinline__ void quad_quality(const double px[4], const double py
     lmax = len2 > lmax ? len2 : lmax;
   }
   asp2 = lmin > 0.0 ? lmax / lmin : DBL_MAX;
+  area = 0.5 * ((px[2] - px[0]) * (py[3] - py[1]) - (px[3] - px[1]) * (py[2] - py[0]));
 }
 
 // one element of the check below
@@ -389,10 +390,10 @@ __device__ __forceinline__ void check_element(int e, const int32_t* __restrict__
     px[c] = x[nd[c]] + wu[nd[c]] * dt;
     py[c] = y[nd[c]] + wv[nd[c]] * dt;
   }
-  double jp, ap;
-  quad_quality(px, py, jp, ap);
-  bool jbad = !(jp > 0.0), abad = !(ap <= amax2);
-  if (relative && (jbad || abad)) {
+  double jp, ap, vp;
+  quad_quality(px, py, jp, ap, vp);
+  bool jbad = !(jp > 0.0), abad = !(ap <= amax2), vbad = false;
+  if (relative) {
     double ox[4], oy[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -400,12 +401,14 @@ __device__ __forceinline__ void check_element(int e, const int32_t* __restrict__
       ox[c] = x[nd[c]] + ((double)d.x / dt) * dt;
       oy[c] = y[nd[c]] + ((double)d.y / dt) * dt;
     }
-    double jo, ao;
-    quad_quality(ox, oy, jo, ao);
+    double jo, ao, vo;
+    quad_quality(ox, oy, jo, ao, vo);
     jbad = jbad && jp < jo;
     abad = abad && ap > ao;
+    // an element the proposal shrinks below half its one-sweep area
+    vbad = !(vp >= 0.5 * vo);
   }
-  const int code = (jbad ? 1 : 0) | (abad ? 2 : 0);
+  const int code = (jbad ? 1 : 0) | (abad ? 2 : 0) | (vbad ? 4 : 0);
   if (!code) return;
   atomicOr(&flag[3], code);
   if (stage == 0) {
@@ -419,9 +422,10 @@ __device__ __forceinline__ void check_element(int e, const int32_t* __restrict__
 
 // The quality check of the proposed mesh x + w dt: an element fails with a
 // corner Jacobian <= 0 (bit 0) or a longest/shortest side ratio above
-// sqrt(amax2) (bit 1).  With relative set, a failing element counts only
-// when the proposal is worse than the same element after one plain sweep
-// x + (d1/dt) dt (what the guard would give it).  stage 0: a counted element
+// sqrt(amax2) (bit 1).  With relative set, such an element counts only when
+// the proposal is worse than the same element after one plain sweep
+// x + (d1/dt) dt (what the guard would give it), and an element also fails
+// when the proposal shrinks it below half its one-sweep area (bit 2).  stage 0: a counted element
 // marks its four nodes for the guard and sets flag[4]; stage 1 (only when
 // flag[4] is set, after the guard): a counted element sets the step's
 // fallback flag[0].  flag[3] keeps every reason seen since the network was

@@ -398,17 +398,19 @@ int bl_relax_collect(bl_handle_t* h, bl_mlp_t* m, int32_t stride);
  * restart. */
 int bl_relax_learned(bl_handle_t* h, bl_mlp_t* m);
 /* The check of the proposed mesh: an element fails with a corner Jacobian
- * <= 0 or a longest/shortest side ratio > max_aspect.  relative 0: any
- * failing element makes the step fall back; relative 1 (the default, with
- * max_aspect 10): only an element the proposal makes worse than the same
- * element of the step's Lagrangian mesh (smaller corner Jacobian, larger
- * ratio), so that a poor element the Lagrangian motion made, and which the
- * plain relaxation would keep as well, does not force the fallback. */
+ * <= 0 or a longest/shortest side ratio > max_aspect.  relative 0: every
+ * such element fails.  relative 1 (the default, with max_aspect 10): such
+ * an element fails only when the proposal is worse (smaller corner
+ * Jacobian, larger ratio) than the same element after one plain sweep from
+ * the step's Lagrangian mesh, so that a poor element the Lagrangian motion
+ * made, and which the plain relaxation keeps as well, does not force the
+ * fallback; and an element the proposal shrinks below half its one-sweep
+ * area fails too (a proposal that collapses elements would collapse dt). */
 int bl_set_learned_check(bl_handle_t* h, double max_aspect, int32_t relative);
 /* since the network was attached: ALE steps run with it, the steps that
  * used the guard, the steps that fell back, and the reasons seen (bit 0: a
- * corner Jacobian <= 0, bit 1: an aspect ratio > max_aspect); any output
- * may be NULL */
+ * corner Jacobian <= 0, bit 1: an aspect ratio > max_aspect, bit 2: an
+ * element shrunk below half its one-sweep area); any output may be NULL */
 int bl_relax_learned_stats(bl_handle_t* h, int32_t* steps, int32_t* guarded, int32_t* fallbacks,
                            int32_t* reasons);
 /* the features of every node at the current positions (nnd x 26, zero
