@@ -116,6 +116,8 @@ def parse():
     ap.add_argument("--diagnostics", action="store_true",
                     help="record the region table and Sod L1 on the device every step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="learned: skip the equal-time comparison with the plain relaxation")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -320,7 +322,7 @@ def main():
         s_, gd, fb, rs = g.learned_stats(full=True)
         out["learned"] = {"training": train_info, "timed_run": {"guarded_steps": gd,
                                                               "fallback_steps": fb},
-                          "equal_time": L.compare(net)}
+                          "equal_time": None if args.no_compare else L.compare(net)}
     print(json.dumps(out))
 
 

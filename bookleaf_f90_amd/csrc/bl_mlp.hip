@@ -205,21 +205,36 @@ __device__ __forceinline__ f32x4 mfma32(const u64x2 a, const bf16x8 b, const f32
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), b, c, 0, 0, 0);
 }
 
-// the three layers for 16 nodes: in the B operand of the 32 (padded)
-// inputs; lane l (l < 16) gets node l's two outputs in [0], [1]
-__device__ __forceinline__ f32x4 mlp_pass(const MlpShared& S, const bf16x8 in) {
+// the three layers for the wave's 64 nodes, as four groups of 16 taken
+// layer by layer (each weight fragment read from LDS once per wave, four
+// independent accumulations per fragment): in[q] the B operand of group
+// q's 32 (padded) inputs; lane l (l < 16) gets node 16q + l's two outputs
+// in y[q][0], y[q][1]
+__device__ __forceinline__ void mlp_wave(const MlpShared& S, const bf16x8 in[4], f32x4 y[4]) {
   const int l = threadIdx.x & 63, g = l >> 4;
-  uint64_t a1[4], a2[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) a1[t] = relu_pack4(mfma32(S.w1[t][l], in, S.b1[t][g]));
+  uint64_t a1[4][4], a2[4][4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    f32x4 acc = mfma32(S.w2[2 * t][l], cat_bf16(a1[0], a1[1]), S.b2[t][g]);
-    acc = mfma32(S.w2[2 * t + 1][l], cat_bf16(a1[2], a1[3]), acc);
-    a2[t] = relu_pack4(acc);
+    const u64x2 w = S.w1[t][l];
+    const f32x4 b = S.b1[t][g];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a1[q][t] = relu_pack4(mfma32(w, in[q], b));
   }
-  f32x4 y = mfma32(S.w3[0][l], cat_bf16(a2[0], a2[1]), S.b3[g]);
-  return mfma32(S.w3[1][l], cat_bf16(a2[2], a2[3]), y);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const u64x2 w0 = S.w2[2 * t][l], w1 = S.w2[2 * t + 1][l];
+    const f32x4 b = S.b2[t][g];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const f32x4 acc = mfma32(w0, cat_bf16(a1[q][0], a1[q][1]), b);
+      a2[q][t] = relu_pack4(mfma32(w1, cat_bf16(a1[q][2], a1[q][3]), acc));
+    }
+  }
+  const u64x2 w0 = S.w3[0][l], w1 = S.w3[1][l];
+  const f32x4 b = S.b3[g];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    y[q] = mfma32(w1, cat_bf16(a2[q][2], a2[q][3]), mfma32(w0, cat_bf16(a2[q][0], a2[q][1]), b));
 }
 
 // The standardisation of the network's inputs and outputs (FP32, applied
@@ -253,12 +268,18 @@ __device__ __forceinline__ float2 mlp_rows64(const MlpShared& S, const float* __
     xw[l * (kInPad / 4) + g] = (uint64_t)pk2bf(v[0], v[1]) | ((uint64_t)pk2bf(v[2], v[3]) << 32);
   }
   __syncthreads();
+  bf16x8 in[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     // inputs 4grp..4grp+3 and 16+4grp..16+4grp+3 of node 16q + col
     const uint64_t* row = xw + (16 * q + col) * (kInPad / 4);
-    const f32x4 y = mlp_pass(S, cat_bf16(row[grp], row[4 + grp]));
-    if (grp == 0) ys[16 * q + col] = make_float2(y[0], y[1]);
+    in[q] = cat_bf16(row[grp], row[4 + grp]);
+  }
+  f32x4 y[4];
+  mlp_wave(S, in, y);
+  if (grp == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ys[16 * q + col] = make_float2(y[q][0], y[q][1]);
   }
   __syncthreads();
   return ys[l];

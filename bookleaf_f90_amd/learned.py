@@ -75,7 +75,7 @@ def run_to(mode, net=None, tend=T_END, nx=1024, ny=512):
         g.relax_learned(net)
     info = {}
     try:
-        g.run(1, runner="run_graph")
+        g.run(1, stop_at_time_end=True, runner="run_graph")
         g.synchronize()
         t0 = time.perf_counter()
         while g.ts.time < tend:
