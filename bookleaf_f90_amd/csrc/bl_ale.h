@@ -20,6 +20,16 @@ __device__ __forceinline__ NbFc load_nbfc(int e, const int32_t* ielel, const int
   return r;
 }
 
+// the same from the packed table of the device-resident step: per face
+// (neighbour - 1) << 2 | (its face - 1), one int32 instead of two
+__device__ __forceinline__ NbFc load_nbfc_packed(int e, const int32_t* nbfc) {
+  const int4 a = reinterpret_cast<const int4*>(nbfc)[e];
+  NbFc r;
+  r.nb[0] = a.x >> 2; r.nb[1] = a.y >> 2; r.nb[2] = a.z >> 2; r.nb[3] = a.w >> 2;
+  r.fc[0] = a.x & 3; r.fc[1] = a.y & 3; r.fc[2] = a.z & 3; r.fc[3] = a.w & 3;
+  return r;
+}
+
 // ale_kn_sumflux for one element (ale_kn_advectors.f90:216-244)
 __device__ __forceinline__ double sumflux_el(int e, int id1, int id2, const int nb[4],
                                              const int fc[4], const double* __restrict__ fl) {

@@ -614,7 +614,7 @@ int bl_destroy(bl_handle_t* h) {
                   h->d_ale_ptr, h->d_ale_idx, h->d_el_ptr, h->d_el_idx, h->d_part,
                   h->d_parti, h->d_ticket, h->d_dtres, h->d_ts, h->d_hist, h->d_outer,
                   h->d_diag_hist, h->d_diag_count, h->d_diag_part, h->d_ring, h->d_work,
-                  h->d_rlist, h->d_zlist, h->d_lflag, h->d_lmark, h->d_ld1, h->d_cw[0], h->d_cw[1],
+                  h->d_rlist, h->d_zlist, h->d_lflag, h->d_lmark, h->d_ld1, h->d_nbfc, h->d_cw[0], h->d_cw[1],
                   h->d_eoff, h->d_lboff, h->d_einst, h->d_ninst, h->d_inst, h->d_inst_dt, h->d_cnt,
                   h->d_epart, h->d_eparti, h->d_ehist};
   for (void* p : bufs) if (p) hipFree(p);
@@ -701,6 +701,7 @@ int bl_upload(bl_handle_t* h, int32_t id, const void* host, int64_t count) {
   BL_HIP(hipMemcpyAsync(h->f[id], host, count * kind_bytes(h->kind[id]), hipMemcpyHostToDevice, h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));
   if (id == BL_IELND || id == BL_IELSORT1 || id == BL_IELSORT2) { h->maps_built = false; invalidate_graph(h); }
+  if (id == BL_IELEL || id == BL_IELFC) { h->nbfc_stale = true; invalidate_graph(h); }
   return BL_SUCCESS;
 }
 
@@ -825,6 +826,7 @@ int bl_build_maps(bl_handle_t* h) {
   if (build_csr(h, nodes, corners, &h->d_el_ptr, &h->d_el_idx)) return BL_FAILURE;
   if (build_outer(h, ielnd)) return BL_FAILURE;
   if (h->relax_type != BL_RELAX_OFF && relax_rebuild(h)) return BL_FAILURE;
+  h->nbfc_stale = true;
   h->maps_built = true;
   invalidate_graph(h);
   return BL_SUCCESS;

@@ -228,22 +228,6 @@ __device__ __forceinline__ double f1_moving(const FusedArgs& A, const FusedNodes
   return (c == c) ? c : DBL_MAX;
 }
 
-// F1L: the worklist's elements, a fixed grid striding over the count F1
-// left; the CFL partials of its workgroups follow F1's (part + 3 * nb1)
-__global__ void __launch_bounds__(kBlock)
-k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
-  if (step_skipped(A, A.ts)) return;
-  const uint32_t count = min(*A.work_count, (uint32_t)A.nel);
-  double best = DBL_MAX;
-  int bi = 0x7fffffff;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
-       k += gridDim.x * blockDim.x) {
-    const int e = A.work[k];
-    better_min(best, bi, f1_moving(A, N, e), e + 1);
-  }
-  block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
-}
-
 // time_dr_calc / time_dr_end's selection (time_dr_advance.f90:38-132):
 // strict '<' minimum over MAXIMUM, GROWTH, INITIAL (first step), CFL, DIV
 __device__ __forceinline__ void select_dt(const FusedArgs& A, double dt_prev, int nstep,
@@ -261,8 +245,27 @@ __device__ __forceinline__ void select_dt(const FusedArgs& A, double dt_prev, in
   if (c < best) { best = c; bidx = div_idx; bsdt = BL_DT_DIV; }
 }
 
-// F1b: one workgroup -- combine the partials of F1, then the hydro_dr_getdt
-// candidates and time_dr_calc / time_dr_end (time_dr_advance.f90:38-132)
+// F1L: the worklist's elements, a fixed grid striding over the count F1
+// left; the CFL partials of its workgroups follow F1's (part + 3 * nb1)
+__global__ void __launch_bounds__(kBlock)
+k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
+  if (step_skipped(A, A.ts)) return;
+  const uint32_t count = min(*A.work_count, (uint32_t)A.nel);
+  double best = DBL_MAX;
+  int bi = 0x7fffffff;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
+       k += gridDim.x * blockDim.x) {
+    const int e = A.work[k];
+    better_min(best, bi, f1_moving(A, N, e), e + 1);
+  }
+  block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
+}
+
+// F1b: one workgroup -- combine the partials of F1 and F1L, then the
+// hydro_dr_getdt candidates and time_dr_calc / time_dr_end
+// (time_dr_advance.f90:38-132).  (A separate launch: handing this to F1L's
+// last workgroup needs a device-scope fence in every workgroup, which costs
+// far more than the launch with an L2 per XCD.)
 __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb, int nbl) {
   DevTimestep* ts = A.ts;
   const bool skip = step_skipped(A, ts);
@@ -535,14 +538,14 @@ __device__ __forceinline__ void mix_f4(const FusedArgs& A, int imx, const Iso& g
   }
 }
 
-// F2m: one thread per mixed cell, after F2 (whose corrector corner force of
-// the cell it replaces): recomputes the cell's geometry at x^n and at the
-// half step exactly as F2 does, then the component terms
-__global__ void __launch_bounds__(kBlock) k_f2_mix(FusedArgs A, FusedNodes N) {
-  const DevTimestep* ts = A.ts;
-  if (ts->skip) return;
-  const int imx = xcd_index();
-  if (imx >= A.mx.nmx) return;
+// F2m, in the workgroups F2 appends after its elements: one thread per
+// mixed cell; F2 leaves the cell's corrector corner force to it.  It
+// recomputes the cell's geometry at x^n and at the half step exactly as F2
+// does, then the component terms, whose frvolume average is the cell's
+// force.  Nothing else F2 reads or writes is touched, so the two run side by
+// side.
+__device__ __forceinline__ void f2_mix_cell(const FusedArgs& A, const FusedNodes& N,
+                                            const DevTimestep* ts, int imx) {
   const int e = A.mx.imxel[imx] - 1;
   const double h = 0.5 * ts->dt;
   int nd[4];
@@ -564,12 +567,11 @@ __global__ void __launch_bounds__(kBlock) k_f2_mix(FusedArgs A, FusedNodes N) {
   }
 }
 
-// F4m: one thread per mixed cell, after F4: the components' corrector state
-__global__ void __launch_bounds__(kBlock) k_f4_mix(FusedArgs A, FusedNodes O) {
-  const DevTimestep* ts = A.ts;
-  if (ts->skip || A.dtres->negvol) return;
-  const int imx = xcd_index();
-  if (imx >= A.mx.nmx) return;
+// F4m, in the workgroups F5 appends after its first: the components'
+// corrector state of a mixed cell (F4 writes only the cell's own fields,
+// which this does not read)
+__device__ __forceinline__ void f4_mix_cell(const FusedArgs& A, const FusedNodes& O,
+                                            const DevTimestep* ts, int imx) {
   const int e = A.mx.imxel[imx] - 1;
   int nd[4];
   load_nd(A.ielnd, e, nd);
@@ -582,10 +584,18 @@ __global__ void __launch_bounds__(kBlock) k_f4_mix(FusedArgs A, FusedNodes O) {
 
 // ---------------------------------------------------------------- F2
 __global__ void __launch_bounds__(kBlock)
-k_f2_force(FusedArgs A, FusedNodes N) {
+k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   const DevTimestep* ts = A.ts;
   if (ts->skip) return;
-  const int e = xcd_index();
+  // the mixed cells' workgroups come first, so that their longer serial
+  // work starts with the grid rather than trailing it; nbm is a multiple of
+  // the XCD count, so the element workgroups keep their XCD mapping
+  if ((int)blockIdx.x < nbm) {
+    const int imx = (int)blockIdx.x * kBlock + threadIdx.x;
+    if (imx < A.mx.nmx) f2_mix_cell(A, N, ts, imx);
+    return;
+  }
+  const int e = xcd_block_of((int)blockIdx.x - nbm, (int)gridDim.x - nbm) * kBlock + threadIdx.x;
   if (e >= A.nel) return;
   const double dt = A.einst ? A.inst_dt[A.einst[e]] : ts->dt;
   const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
@@ -643,14 +653,18 @@ k_f2_force(FusedArgs A, FusedNodes N) {
   double cm[4];
   ld4(A.cnmass, e, cm);
   const int nel = A.nel;
+  // a mixed cell's force comes from its components (F2m)
+  const bool own_force = !(A.mx.ncp > 0 && A.ielmat[e] < 0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     double mt;
     if (cm[j] > A.zerocut) mt = cm[j];
     else if (cm[(j + 3) & 3] > A.zerocut) mt = cm[(j + 3) & 3];
     else mt = rhoh * gh.wt[j];
-    A.fxT[j * nel + e] = fx[j];
-    A.fyT[j * nel + e] = fy[j];
+    if (own_force) {
+      A.fxT[j * nel + e] = fx[j];
+      A.fyT[j * nel + e] = fy[j];
+    }
     A.wtT[j * nel + e] = gh.wt[j];
     A.mT[j * nel + e] = mt;
   }
@@ -755,6 +769,7 @@ struct AleArgs {
   const int32_t* ielnd;
   const int32_t* ielel;
   const int32_t* ielfc;
+  const int32_t* nbfc;  // ielel and ielfc packed per face (load_nbfc_packed)
   const int32_t* indtype;
   const int32_t* ielmat;
   const int32_t* eos_itype;
@@ -808,13 +823,37 @@ __device__ __forceinline__ void remap_faces(const AleArgs& B, const int nd[4], d
 
 // R1: ale_kn_sumflux(fcdv) -> volume change, ale_kn_fluxelvl(cnwt, fcdv,
 // rho) -> face mass fluxes (ale_dr_advect.f90:175-190)
-__global__ void __launch_bounds__(kBlock) k_r1(AleArgs B) {
-  REMAP_GATE(B);
+// With FOLD (Eulerian ALE: no kernel between F4 and R1 reads what F5
+// writes; the mixed cells' F4m then runs in R4), R1 also does F5's work: workgroup 0's first thread
+// records the step's negative-volume outcome in ts, and every workgroup
+// tests the negative-volume flags itself rather than the ts->skip that
+// thread may be setting.
+template <bool FOLD>
+__global__ void __launch_bounds__(kBlock) k_r1(AleArgs B, DevTimestep* tsw) {
+  if (FOLD) {
+    const DtResult* res = B.dtres;
+    const bool skip0 = tsw->skip != 0;  // as F1b left it
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !skip0) {
+      if (res->negvol) {
+        tsw->err = 1; tsw->err_idx = -res->negvol; tsw->done = 1; tsw->skip = 1;
+        tsw->err_step = tsw->nstep;
+      } else {
+        tsw->flips = tsw->flips + 1;
+        if (res->negvol2) {
+          tsw->err = 1; tsw->err_idx = -res->negvol2; tsw->done = 1; tsw->skip = 1;
+          tsw->err_step = tsw->nstep;
+        }
+      }
+    }
+    if (skip0 || res->negvol || res->negvol2 || !tsw->ale_on) return;
+  } else {
+    REMAP_GATE(B);
+  }
   const int e = xcd_index();
   if (e >= B.nel) return;
   int nd[4];
   load_nd(B.ielnd, e, nd);
-  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  const NbFc c = load_nbfc_packed(e, B.nbfc);
   double ax[4], ay[4], bx[4], by[4], own[4], nbv[4];
   remap_faces(B, nd, B.ts->dt, ax, ay, bx, by, own, nbv);
   B.dv[e] = sumflux_regs(e, 0, 1, c.nb, own, nbv);
@@ -830,7 +869,7 @@ __global__ void __launch_bounds__(kBlock) k_r2(AleArgs B) {
   REMAP_GATE(B);
   const int e = xcd_index();
   if (e >= B.nel) return;
-  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  const NbFc c = load_nbfc_packed(e, B.nbfc);
   const double totm = sumflux_el(e, 0, 1, c.nb, c.fc, B.fcdm);
   const double v0 = B.vol[e], m0 = B.mass[e];
   B.v0[e] = v0;
@@ -850,7 +889,7 @@ __global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
   if (e >= B.nel) return;
   int nd[4];
   load_nd(B.ielnd, e, nd);
-  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  const NbFc c = load_nbfc_packed(e, B.nbfc);
   // ale_kn_updateel
   const double s = sumflux_el(e, 0, 1, c.nb, c.fc, B.flux);
   const double m1 = B.mass[e], v0 = B.v0[e];
@@ -912,11 +951,24 @@ __global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
 
 // R4: ale_kn_fluxndvl for both velocity components (ale_dr_advect.f90:
 // 241-268; the v flux reads the corner v gathered before the u update)
-__global__ void __launch_bounds__(kBlock) k_r4(AleArgs B) {
+// With MIX (R1 does F5's work and the step has mixed cells), R4's first
+// nbm8 workgroups are F4m, one thread per mixed cell, gated as F5 gated
+// it: not skipped, whether or not this step remaps (ts->skip now includes
+// the step's negative volumes, recorded by R1).
+template <bool MIX>
+__global__ void __launch_bounds__(kBlock) k_r4(AleArgs B, FusedArgs A, FusedNodes O, int nbm8) {
+  if (MIX && (int)blockIdx.x < nbm8) {
+    if (A.ts->skip) return;
+    const int imx = (int)blockIdx.x * kBlock + threadIdx.x;
+    if (imx < A.mx.nmx) f4_mix_cell(A, O, A.ts, imx);
+    return;
+  }
   REMAP_GATE(B);
-  const int e = xcd_index();
+  const int e = MIX ? xcd_block_of((int)blockIdx.x - nbm8, (int)gridDim.x - nbm8) * kBlock +
+                          threadIdx.x
+                    : xcd_index();
   if (e >= B.nel) return;
-  const NbFc c = load_nbfc(e, B.ielel, B.ielfc);
+  const NbFc c = load_nbfc_packed(e, B.nbfc);
   double out[4];
   const int nel = B.nel;
   fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnu, out);
@@ -970,8 +1022,20 @@ __global__ void __launch_bounds__(kBlock) k_r5(AleArgs B) {
 }
 
 // negative volume found by F2 (F3/F4 did not run) or F4 (F3 did): stop the
-// batch at this step; the ALE kernels that follow are gated by skip
-__global__ void k_f5_end(DevTimestep* __restrict__ ts, const DtResult* __restrict__ res) {
+// batch at this step; the ALE kernels that follow are gated by skip.
+// Workgroups from 1 on are F4m, one thread per mixed cell: they run when
+// neither F2 nor F4 found a negative volume (the reference halts in the
+// corrector's geometry before the component state), a test that does not
+// depend on workgroup 0's update of ts.
+__global__ void __launch_bounds__(kBlock) k_f5_end(FusedArgs A, FusedNodes O) {
+  DevTimestep* ts = A.ts;
+  const DtResult* res = A.dtres;
+  if (blockIdx.x > 0) {
+    if (ts->skip || res->negvol || res->negvol2) return;
+    const int imx = ((int)blockIdx.x - 1) * kBlock + threadIdx.x;
+    if (imx < A.mx.nmx) f4_mix_cell(A, O, ts, imx);
+    return;
+  }
   if (threadIdx.x != 0 || ts->skip) return;
   if (res->negvol) {
     ts->err = 1; ts->err_idx = -res->negvol; ts->done = 1; ts->skip = 1;
@@ -985,9 +1049,25 @@ __global__ void k_f5_end(DevTimestep* __restrict__ ts, const DtResult* __restric
   }
 }
 
-// negative volume in the ALE hydro update (ale_dr_update.f90:44)
-__global__ void k_f7_aleend(DevTimestep* __restrict__ ts, const DtResult* __restrict__ res) {
-  if (threadIdx.x != 0 || ts->skip || !ts->ale_on) return;
+// negative volume in the ALE hydro update (ale_dr_update.f90:44).
+// Workgroups from 1 on are the mixed cells' part of the hydro update (one
+// thread per cell, on the remapped mesh x, y); they run unless the update
+// found a negative volume (the reference halts in its geometry before the
+// component state), a test that does not depend on workgroup 0's update of
+// ts.
+__global__ void __launch_bounds__(kBlock) k_f7_aleend(FusedArgs A, const double* __restrict__ x,
+                                                      const double* __restrict__ y) {
+  DevTimestep* ts = A.ts;
+  const DtResult* res = A.dtres;
+  if (ts->skip || !ts->ale_on) return;
+  if (blockIdx.x > 0) {
+    if (res->negvol) return;
+    const int imx = ((int)blockIdx.x - 1) * kBlock + threadIdx.x;
+    if (imx < A.mx.nmx)
+      mix_hydroupdate_cell(A.mx, imx, A.ielnd, x, y, A.eos_itype, A.eos_param, A.pcut, A.ccut);
+    return;
+  }
+  if (threadIdx.x != 0) return;
   if (res->negvol) {
     ts->err = 1; ts->err_idx = -res->negvol; ts->done = 1; ts->skip = 1;
     ts->err_step = ts->nstep;
@@ -1054,6 +1134,7 @@ AleArgs make_ale(bl_handle_t* h, const FusedNodesOut& O) {
   B.zerocut = h->cfg.zerocut; B.dencut = h->cfg.dencut;
   B.pcut = h->cfg.pcut; B.ccut = h->cfg.ccut;
   B.ielnd = h->I(BL_IELND); B.ielel = h->I(BL_IELEL); B.ielfc = h->I(BL_IELFC);
+  B.nbfc = h->d_nbfc;
   B.indtype = h->I(BL_INDTYPE); B.ielmat = h->I(BL_IELMAT);
   B.eos_itype = h->d_eos_itype; B.eos_param = h->d_eos_param;
   B.ale_ptr = h->d_ale_ptr; B.ale_idx = h->d_ale_idx;
@@ -1090,12 +1171,16 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
   BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, kBlock, A, N, (int)nbe);
   BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe, (int)nbl);
-  BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
-  if (A.mx.ncp > 0) BL_LAUNCH(h, "f2_mix", k_f2_mix, nblocks(A.mx.nmx), kBlock, A, N);
+  // F2 and F5 (or R1) take a workgroup per 256 mixed cells for the
+  // component work; F2 puts them first, rounded up to whole XCD rounds
+  const unsigned nbm = A.mx.ncp > 0 ? nblocks(A.mx.nmx) : 0;
+  const unsigned nbm8 = (nbm + kNumXcd - 1) / kNumXcd * kNumXcd;
+  BL_LAUNCH(h, "f2_force", k_f2_force, nbm8 + nbe, kBlock, A, N, (int)nbm8);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
   BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
-  if (A.mx.ncp > 0) BL_LAUNCH(h, "f4_mix", k_f4_mix, nblocks(A.mx.nmx), kBlock, A, Oc);
-  BL_LAUNCH(h, "f5_end", k_f5_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  // Eulerian ALE: F5 runs inside R1 and F4m inside R4
+  const bool fold_f5 = h->cfg.zexist && h->cfg.zeul;
+  if (!fold_f5) BL_LAUNCH(h, "f5_end", k_f5_end, 1 + nbm, kBlock, A, Oc);
   if (h->cfg.zexist) {
     // the remap on the node set this step wrote
     AleArgs B = make_ale(h, O);
@@ -1103,13 +1188,20 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     if (!h->cfg.zeul &&
         mesh_motion(h, O.x, O.y, h->R(BL_RSCRATCH14), h->R(BL_RSCRATCH15), 0.0, h->d_ts))
       return BL_FAILURE;
-    BL_LAUNCH(h, "r1_volflux", k_r1, nbe, kBlock, B);
+    if (fold_f5)
+      BL_LAUNCH(h, "r1_volflux", k_r1<true>, nbe, kBlock, B, h->d_ts);
+    else
+      BL_LAUNCH(h, "r1_volflux", k_r1<false>, nbe, kBlock, B, h->d_ts);
     BL_LAUNCH(h, "r2_massflux", k_r2, nbe, kBlock, B);
     BL_LAUNCH(h, "r3_energy_update", k_r3, nbe, kBlock, B);
-    BL_LAUNCH(h, "r4_momflux", k_r4, nbe, kBlock, B);
+    if (fold_f5 && nbm > 0)
+      BL_LAUNCH(h, "r4_momflux", k_r4<true>, nbm8 + nbe, kBlock, B, A, Oc, (int)nbm8);
+    else
+      BL_LAUNCH(h, "r4_momflux", k_r4<false>, nbe, kBlock, B, A, Oc, 0);
     BL_LAUNCH(h, "r5_nodes", k_r5, nbn, kBlock, B);
-    if (mix_hydroupdate(h, O.x, O.y, h->d_ts)) return BL_FAILURE;
-    BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+    // F7, with the mixed cells' hydro update in its appended workgroups
+    BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1 + nbm, kBlock, A, (const double*)O.x,
+              (const double*)O.y);
   }
   return launch_diag(h, O.x, O.y, O.u, O.v);
 }
@@ -1141,8 +1233,34 @@ void point_fields_at(bl_handle_t* h, int s) {
   h->cur_set = s;
 }
 
+// the packed neighbour/face table of the fused remap, from ielel and ielfc
+// (rebuilt at a capture after they were uploaded or the maps rebuilt)
+constexpr int kMinHistory = 8192;  // dt-history records a graph run can hold
+
+int build_nbfc(bl_handle_t* h) {
+  const int nel = h->sz.nel;
+  std::vector<int32_t> el(4 * (size_t)nel), fc(4 * (size_t)nel), pk(4 * (size_t)nel);
+  if (bl_download(h, BL_IELEL, el.data(), 4 * (int64_t)nel) ||
+      bl_download(h, BL_IELFC, fc.data(), 4 * (int64_t)nel))
+    return BL_FAILURE;
+  for (size_t k = 0; k < pk.size(); ++k) {
+    if (el[k] < 1 || el[k] > nel || fc[k] < 1 || fc[k] > 4) {
+      set_error("ielel/ielfc out of range for the remap (element " +
+                std::to_string(k / 4 + 1) + ")");
+      return BL_FAILURE;
+    }
+    pk[k] = ((el[k] - 1) << 2) | (fc[k] - 1);
+  }
+  if (!h->d_nbfc)
+    BL_HIP(hipMalloc(&h->d_nbfc, sizeof(int32_t) * std::max<size_t>(pk.size(), 4)));
+  BL_HIP(hipMemcpy(h->d_nbfc, pk.data(), sizeof(int32_t) * pk.size(), hipMemcpyHostToDevice));
+  h->nbfc_stale = false;
+  return BL_SUCCESS;
+}
+
 int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
   invalidate_graph(h);
+  if (h->cfg.zexist && (h->nbfc_stale || !h->d_nbfc) && build_nbfc(h)) return BL_FAILURE;
   bool timing = h->timing;
   h->timing = false;  // no event records inside a capture
   for (int p = 0; p < 2; ++p) {
@@ -1255,10 +1373,10 @@ int launch_ensemble_step(bl_handle_t* h, int p) {
   BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nbe, kBlock, A, N, E);
   BL_LAUNCH(h, "e1_limiter", k_e1_limiter, (unsigned)h->ens_lblocks, kBlock, A, N, E);
   BL_LAUNCH(h, "e1b_select", k_e1b_select, (unsigned)h->ens_n, kBlock, A, E, (int)nbe);
-  BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N);
+  BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N, 0);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
   BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
-  BL_LAUNCH(h, "f5_end", k_f5_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  BL_LAUNCH(h, "f5_end", k_f5_end, 1, kBlock, A, Oc);
   return BL_SUCCESS;
 }
 
@@ -1304,10 +1422,12 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   if (nsteps <= 0) { if (steps_taken) *steps_taken = 0; return BL_SUCCESS; }
   if (ensure_nodesets(h)) return BL_FAILURE;
   if (h->ehist_cap < nsteps) {
+    // as bl_run_graph: room for runs of up to 1024 steps without re-capture
+    const int cap = std::max(nsteps, 1024);
     if (h->d_ehist) hipFree(h->d_ehist);
     h->d_ehist = nullptr;
-    BL_HIP(hipMalloc(&h->d_ehist, sizeof(double) * 3 * (size_t)nsteps * h->ens_n));
-    h->ehist_cap = nsteps;
+    BL_HIP(hipMalloc(&h->d_ehist, sizeof(double) * 3 * (size_t)cap * h->ens_n));
+    h->ehist_cap = cap;
     invalidate_graph(h);
   }
   if (!h->graph_exec[0] || h->graph_stop_at_end != -2)
@@ -1397,10 +1517,13 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   fill_err(err, BL_SUCCESS, 0, "");
   if (ensure_nodesets(h)) return BL_FAILURE;
   if (h->hist_cap < nsteps) {
+    // the graphs bake in the history's capacity: size it generously so that
+    // runs of different lengths do not re-capture
+    const int cap = std::max(nsteps, kMinHistory);
     if (h->d_hist) hipFree(h->d_hist);
     h->d_hist = nullptr;
-    BL_HIP(hipMalloc(&h->d_hist, sizeof(double) * 3 * std::max(nsteps, 1)));
-    h->hist_cap = nsteps;
+    BL_HIP(hipMalloc(&h->d_hist, sizeof(double) * 3 * cap));
+    h->hist_cap = cap;
     invalidate_graph(h);
   }
   // the graphs bake in the node set pointers, which are fixed; capture once

@@ -154,6 +154,8 @@ struct Handle {
   int32_t nzero = 0;
   int32_t* d_lflag = nullptr;     // learned-motion flags and counters
   int32_t* d_lmark = nullptr;     // nnd: node marked by the check for the guard
+  int32_t* d_nbfc = nullptr;      // (4, nel) ielel/ielfc packed for the fused remap
+  bool nbfc_stale = true;         // rebuild it before the next capture
   float* d_ld1 = nullptr;         // (nnd, 2): one-sweep displacement of the node
   // per-step diagnostics of the fused step: region table + Sod L1 records
   bool diag_on = false;

@@ -19,12 +19,14 @@ namespace bl {
 // the limiter and the remap) then hit the same L2 instead of being fetched
 // from HBM once per XCD.  A bijection on [0, gridDim.x).
 constexpr int kNumXcd = 8;
-__device__ __forceinline__ int xcd_block() {
-  const int b = blockIdx.x, n = gridDim.x;
+// (xcd_block_of(b, n): the same over the first n workgroups of a grid that
+// appends workgroups of another kind after them)
+__device__ __forceinline__ int xcd_block_of(int b, int n) {
   const int x = b % kNumXcd, local = b / kNumXcd;
   const int q = n / kNumXcd, r = n % kNumXcd;
   return x * q + (x < r ? x : r) + local;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_block_of(blockIdx.x, gridDim.x); }
 __device__ __forceinline__ int xcd_index() { return xcd_block() * blockDim.x + threadIdx.x; }
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }

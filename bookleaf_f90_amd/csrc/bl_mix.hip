@@ -44,28 +44,7 @@ __global__ void k_mx_hydroupdate(MixArgs M, const int32_t* __restrict__ ielnd,
   MIX_GATE(ts);
   const int imx = xcd_index();
   if (imx >= M.nmx) return;
-  const int iel = M.imxel[imx] - 1;
-  const int4 q = reinterpret_cast<const int4*>(ielnd)[iel];
-  const double x[4] = {ndx[q.x - 1], ndx[q.y - 1], ndx[q.z - 1], ndx[q.w - 1]};
-  const double y[4] = {ndy[q.x - 1], ndy[q.y - 1], ndy[q.z - 1], ndy[q.w - 1]};
-  Iso g;
-  getiso(x, y, g);
-  const double w1 = g.a1, w2 = g.a3, w3 = g.b1, w4 = g.b3;
-  int icp = M.imxfcp[imx] - 1;
-  for (int ii = 0; ii < M.imxncp[imx]; ++ii, ++icp) {
-    M.cpa1[icp] = w1; M.cpa3[icp] = w2; M.cpb1[icp] = w3; M.cpb3[icp] = w4;
-    const double cvol = (4.0 * (w1 * w4 - w2 * w3)) * M.frvolume[icp];
-    M.cpvolume[icp] = cvol;
-    const double r = M.cpmass[icp] / cvol;
-    M.cpdensity[icp] = r;
-    int mat = M.icpmat[icp];
-    mat = mat > 0 ? mat : 0;
-    const int et = eos_itype[mat];
-    const double* ep = eos_param + BL_NEOSPARAM * mat;
-    const double e = M.cpenergy[icp];
-    M.cppressure[icp] = eos_pressure(et, ep, r, e, pcut);
-    M.cpcs2[icp] = eos_cs2(et, ep, r, e, pcut, ccut);
-  }
+  mix_hydroupdate_cell(M, imx, ielnd, ndx, ndy, eos_itype, eos_param, pcut, ccut);
 }
 
 // density_dr_getdensity.f90:40-43
