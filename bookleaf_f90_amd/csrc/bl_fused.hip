@@ -954,13 +954,33 @@ __global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
 // With MIX (R1 does F5's work and the step has mixed cells), R4's first
 // nbm8 workgroups are F4m, one thread per mixed cell, gated as F5 gated
 // it: not skipped, whether or not this step remaps (ts->skip now includes
-// the step's negative volumes, recorded by R1).
+// the step's negative volumes, recorded by R1).  With fold7 they then do
+// the mixed cells' part of the hydro update as F7 did, on the positions R5
+// will give the nodes, x + dt w, computed here the same way.
 template <bool MIX>
-__global__ void __launch_bounds__(kBlock) k_r4(AleArgs B, FusedArgs A, FusedNodes O, int nbm8) {
+__global__ void __launch_bounds__(kBlock) k_r4(AleArgs B, FusedArgs A, FusedNodes O, int nbm8,
+                                               int fold7) {
   if (MIX && (int)blockIdx.x < nbm8) {
     if (A.ts->skip) return;
     const int imx = (int)blockIdx.x * kBlock + threadIdx.x;
-    if (imx < A.mx.nmx) f4_mix_cell(A, O, A.ts, imx);
+    if (imx >= A.mx.nmx) return;
+    f4_mix_cell(A, O, A.ts, imx);
+    if (fold7 && A.ts->ale_on && !A.dtres->negvol) {
+      int nd[4];
+      load_nd(A.ielnd, A.mx.imxel[imx] - 1, nd);
+      const double dt = A.ts->dt;
+      double px[4], py[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double wu = B.ubar[nd[c]], wv = B.vbar[nd[c]];
+        if (B.zeul) { wu = -wu; wv = -wv; }
+        px[c] = O.x[nd[c]] + dt * wu;
+        py[c] = O.y[nd[c]] + dt * wv;
+      }
+      Iso g;
+      getiso(px, py, g);
+      mix_hydroupdate_iso(A.mx, imx, g, A.eos_itype, A.eos_param, A.pcut, A.ccut);
+    }
     return;
   }
   REMAP_GATE(B);
@@ -983,7 +1003,17 @@ __global__ void __launch_bounds__(kBlock) k_r4(AleArgs B, FusedArgs A, FusedNode
 // in ALE order, ale_kn_cutbasisnd), ale_kn_activend with indstatus = 2
 // (Eulerian), ale_kn_updatend for u and v (element order), and the moved
 // node positions
-__global__ void __launch_bounds__(kBlock) k_r5(AleArgs B) {
+// With FOLD7 (Eulerian, no per-step diagnostics), R5's first thread also
+// does F7's check: a negative volume in the hydro update (R3's) records the
+// error and ends the run at this step (ts->done; ts->skip is left for the
+// next step's F1b, so that no workgroup of R5 depends on it).
+template <bool FOLD7>
+__global__ void __launch_bounds__(kBlock) k_r5(AleArgs B, DevTimestep* tsw) {
+  if (FOLD7 && blockIdx.x == 0 && threadIdx.x == 0 && !tsw->skip && tsw->ale_on &&
+      B.dtres->negvol) {
+    tsw->err = 1; tsw->err_idx = -B.dtres->negvol; tsw->done = 1;
+    tsw->err_step = tsw->nstep;
+  }
   REMAP_GATE(B);
   const int n = xcd_index();
   if (n >= B.nnd) return;
@@ -1194,14 +1224,21 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
       BL_LAUNCH(h, "r1_volflux", k_r1<false>, nbe, kBlock, B, h->d_ts);
     BL_LAUNCH(h, "r2_massflux", k_r2, nbe, kBlock, B);
     BL_LAUNCH(h, "r3_energy_update", k_r3, nbe, kBlock, B);
+    // with no per-step diagnostics, F7 folds into R4 (mixed cells) and R5
+    const bool fold_f7 = fold_f5 && !h->diag_on;
     if (fold_f5 && nbm > 0)
-      BL_LAUNCH(h, "r4_momflux", k_r4<true>, nbm8 + nbe, kBlock, B, A, Oc, (int)nbm8);
+      BL_LAUNCH(h, "r4_momflux", k_r4<true>, nbm8 + nbe, kBlock, B, A, Oc, (int)nbm8,
+                fold_f7 ? 1 : 0);
     else
-      BL_LAUNCH(h, "r4_momflux", k_r4<false>, nbe, kBlock, B, A, Oc, 0);
-    BL_LAUNCH(h, "r5_nodes", k_r5, nbn, kBlock, B);
-    // F7, with the mixed cells' hydro update in its appended workgroups
-    BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1 + nbm, kBlock, A, (const double*)O.x,
-              (const double*)O.y);
+      BL_LAUNCH(h, "r4_momflux", k_r4<false>, nbe, kBlock, B, A, Oc, 0, 0);
+    if (fold_f7) {
+      BL_LAUNCH(h, "r5_nodes", k_r5<true>, nbn, kBlock, B, h->d_ts);
+    } else {
+      BL_LAUNCH(h, "r5_nodes", k_r5<false>, nbn, kBlock, B, h->d_ts);
+      // F7, with the mixed cells' hydro update in its appended workgroups
+      BL_LAUNCH(h, "f7_aleend", k_f7_aleend, 1 + nbm, kBlock, A, (const double*)O.x,
+                (const double*)O.y);
+    }
   }
   return launch_diag(h, O.x, O.y, O.u, O.v);
 }

@@ -94,22 +94,13 @@ int mix_average_force(Handle* h);
 int mix_cpu_overlay(Handle* h, const DevTimestep* ts);
 
 
-// ale_dr_hydroupdate's component part for mixed cell imx: the cell's iso
-// terms on the remapped mesh to its components, then component volume,
-// density and EOS (component mass and energy are not remapped)
-__device__ __forceinline__ void mix_hydroupdate_cell(const MixArgs& M, int imx,
-                                                     const int32_t* __restrict__ ielnd,
-                                                     const double* __restrict__ ndx,
-                                                     const double* __restrict__ ndy,
-                                                     const int32_t* __restrict__ eos_itype,
-                                                     const double* __restrict__ eos_param,
-                                                     double pcut, double ccut) {
-  const int iel = M.imxel[imx] - 1;
-  const int4 q = reinterpret_cast<const int4*>(ielnd)[iel];
-  const double x[4] = {ndx[q.x - 1], ndx[q.y - 1], ndx[q.z - 1], ndx[q.w - 1]};
-  const double y[4] = {ndy[q.x - 1], ndy[q.y - 1], ndy[q.z - 1], ndy[q.w - 1]};
-  Iso g;
-  getiso(x, y, g);
+// ale_dr_hydroupdate's component part for mixed cell imx, from the cell's
+// iso terms g on the remapped mesh: component volume, density and EOS
+// (component mass and energy are not remapped)
+__device__ __forceinline__ void mix_hydroupdate_iso(const MixArgs& M, int imx, const Iso& g,
+                                                    const int32_t* __restrict__ eos_itype,
+                                                    const double* __restrict__ eos_param,
+                                                    double pcut, double ccut) {
   const double w1 = g.a1, w2 = g.a3, w3 = g.b1, w4 = g.b3;
   int icp = M.imxfcp[imx] - 1;
   for (int ii = 0; ii < M.imxncp[imx]; ++ii, ++icp) {
@@ -126,6 +117,23 @@ __device__ __forceinline__ void mix_hydroupdate_cell(const MixArgs& M, int imx,
     M.cppressure[icp] = eos_pressure(et, ep, r, e, pcut);
     M.cpcs2[icp] = eos_cs2(et, ep, r, e, pcut, ccut);
   }
+}
+
+// the same with the cell's nodes read from the remapped positions
+__device__ __forceinline__ void mix_hydroupdate_cell(const MixArgs& M, int imx,
+                                                     const int32_t* __restrict__ ielnd,
+                                                     const double* __restrict__ ndx,
+                                                     const double* __restrict__ ndy,
+                                                     const int32_t* __restrict__ eos_itype,
+                                                     const double* __restrict__ eos_param,
+                                                     double pcut, double ccut) {
+  const int iel = M.imxel[imx] - 1;
+  const int4 q = reinterpret_cast<const int4*>(ielnd)[iel];
+  const double x[4] = {ndx[q.x - 1], ndx[q.y - 1], ndx[q.z - 1], ndx[q.w - 1]};
+  const double y[4] = {ndy[q.x - 1], ndy[q.y - 1], ndy[q.z - 1], ndy[q.w - 1]};
+  Iso g;
+  getiso(x, y, g);
+  mix_hydroupdate_iso(M, imx, g, eos_itype, eos_param, pcut, ccut);
 }
 
 }  // namespace bl
