@@ -61,23 +61,22 @@ KERNEL_BYTES = {
     "f4_state": (132, 32),
     # ensemble F1: F1's bytes and the element instance id 2
     "e1_getdt": (98, 32),
-    # remap (ALE step), per element | per node:
-    # ielnd, ielel, ielfc 48, cnwt 32, rho 8; writes dv 8, face mass flux 32
-    # | x, y, mesh velocity 32
-    "r1_volflux": (128, 32),
-    # ielel, ielfc 32, face mass flux 32, V, m 16, cnmass 32, e 8; writes
-    # v0, m0, m 24, face energy flux 32
-    "r2_massflux": (176, 0),
-    # ielnd, ielel, ielfc 48, energy flux 32, m, v0, m0 24, e r/w 16, cnmass
-    # r/w 64, face mass flux 32, ielmat 4; writes corner u, v, m0, dm,
+    # remap (ALE step), per element | per node (the neighbours and their
+    # faces come from one packed int32 per face, 16 B an element):
+    # ielnd, packed ielel/ielfc 32, cnwt 32, rho 8; writes dv 8, face mass
+    # flux 32 | x, y, mesh velocity 32
+    "r1_volflux": (112, 32),
+    # packed ielel/ielfc 16, face mass flux 32, V, m 16, cnmass 32, e 8;
+    # writes v0, m0, m 24, face energy flux 32
+    "r2_massflux": (160, 0),
+    # ielnd, packed ielel/ielfc 32, energy flux 32, m, v0, m0 24, e r/w 16,
+    # cnmass r/w 64, face mass flux 32, ielmat 4; writes corner u, v, m0, dm,
     # mass flux, m0 (corner-major) 192, rho, p, cs2 24 | u, v, x, y, mesh
     # velocity 48
-    "r3_energy_update": (444, 48),
-    # ielel, ielfc 32, corner m0, dm, u, v 128; writes corner fluxes 64 (a
-    # corner with zero mass flux reads only its flux: at rest fewer bytes)
-    "r4_momflux": (224, 0),
-    # v0 8, corner m0, mass flux 64, corner fluxes u, v 64 | CSR ptr/idx
-    # (two orders) 40, indtype 4, u, v, x, y r/w 64, mesh velocity 16
+    "r3_energy_update": (428, 48),
+    # packed ielel/ielfc 16, corner m0, dm, u, v 128; writes corner fluxes 64
+    # (a corner with zero mass flux reads only its flux: at rest fewer bytes)
+    "r4_momflux": (208, 0),
     "r5_nodes": (136, 124),
     # node relaxation sweeps: stencil table 32, x, y 16, writes 16 | the last
     # sweep also reads the Lagrangian x, y 16
