@@ -117,6 +117,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="learned: skip the equal-time comparison with the plain relaxation")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape of the limiter pass instead of "
+                         "timing the candidates (bl_autotune) before the last warm-up step")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -238,8 +241,15 @@ def main():
     print("# workload %s nel=%d nnd=%d" % (wname, nel, nnd), file=sys.stderr, flush=True)
     if args.diagnostics:
         g.set_diagnostics(True, maxrec=args.warmup + args.steps + args.profile_steps)
+    # the launch shape is tuned on the state the timed steps start from; the
+    # last warm-up step then captures the step graphs with it
+    tune = None
     if args.warmup:
-        run(args.warmup)
+        run(args.warmup - 1)
+        if not args.no_autotune:
+            tune = g.autotune(reps=5)
+            print("# autotune %s" % tune, file=sys.stderr, flush=True)
+        run(1)
     g.synchronize()
     t0 = time.perf_counter()
     run(args.steps)
@@ -308,7 +318,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wname, "nel": nel, "nnd": nnd, "runner": args.runner,
-                   "diagnostics": bool(args.diagnostics)},
+                   "diagnostics": bool(args.diagnostics),
+                   "launch": tune if tune is not None else g.get_tuning()},
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,

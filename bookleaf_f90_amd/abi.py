@@ -107,6 +107,13 @@ class FlagTotals(C.Structure):
     _fields_ = [(n, C.c_double) for n in FLAG_TOTALS]
 
 
+class Tuning(C.Structure):
+    """The launch shape bl_autotune chose."""
+    _fields_ = [("limiter_block", C.c_int32), ("limiter_grid", C.c_int32),
+                ("ens_chunk", C.c_int32), ("trials", C.c_int32),
+                ("best_us", C.c_double), ("default_us", C.c_double)]
+
+
 def field_shape(sizes, fid):
     """(kind, count) of a dataAPI field, as src/utilities/data/data.f90:169-430."""
     nel, nnd = sizes.nel2, sizes.nnd2

@@ -580,6 +580,7 @@ int bl_create(const bl_sizes_t* sizes, bl_handle_t** out) {
   }
   hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t) * 16, h->stream);
   hipMemsetAsync(h->d_dtres, 0, sizeof(DtResult), h->stream);
+  hipMemsetAsync(h->d_ts, 0, sizeof(DevTimestep), h->stream);
   if (hipStreamSynchronize(h->stream) != hipSuccess) {
     set_error("bl_create: initialisation failed"); bl_destroy(h); return BL_FAILURE;
   }

@@ -185,6 +185,41 @@ int dev_alloc(T** p, size_t n) {
 
 }  // namespace
 
+namespace bl {
+
+// the limiter pass's workgroups per instance, ceil(nel_k / chunk), their
+// offsets lboff on the device, and room for their partials
+int ens_set_chunk(Handle* hh, int chunk) {
+  bl_handle_t* h = static_cast<bl_handle_t*>(hh);
+  const int ninst = h->ens_n;
+  if (chunk < 64) {
+    set_error("ensemble limiter chunk must be >= 64 elements");
+    return BL_FAILURE;
+  }
+  std::vector<int32_t> lboff(ninst + 1, 0);
+  for (int k = 0; k < ninst; ++k)
+    lboff[k + 1] = lboff[k] + (h->ens_eoff[k + 1] - h->ens_eoff[k] + chunk - 1) / chunk;
+  const size_t nb = nblocks(h->sz.nel);
+  if (lboff[ninst] > h->ens_lblocks_cap) {
+    hipFree(h->d_epart);
+    hipFree(h->d_eparti);
+    h->d_epart = nullptr;
+    h->d_eparti = nullptr;
+    h->ens_lblocks_cap = 0;
+    BL_HIP(hipMalloc(&h->d_epart, sizeof(double) * (4 * nb + (size_t)lboff[ninst])));
+    BL_HIP(hipMalloc(&h->d_eparti, sizeof(int32_t) * (4 * nb + (size_t)lboff[ninst])));
+    h->ens_lblocks_cap = lboff[ninst];
+  }
+  BL_HIP(hipMemcpy(h->d_lboff, lboff.data(), sizeof(int32_t) * (ninst + 1),
+                   hipMemcpyHostToDevice));
+  h->ens_lblocks = lboff[ninst];
+  h->tune_ens_chunk = chunk;
+  invalidate_graph(h);
+  return BL_SUCCESS;
+}
+
+}  // namespace bl
+
 extern "C" {
 
 int bl_ensemble_create(const bl_instance_t* inst, int32_t ninst, const bl_config_t* cfg,
@@ -251,10 +286,6 @@ int bl_ensemble_create(const bl_instance_t* inst, int32_t ninst, const bl_config
   h->ens_n = ninst;
   h->ens_eoff = eoff;
   h->ens_noff = noff;
-  std::vector<int32_t> lboff(ninst + 1, 0);
-  for (int k = 0; k < ninst; ++k)
-    lboff[k + 1] = lboff[k] + (eoff[k + 1] - eoff[k] + kEnsChunk - 1) / kEnsChunk;
-  h->ens_lblocks = lboff[ninst];
   int32_t* d_noff = nullptr;
   bl_instance_t* d_inst = nullptr;
   const size_t nel = ne, nnd = nn, nb = nblocks((int)ne);
@@ -265,14 +296,11 @@ int bl_ensemble_create(const bl_instance_t* inst, int32_t ninst, const bl_config
       dev_alloc(&h->d_el_ptr, nnd + 1) || dev_alloc(&h->d_el_idx, 4 * nel) ||
       dev_alloc(&h->d_outer, 8 * nel) || dev_alloc((InstTs**)&h->d_inst, ninst) ||
       dev_alloc(&h->d_inst_dt, ninst) || dev_alloc(&h->d_cnt, ninst) ||
-      dev_alloc(&h->d_lboff, ninst + 1) ||
-      dev_alloc(&h->d_epart, 4 * nb + (size_t)h->ens_lblocks) ||
-      dev_alloc(&h->d_eparti, 4 * nb + (size_t)h->ens_lblocks)) {
+      dev_alloc(&h->d_lboff, ninst + 1) || ens_set_chunk(h, h->tune_ens_chunk)) {
     hipFree(d_noff); hipFree(d_inst);
     return fail();
   }
   hipMemcpy(h->d_eoff, eoff.data(), sizeof(int32_t) * (ninst + 1), hipMemcpyHostToDevice);
-  hipMemcpy(h->d_lboff, lboff.data(), sizeof(int32_t) * (ninst + 1), hipMemcpyHostToDevice);
   hipMemcpy(d_noff, noff.data(), sizeof(int32_t) * (ninst + 1), hipMemcpyHostToDevice);
   hipMemcpy(d_inst, inst, sizeof(bl_instance_t) * ninst, hipMemcpyHostToDevice);
   std::vector<InstTs> its(ninst);

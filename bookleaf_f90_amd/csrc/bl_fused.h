@@ -119,10 +119,5 @@ struct EnsArgs {
   double* hist;
   int hist_cap;
 };
-// F1L gives instance k ceil(nel_k / kEnsChunk) workgroups
-constexpr int kEnsChunk = 4096;
-
-// F1L's fixed grid: enough workgroups to fill the chip at F1L's occupancy
-constexpr unsigned kLimiterBlocks = 512;
 
 }  // namespace bl

@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
 // union of the instances' meshes, instance k owning elements [eoff[k],
 // eoff[k+1]).  Every instance has at least kBlock elements, so a workgroup
 // of F1 spans at most two instances (a and b); it keeps two partial slots of
-// each control.  F1L gives each instance one workgroup per kEnsChunk of its
+// each control.  F1L gives each instance one workgroup per chunk (4096 unless tuned) of its
 // elements, striding over the instance's own worklist segment, and E1b
 // selects each instance's dt in one workgroup per instance.  Each instance's selection is the first-index
 // extremum over its own (value, index) pairs, as a run of that instance
@@ -391,7 +391,7 @@ k_e1_limiter(FusedArgs A, FusedNodes N, EnsArgs E) {
   const uint32_t count = min(E.cnt[k], (uint32_t)(E.eoff[k + 1] - base));
   double best = DBL_MAX;
   int bi = 0x7fffffff;
-  for (uint32_t q = sub * kBlock + threadIdx.x; q < count; q += nbk * kBlock) {
+  for (uint32_t q = sub * blockDim.x + threadIdx.x; q < count; q += nbk * blockDim.x) {
     const int e = A.work[base + q];
     better_min(best, bi, f1_moving(A, N, e), e + 1);
   }
@@ -1197,9 +1197,9 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
-  const unsigned nbl = std::min(nbe, kLimiterBlocks);
+  const unsigned nbl = std::min(nbe, (unsigned)h->tune_lim_grid);
   BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
-  BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, kBlock, A, N, (int)nbe);
+  BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nbe);
   BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe, (int)nbl);
   // F2 and F5 (or R1) take a workgroup per 256 mixed cells for the
   // component work; F2 puts them first, rounded up to whole XCD rounds
@@ -1408,7 +1408,8 @@ int launch_ensemble_step(bl_handle_t* h, int p) {
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
   BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nbe, kBlock, A, N, E);
-  BL_LAUNCH(h, "e1_limiter", k_e1_limiter, (unsigned)h->ens_lblocks, kBlock, A, N, E);
+  BL_LAUNCH(h, "e1_limiter", k_e1_limiter, (unsigned)h->ens_lblocks, (unsigned)h->tune_lim_block,
+            A, N, E);
   BL_LAUNCH(h, "e1b_select", k_e1b_select, (unsigned)h->ens_n, kBlock, A, E, (int)nbe);
   BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N, 0);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
@@ -1530,6 +1531,177 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
     default:
       return BL_SUCCESS;
   }
+}
+
+// The launch-shape autotuner: times the limiter pass (F1L, or E1L for an
+// ensemble) of the current state under each candidate shape -- threads per
+// workgroup 64/128/256 and workgroups 128..2048 (single problem) or
+// elements per workgroup and instance 1024..16384 (ensemble) -- and keeps
+// the fastest.  Each trial runs the viscosity pass (F1/E1) that fills the
+// worklist, then the limiter pass between two events; neither touches the
+// state a step carries over (the viscosity fields they write are saved and
+// restored, the worklist counters cleared).  The choice changes no result:
+// the limiter's CFL selection is a first-index minimum whatever the split.
+int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
+  if (!h || reps < 1) {
+    set_error("bl_autotune: reps >= 1");
+    return BL_FAILURE;
+  }
+  const bool ens = h->ens_n > 0;
+  if (!ens && (!h->maps_built || !h->d_eos_itype || !h->d_zdtnotreg || !h->d_outer)) {
+    set_error("bl_autotune: mesh maps, regions or EOS not set");
+    return BL_FAILURE;
+  }
+  if (ensure_nodesets(h)) return BL_FAILURE;
+  // the device timestep as the last run left it (done set after an error or
+  // time_end), put back at the end; the trials run with done and skip clear,
+  // as the next run starts
+  DevTimestep ts_saved, ts_trial;
+  BL_HIP(hipMemcpyAsync(&ts_saved, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost,
+                        h->stream));
+  BL_HIP(hipStreamSynchronize(h->stream));
+  ts_trial = ts_saved;
+  ts_trial.done = 0;
+  ts_trial.skip = 0;
+  BL_HIP(hipMemcpyAsync(h->d_ts, &ts_trial, sizeof(DevTimestep), hipMemcpyHostToDevice,
+                        h->stream));
+  const int nel = h->sz.nel;
+  const unsigned nbe = nblocks(nel);
+  // save the viscosity outputs the trials overwrite
+  double* save = nullptr;
+  BL_HIP(hipMalloc(&save, sizeof(double) * 9 * (size_t)std::max(nel, 1)));
+  const size_t c4 = sizeof(double) * 4 * (size_t)nel, c1 = sizeof(double) * (size_t)nel;
+  hipMemcpyAsync(save, h->R(BL_CNVISCX), c4, hipMemcpyDeviceToDevice, h->stream);
+  hipMemcpyAsync(save + 4 * (size_t)nel, h->R(BL_CNVISCY), c4, hipMemcpyDeviceToDevice, h->stream);
+  hipMemcpyAsync(save + 8 * (size_t)nel, h->R(BL_ELVISC), c1, hipMemcpyDeviceToDevice, h->stream);
+  FusedArgs A = make_args(h);
+  A.stop_at_time_end = 0;
+  A.time_end = h->cfg.time_end;
+  const int p = h->cur_set;
+  FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
+  EnsArgs E{};
+  if (ens) {
+    A.einst = h->d_einst;
+    A.ninst = h->d_ninst;
+    A.inst_dt = h->d_inst_dt;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  BL_HIP(hipEventCreate(&e0));
+  BL_HIP(hipEventCreate(&e1));
+  const int blocks[3] = {64, 128, 256};
+  const int grids[5] = {128, 256, 512, 1024, 2048};
+  const int chunks[5] = {1024, 2048, 4096, 8192, 16384};
+  const int def_block = h->tune_lim_block, def_second = ens ? h->tune_ens_chunk : h->tune_lim_grid;
+  double best = DBL_MAX, def_us = 0.0;
+  int best_block = def_block, best_second = def_second, trials = 0;
+  int rc = BL_SUCCESS;
+  for (int b = 0; b < 3 && rc == BL_SUCCESS; ++b)
+    for (int g = 0; g < 5 && rc == BL_SUCCESS; ++g) {
+      const int block = blocks[b], second = ens ? chunks[g] : grids[g];
+      if (ens && ens_set_chunk(h, second)) { rc = BL_FAILURE; break; }
+      if (ens) E = make_ens(h);
+      std::vector<float> t;
+      for (int r = 0; r <= reps; ++r) {
+        hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream);
+        if (ens) {
+          hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream);
+          hipLaunchKernelGGL(k_e1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N, E);
+          hipEventRecord(e0, h->stream);
+          hipLaunchKernelGGL(k_e1_limiter, dim3(h->ens_lblocks), dim3(block), 0, h->stream, A, N,
+                             E);
+        } else {
+          hipLaunchKernelGGL(k_f1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N);
+          hipEventRecord(e0, h->stream);
+          hipLaunchKernelGGL(k_f1_limiter, dim3(std::min(nbe, (unsigned)second)), dim3(block), 0,
+                             h->stream, A, N, (int)nbe);
+        }
+        hipEventRecord(e1, h->stream);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          set_error("bl_autotune: trial launch failed");
+          rc = BL_FAILURE;
+          break;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) t.push_back(ms);  // the first run warms the caches
+      }
+      if (rc != BL_SUCCESS) break;
+      std::sort(t.begin(), t.end());
+      const double us = 1e3 * t[t.size() / 2];
+      ++trials;
+      if (block == def_block && second == def_second) def_us = us;
+      if (us < best) { best = us; best_block = block; best_second = second; }
+    }
+  // put back what the trials changed
+  hipMemcpyAsync(h->R(BL_CNVISCX), save, c4, hipMemcpyDeviceToDevice, h->stream);
+  hipMemcpyAsync(h->R(BL_CNVISCY), save + 4 * (size_t)nel, c4, hipMemcpyDeviceToDevice, h->stream);
+  hipMemcpyAsync(h->R(BL_ELVISC), save + 8 * (size_t)nel, c1, hipMemcpyDeviceToDevice, h->stream);
+  hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream);
+  if (ens) hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream);
+  hipMemcpyAsync(h->d_ts, &ts_saved, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream);
+  hipStreamSynchronize(h->stream);
+  hipFree(save);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != BL_SUCCESS) {
+    if (ens) ens_set_chunk(h, def_second);
+    return rc;
+  }
+  h->tune_lim_block = best_block;
+  if (ens) {
+    if (ens_set_chunk(h, best_second)) return BL_FAILURE;
+  } else {
+    h->tune_lim_grid = best_second;
+  }
+  invalidate_graph(h);
+  if (out) {
+    out->limiter_block = best_block;
+    out->limiter_grid = ens ? (int32_t)h->ens_lblocks : (int32_t)std::min(nbe, (unsigned)best_second);
+    out->ens_chunk = ens ? best_second : 0;
+    out->trials = trials;
+    out->best_us = best;
+    out->default_us = def_us;
+  }
+  return BL_SUCCESS;
+}
+
+// a launch shape chosen before (by bl_autotune on an earlier handle of the
+// same problem, say) applied without timing
+int bl_set_tuning(bl_handle_t* h, const bl_tuning_t* t) {
+  if (!h || !t || t->limiter_block < 64 || t->limiter_block > kBlock ||
+      t->limiter_block % 64 != 0) {
+    set_error("bl_set_tuning: limiter_block must be 64, 128, 192 or 256");
+    return BL_FAILURE;
+  }
+  if (h->ens_n > 0) {
+    if (ens_set_chunk(h, t->ens_chunk)) return BL_FAILURE;
+  } else {
+    if (t->limiter_grid < 1) {
+      set_error("bl_set_tuning: limiter_grid must be >= 1");
+      return BL_FAILURE;
+    }
+    h->tune_lim_grid = t->limiter_grid;
+  }
+  h->tune_lim_block = t->limiter_block;
+  invalidate_graph(h);
+  return BL_SUCCESS;
+}
+
+// the launch shape in use
+int bl_get_tuning(bl_handle_t* h, bl_tuning_t* t) {
+  if (!h || !t) {
+    set_error("bl_get_tuning: null argument");
+    return BL_FAILURE;
+  }
+  const bool ens = h->ens_n > 0;
+  t->limiter_block = h->tune_lim_block;
+  t->limiter_grid = ens ? (int32_t)h->ens_lblocks
+                        : (int32_t)std::min(nblocks(h->sz.nel), (unsigned)h->tune_lim_grid);
+  t->ens_chunk = ens ? h->tune_ens_chunk : 0;
+  t->trials = 0;
+  t->best_us = 0.0;
+  t->default_us = 0.0;
+  return BL_SUCCESS;
 }
 
 int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep_t* ts,

@@ -155,6 +155,11 @@ struct Handle {
   int32_t* d_lflag = nullptr;     // learned-motion flags and counters
   int32_t* d_lmark = nullptr;     // nnd: node marked by the check for the guard
   int32_t* d_nbfc = nullptr;      // (4, nel) ielel/ielfc packed for the fused remap
+  // launch shape of the limiter pass, set by bl_autotune: threads per
+  // workgroup, workgroups (single problem), elements per workgroup and
+  // instance (ensembles)
+  int32_t tune_lim_block = 256, tune_lim_grid = 512, tune_ens_chunk = 4096;
+  int32_t ens_lblocks_cap = 0;    // limiter partial slots allocated
   bool nbfc_stale = true;         // rebuild it before the next capture
   float* d_ld1 = nullptr;         // (nnd, 2): one-sweep displacement of the node
   // per-step diagnostics of the fused step: region table + Sod L1 records
@@ -205,6 +210,9 @@ int relax_rebuild(Handle* h);
 // the per-step diagnostics record from the node set a step wrote
 int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* ndu,
                 const double* ndv);
+// ensembles: the limiter pass's workgroups per instance for a chunk of
+// elements each (bl_ensemble.hip)
+int ens_set_chunk(Handle* h, int chunk);
 // drop the captured step graphs (config or maps changed)
 void invalidate_graph(Handle* h);
 

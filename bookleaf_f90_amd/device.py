@@ -73,6 +73,9 @@ def load_library(path=LIBPATH):
         lib.bl_relax_learned_stats.argtypes = [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32),
                                                P(C.c_int32)]
         lib.bl_relax_features.argtypes = [vp, fp]
+        lib.bl_autotune.argtypes = [vp, C.c_int32, P(abi.Tuning)]
+        lib.bl_set_tuning.argtypes = [vp, P(abi.Tuning)]
+        lib.bl_get_tuning.argtypes = [vp, P(abi.Tuning)]
         lib.bl_timing_enable.argtypes = [vp, C.c_int32]
         lib.bl_timing_reset.argtypes = [vp]
         lib.bl_timing_query.argtypes = [vp, C.c_char_p, P(C.c_double), P(C.c_int64)]
@@ -418,6 +421,25 @@ class BookLeaf(DriverState):
         out = np.zeros((self.sizes.nnd, abi.MLP_NIN), np.float32)
         self._check(self.lib.bl_relax_features(self.h, _fptr(out)), "relax_features")
         return out
+
+    def autotune(self, reps=5):
+        """Pick the limiter pass's launch shape by timing (bl_autotune);
+        returns the choice as a dict.  Results do not depend on it."""
+        t = abi.Tuning()
+        self._check(self.lib.bl_autotune(self.h, int(reps), C.byref(t)), "autotune")
+        return {k: getattr(t, k) for k, _ in abi.Tuning._fields_}
+
+    def set_tuning(self, limiter_block=256, limiter_grid=512, ens_chunk=4096):
+        """Apply a launch shape without timing (bl_set_tuning)."""
+        t = abi.Tuning(limiter_block=limiter_block, limiter_grid=limiter_grid,
+                       ens_chunk=ens_chunk)
+        self._check(self.lib.bl_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def get_tuning(self):
+        """The launch shape in use (bl_get_tuning)."""
+        t = abi.Tuning()
+        self._check(self.lib.bl_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return {k: getattr(t, k) for k, _ in abi.Tuning._fields_[:3]}
 
     def timing(self, on=True):
         self._check(self.lib.bl_timing_enable(self.h, int(on)), "timing_enable")

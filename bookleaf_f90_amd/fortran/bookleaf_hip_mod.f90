@@ -81,6 +81,12 @@ MODULE bookleaf_hip_mod
 &                          pressure_max,density_min,density_max
   END TYPE bl_flag_totals_t
 
+  ! the launch shape bl_autotune chose (no reference counterpart)
+  TYPE,BIND(C) :: bl_tuning_t
+    INTEGER(KIND=C_INT32_T) :: limiter_block,limiter_grid,ens_chunk,trials
+    REAL(KIND=C_DOUBLE)     :: best_us,default_us
+  END TYPE bl_tuning_t
+
   INTERFACE
 
     ! ---- lifetime and registry -------------------------------------------
@@ -500,6 +506,25 @@ MODULE bookleaf_hip_mod
       CHARACTER(KIND=C_CHAR),DIMENSION(*),INTENT(OUT)   :: buf
       INTEGER(KIND=C_INT32_T),VALUE                     :: buflen
     END FUNCTION bl_timing_names
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_autotune(h,reps,out)                    &
+&                                                    BIND(C,NAME="bl_autotune")
+      IMPORT :: C_INT32_T,C_PTR,bl_tuning_t
+      TYPE(C_PTR),VALUE                                 :: h
+      INTEGER(KIND=C_INT32_T),VALUE                     :: reps
+      TYPE(bl_tuning_t),                  INTENT(OUT)   :: out
+    END FUNCTION bl_autotune
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_set_tuning(h,t)                         &
+&                                                    BIND(C,NAME="bl_set_tuning")
+      IMPORT :: C_INT32_T,C_PTR,bl_tuning_t
+      TYPE(C_PTR),VALUE                                 :: h
+      TYPE(bl_tuning_t),                  INTENT(IN)    :: t
+    END FUNCTION bl_set_tuning
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_get_tuning(h,t)                         &
+&                                                    BIND(C,NAME="bl_get_tuning")
+      IMPORT :: C_INT32_T,C_PTR,bl_tuning_t
+      TYPE(C_PTR),VALUE                                 :: h
+      TYPE(bl_tuning_t),                  INTENT(OUT)   :: t
+    END FUNCTION bl_get_tuning
     TYPE(C_PTR) FUNCTION bl_last_error_c() BIND(C,NAME="bl_last_error")
       IMPORT :: C_PTR
     END FUNCTION bl_last_error_c

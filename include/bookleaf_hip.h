@@ -427,6 +427,35 @@ int bl_timing_query(bl_handle_t* h, const char* name, double* avg_ms,
 /* comma-separated list of the kernel classes seen so far */
 int bl_timing_names(bl_handle_t* h, char* buf, int32_t buflen);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose loops have
+ * no launch shape).  Times the limiter pass of the handle's current state
+ * -- the viscosity and CFL of the elements with moving nodes, the step's
+ * one worklist-driven kernel -- under 15 shapes: 64, 128 or 256 threads per
+ * workgroup times 128..2048 workgroups (a single problem) or 1024..16384
+ * elements per workgroup and instance (an ensemble), each the median of
+ * reps timed launches after one warm-up, and keeps the fastest for every
+ * later step.  The state is left as it was (the viscosity fields the trials
+ * write are restored) and no result depends on the choice: every reduction
+ * of the pass is a first-index extremum over (value, index) pairs.  Fails
+ * when the run has ended.  out (may be NULL) receives the choice, the
+ * number of shapes timed, and the chosen and the default shape's times. */
+typedef struct {
+  int32_t limiter_block; /* threads per workgroup */
+  int32_t limiter_grid;  /* workgroups launched */
+  int32_t ens_chunk;     /* ensemble: elements per workgroup and instance; 0 otherwise */
+  int32_t trials;        /* shapes timed */
+  double best_us;        /* median duration of the chosen shape */
+  double default_us;     /* median duration of the shape in use before the call */
+} bl_tuning_t;
+int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out);
+/* Apply a launch shape without timing (one bl_autotune chose for the same
+ * problem before, say): limiter_block (a multiple of 64 up to 256) and
+ * limiter_grid (>= 1, a single problem) or ens_chunk (>= 64, an ensemble);
+ * the other fields are ignored.  No result depends on it. */
+int bl_set_tuning(bl_handle_t* h, const bl_tuning_t* t);
+/* the shape in use: limiter_grid is the number of workgroups launched */
+int bl_get_tuning(bl_handle_t* h, bl_tuning_t* t);
+
 /* last error message of the library (thread-local), for diagnostics */
 const char* bl_last_error(void);
 

@@ -1,0 +1,132 @@
+"""The launch-shape autotuner (bl_autotune) and bl_set_tuning.
+
+Bar: the tuner leaves every field and the timestep as they were, and no
+result depends on the launch shape -- the limiter pass's CFL selection is a
+first-index minimum over (value, index) pairs whatever the number of
+workgroups or threads -- so runs with forced extreme shapes, the tuned shape
+and the default are bit-identical to each other and to the oracle."""
+import numpy as np
+import pytest
+
+from bookleaf_f90_amd import problem as P
+from tests.helpers import diff_fields
+from tests.test_gpu_graph import compared, LAG_SCRATCH
+
+pytestmark = pytest.mark.gpu
+
+# (threads per workgroup, workgroups) of the limiter pass: one workgroup
+# striding over everything, a few, more than there are elements' worth
+SHAPES = [(64, 1), (128, 7), (192, 2048), (256, 3)]
+
+
+def _sedov():
+    return P.sedov(n=40, nsteps=40, kappa=0.01)
+
+
+def _mixed():
+    return P.sod2mat_mixed(nx=32, ny=6, nsteps=30)
+
+
+@pytest.mark.parametrize("make", [_sedov, _mixed], ids=["sedov_hg", "sod2mat_mixed_eul"])
+def test_autotune_keeps_state_and_results(make, oracle_lib):
+    from bookleaf_f90_amd.device import BookLeaf
+    from oracle.pyoracle import Oracle
+    p = make()
+    print("autotune", p.name, "nel", p.nel, flush=True)
+    g = BookLeaf(p)
+    o = Oracle(p)
+    h1 = g.run(5, runner="run_graph")
+    before = g.state()
+    t = g.autotune(reps=2)
+    assert t["trials"] == 15
+    assert t["limiter_block"] in (64, 128, 256)
+    assert 1 <= t["limiter_grid"] <= -(-p.nel // 256)
+    assert t["best_us"] > 0.0 and t["default_us"] >= t["best_us"]
+    assert g.get_tuning()["limiter_block"] == t["limiter_block"]
+    # nothing moved
+    assert diff_fields(g.state(), before) == []
+    h2 = g.run(p.nsteps - 5, runner="run_graph")
+    ho = o.run(p.nsteps)
+    for a, b, c in zip(h1, h2, ho):
+        assert np.array_equal(np.concatenate([a, b]), c)
+    assert diff_fields(g.state(), o.state(), compared(p)) == []
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["%dx%d" % s for s in SHAPES])
+def test_forced_shapes_match_oracle(shape, oracle_lib):
+    from bookleaf_f90_amd.device import BookLeaf
+    from oracle.pyoracle import Oracle
+    p = _mixed() if shape[0] in (128, 192) else _sedov()
+    print("shape", shape, p.name, flush=True)
+    g = BookLeaf(p)
+    g.set_tuning(limiter_block=shape[0], limiter_grid=shape[1])
+    tun = g.get_tuning()
+    assert tun["limiter_block"] == shape[0]
+    assert tun["limiter_grid"] == min(shape[1], -(-p.nel // 256))
+    hg = g.run(p.nsteps, runner="run_graph")
+    ho = Oracle(p).run(p.nsteps)
+    for a, b in zip(hg, ho):
+        assert np.array_equal(a, b)
+    # the per-step runner launches the same shape
+    g2 = BookLeaf(p)
+    g2.set_tuning(limiter_block=shape[0], limiter_grid=shape[1])
+    hr = g2.run(p.nsteps, runner="run")
+    for a, b in zip(hr, ho):
+        assert np.array_equal(a, b)
+    assert diff_fields(g.state(), g2.state(), compared(p)) == []
+
+
+def test_ensemble_autotune_and_chunks():
+    """An ensemble after autotune and with forced chunks (64 elements per
+    workgroup and instance: several workgroups per instance) steps exactly as
+    with the default chunk."""
+    from bookleaf_f90_amd.device import Ensemble
+    from tests.test_gpu_ensemble import CFG, STATE_FIELDS
+    params = P.ensemble_params(6, seed=21, nmin=16, nmax=40)
+    print("ensemble autotune", [(q["kind"], q["nx"]) for q in params], flush=True)
+    runs = []
+    for mode in ("default", "tuned", "chunk64", "chunk16384"):
+        ens = Ensemble(params, config=CFG)
+        h1 = ens.run(4)
+        if mode == "tuned":
+            before = {f: ens.get(f) for f in STATE_FIELDS}
+            t = ens.autotune(reps=2)
+            assert t["trials"] == 15 and t["ens_chunk"] in (1024, 2048, 4096, 8192, 16384)
+            assert all(np.array_equal(ens.get(f), before[f]) for f in STATE_FIELDS)
+        elif mode == "chunk64":
+            ens.set_tuning(limiter_block=64, ens_chunk=64)
+            assert ens.get_tuning()["ens_chunk"] == 64
+        elif mode == "chunk16384":
+            ens.set_tuning(limiter_block=128, ens_chunk=16384)
+        h2 = ens.run(26)
+        hist = tuple(np.concatenate([a, b]) for a, b in zip(h1, h2))
+        runs.append((hist, {f: ens.get(f) for f in STATE_FIELDS}, ens.times()))
+        ens.close()
+    ref = runs[0]
+    for hist, state, times in runs[1:]:
+        for a, b in zip(hist, ref[0]):
+            assert np.array_equal(a, b)
+        for a, b in zip(times, ref[2]):
+            assert np.array_equal(a, b)
+        assert diff_fields(state, ref[1], [f for f in STATE_FIELDS if f not in LAG_SCRATCH]) == []
+
+
+def test_autotune_after_time_end_and_errors():
+    """The tuner times a handle whose last run ended at time_end and leaves
+    it ended; bad shapes are refused with the handle unchanged."""
+    from bookleaf_f90_amd.device import BookLeaf, BookLeafError
+    p = P.sod(nx=40, ny=4, time_end=2.0e-3, dt_initial=1.0e-4)
+    g = BookLeaf(p)
+    g.run(200, stop_at_time_end=True, runner="run_graph")
+    n0 = g.ts.nstep
+    g.autotune(reps=1)
+    assert len(g.run(5, stop_at_time_end=True, runner="run_graph")[0]) == 0
+    assert g.ts.nstep == n0
+    shape = g.get_tuning()
+    for bad in (dict(limiter_block=32), dict(limiter_block=320), dict(limiter_block=96),
+                dict(limiter_grid=0)):
+        with pytest.raises(BookLeafError):
+            g.set_tuning(**dict(dict(limiter_block=256, limiter_grid=512), **bad))
+    assert g.get_tuning() == shape
+    with pytest.raises(BookLeafError):
+        g.autotune(reps=0)
