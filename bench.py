@@ -44,15 +44,16 @@ HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 KAPPA = 0.01       # anti-hourglass coefficient (input/sedov_lag_small kappaall)
 # algorithmic bytes per launch of each fused kernel: (bytes/element, bytes/node)
 KERNEL_BYTES = {
-    # every element: ielnd 16, cs2 8; writes cnviscx/y 64, elvisc 8 | node
-    # x,y,u,v 32.  The limiter's stencil (ielel, outer nodes, neighbour rho
-    # and cs2) is read by f1_limiter for the elements on F1's worklist only,
-    # a data-dependent count, so f1_limiter has no entry here
-    "f1_getdt": (96, 32),
-    # ielnd 16, ielmat 4, ielreg 4 (hourglass kappa), p, m, e 24, cnviscx/y
-    # 64, cnmass 32; writes corner force x/y, weight, mass term 128 |
-    # node x,y,u,v 32
-    "f2_force": (272, 32),
+    # every element: ielnd 16, cs2 8, the at-rest mask 0.25 (read and
+    # written, one bit) | node x,y,u,v 32; plus ZEROED_BYTES per element
+    # whose zero viscosity it writes.  The limiter's stencil (ielel, outer
+    # nodes, neighbour rho and cs2) is read by f1_limiter for the elements on
+    # F1's worklist only, so f1_limiter has no entry here
+    "f1_getdt": (24.25, 32),
+    # ielnd 16, ielmat 4, ielreg 4 (hourglass kappa), p, m, e 24, cnmass 32,
+    # the at-rest mask 0.125; writes corner force x/y, weight, mass term 128
+    # | node x,y,u,v 32; plus MOVING_BYTES (cnviscx/y) per element in motion
+    "f2_force": (208.125, 32),
     # CSR corner ids 16, corner force x/y, weight, mass term 128 | ptr 4,
     # indtype 4, x,y,u,v 32, writes x,y,u,v 32, ubar,vbar 16
     "f3_accel": (144, 88),
@@ -60,7 +61,7 @@ KERNEL_BYTES = {
     # cs2 32 | x,y 16, ubar,vbar 16
     "f4_state": (132, 32),
     # ensemble F1: F1's bytes and the element instance id 2
-    "e1_getdt": (98, 32),
+    "e1_getdt": (26.25, 32),
     # remap (ALE step), per element | per node (the neighbours and their
     # faces come from one packed int32 per face, 16 B an element):
     # ielnd, packed ielel/ielfc 32, cnwt 32, rho 8; writes dv 8, face mass
@@ -89,6 +90,22 @@ KERNEL_BYTES = {
     # the check: ielnd 16 | x, y, mesh velocity 32
     "learned_check": (16, 32),
 }
+# data-dependent parts, from the run's own counters (bl_run_stats): F1 and
+# E1 write cnviscx/y and elvisc (72 B) for an element at rest only when it
+# moved in the step before (or on a call's first step); F2 reads cnviscx/y
+# (64 B) only for an element in motion
+ZEROED_BYTES = {"f1_getdt": 72, "e1_getdt": 72}
+MOVING_BYTES = {"f2_force": 64}
+
+
+def kernel_bytes(k, nel, nnd, moving, zeroed):
+    """algorithmic bytes of one launch of kernel k; moving and zeroed are the
+    per-step averages of bl_run_stats"""
+    be, bn = KERNEL_BYTES[k]
+    return (be * nel + bn * nnd + ZEROED_BYTES.get(k, 0) * zeroed +
+            MOVING_BYTES.get(k, 0) * moving)
+
+
 # per Lagrangian step at the driver boundary: 1664 B/element + 272 B/node,
 # +84 B/element with the anti-hourglass force on
 DRIVER_BOUNDARY = (1664 + 84, 272)
@@ -269,12 +286,13 @@ def main():
         if n:
             kern[name] = {"avg_ms": ms, "launches": n, "total_ms": ms * n}
     g.timing(False)
+    mv, zr = g.run_stats()
+    moving, zeroed = mv / max(args.profile_steps, 1), zr / max(args.profile_steps, 1)
     dom = max((k for k in kern if k in KERNEL_BYTES), key=lambda k: kern[k]["total_ms"],
               default=None)
     roof = None
     if dom:
-        be, bn = KERNEL_BYTES[dom]
-        alg = be * nel + bn * nnd
+        alg = kernel_bytes(dom, nel, nnd, moving, zeroed)
         avg_s = kern[dom]["avg_ms"] * 1e-3
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
@@ -285,8 +303,8 @@ def main():
                 "traffic": traffic, "algorithmic_bytes": alg, "avg_ms": kern[dom]["avg_ms"]}
         for k in kern:
             if k in KERNEL_BYTES:
-                be, bn = KERNEL_BYTES[k]
-                kern[k]["alg_GBps"] = (be * nel + bn * nnd) / (kern[k]["avg_ms"] * 1e-3) / 1e9
+                kern[k]["alg_GBps"] = (kernel_bytes(k, nel, nnd, moving, zeroed) /
+                                       (kern[k]["avg_ms"] * 1e-3) / 1e9)
     for _ in range(args.calibrate_copy):
         g.hydro_setpredictor()  # k_copy_r: reads and writes 8*nel bytes
     g.synchronize()
@@ -300,7 +318,7 @@ def main():
                 else DRIVER_BOUNDARY_EUL)
     step_bytes = dbe * nel + dbn * nnd
     # algorithmic bytes of the kernels one step launches (from the profile pass)
-    fused_bytes = sum((KERNEL_BYTES[k][0] * nel + KERNEL_BYTES[k][1] * nnd) *
+    fused_bytes = sum(kernel_bytes(k, nel, nnd, moving, zeroed) *
                       kern[k]["launches"] / max(args.profile_steps, 1)
                       for k in kern if k in KERNEL_BYTES)
     out = {
@@ -327,6 +345,8 @@ def main():
         "driver_boundary_GBps": step_bytes / (sec / args.steps) / 1e9,
         "driver_boundary_hbm_frac": step_bytes / (sec / args.steps) / HBM_PEAK,
         "kernels": kern,
+        "profile_pass": {"steps": args.profile_steps, "moving_per_step": moving,
+                         "zeroed_per_step": zeroed},
     }
     if args.workload == "learned":
         s_, gd, fb, rs = g.learned_stats(full=True)

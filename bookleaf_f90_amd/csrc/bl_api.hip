@@ -614,7 +614,7 @@ int bl_destroy(bl_handle_t* h) {
                   h->d_eos_itype, h->d_eos_param, h->d_hyd_ptr, h->d_hyd_idx,
                   h->d_ale_ptr, h->d_ale_idx, h->d_el_ptr, h->d_el_idx, h->d_part,
                   h->d_parti, h->d_ticket, h->d_dtres, h->d_ts, h->d_hist, h->d_outer,
-                  h->d_diag_hist, h->d_diag_count, h->d_diag_part, h->d_ring, h->d_work,
+                  h->d_diag_hist, h->d_diag_count, h->d_diag_part, h->d_ring, h->d_work, h->d_rest, h->d_vstats,
                   h->d_rlist, h->d_zlist, h->d_lflag, h->d_lmark, h->d_ld1, h->d_nbfc, h->d_cw[0], h->d_cw[1],
                   h->d_eoff, h->d_lboff, h->d_einst, h->d_ninst, h->d_inst, h->d_inst_dt, h->d_cnt,
                   h->d_epart, h->d_eparti, h->d_ehist};

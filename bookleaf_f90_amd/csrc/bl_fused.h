@@ -82,6 +82,10 @@ struct FusedArgs {
   int32_t* parti;
   int32_t* work;          // F1's worklist of elements whose nodes move unequally
   uint32_t* work_count;   // its length (reset by F1b)
+  // bit e of rest[e / 64]: element e is at rest this step and its cnviscx/y
+  // and elvisc hold the +0.0 F1 wrote (F1 sets it; F2 reads it)
+  uint64_t* rest;
+  unsigned long long* vstats;  // [0] worklist elements, [1] elements F1 zeroed
   DtResult* dtres;
   DevTimestep* ts;
   double* hist;

@@ -161,6 +161,18 @@ __device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double
   return dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
 }
 
+// the wave's word of the at-rest mask, and the count of elements whose
+// zeros F1 wrote (one atomic per wave that wrote any)
+__device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane, uint64_t prev,
+                                            bool at_rest, bool zeroed) {
+  const uint64_t now = __ballot(at_rest);
+  const uint64_t wrote = __ballot(zeroed);
+  if (lane == 0) {
+    if (now != prev) A.rest[e >> 6] = now;
+    if (wrote) atomicAdd(A.vstats + 1, (unsigned long long)__popcll(wrote));
+  }
+}
+
 // F1: every element.  Four nodes with one velocity make every own-edge
 // velocity difference zero, so geteviscosity zeroes every edge (|du| <=
 // zerocut) whatever the limiter gives, and each edge's elvisc term is
@@ -173,9 +185,11 @@ __device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double
 __global__ void __launch_bounds__(kBlock)
 k_f1_getdt(FusedArgs A, FusedNodes N) {
   if (step_skipped(A, A.ts)) return;
-  const int e = xcd_index();
+  const int e = xcd_index();  // a wave holds 64 consecutive elements from a multiple of 64
+  const int lane = threadIdx.x & 63;
+  const uint64_t prev = A.rest[e >> 6];
   double cflv = DBL_MAX, divv = -1.0;
-  bool moving = false;
+  bool moving = false, zeroed = false;
   if (e < A.nel) {
     int nd[4];
     load_nd(A.ielnd, e, nd);
@@ -184,10 +198,14 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
     moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] &&
                v[0] == v[2] && v[0] == v[3]);
     if (!moving) {
-      const double z[4] = {0.0, 0.0, 0.0, 0.0};
-      st4(A.cnviscx, e, z);
-      st4(A.cnviscy, e, z);
-      A.elvisc[e] = 0.0;
+      // at rest in the previous step of this run too: its zeros are in place
+      if (!((prev >> lane) & 1ull)) {
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+        st4(A.cnviscx, e, z);
+        st4(A.cnviscy, e, z);
+        A.elvisc[e] = 0.0;
+        zeroed = true;
+      }
       cflv = f1_cfl(A, e, x, y, A.cs2[e], 0.0);
     }
     // divergence (hydro_kn_getdtdiv) from the geometry at x^n
@@ -195,10 +213,10 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
     getiso(x, y, g);
     divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
   }
+  rest_update(A, e, lane, prev, e < A.nel && !moving, zeroed);
   // worklist append, one atomic per wave
   const uint64_t mask = __ballot(moving);
   if (mask) {
-    const int lane = threadIdx.x & 63;
     const int lead = __ffsll((unsigned long long)mask) - 1;
     uint32_t base = 0;
     if (lane == lead) base = atomicAdd(A.work_count, (uint32_t)__popcll(mask));
@@ -275,6 +293,7 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
     ts->skip = skip ? 1 : 0;
     A.dtres->negvol = 0;
     A.dtres->negvol2 = 0;
+    if (!skip) A.vstats[0] += min(*A.work_count, (uint32_t)A.nel);
     *A.work_count = 0;  // F1L has read it
   }
   if (skip) return;
@@ -326,8 +345,10 @@ k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
   const int ia = A.einst[lb * kBlock];
   const int ib = A.einst[min(lb * kBlock + kBlock - 1, A.nel - 1)];
   const int my = e < A.nel ? (int)A.einst[e] : ib;
+  const int lane = threadIdx.x & 63;
+  const uint64_t prev = A.rest[e >> 6];
   double cflv = DBL_MAX, divv = -1.0;
-  bool moving = false;
+  bool moving = false, zeroed = false;
   if (e < A.nel) {
     int nd[4];
     load_nd(A.ielnd, e, nd);
@@ -336,10 +357,13 @@ k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
     moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] &&
                v[0] == v[2] && v[0] == v[3]);
     if (!moving) {
-      const double z[4] = {0.0, 0.0, 0.0, 0.0};
-      st4(A.cnviscx, e, z);
-      st4(A.cnviscy, e, z);
-      A.elvisc[e] = 0.0;
+      if (!((prev >> lane) & 1ull)) {
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+        st4(A.cnviscx, e, z);
+        st4(A.cnviscy, e, z);
+        A.elvisc[e] = 0.0;
+        zeroed = true;
+      }
       cflv = f1_cfl(A, e, x, y, A.cs2[e], 0.0);
       if (!(cflv == cflv)) cflv = DBL_MAX;
     }
@@ -348,9 +372,9 @@ k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
     divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
     if (!(divv == divv)) divv = -1.0;
   }
+  rest_update(A, e, lane, prev, e < A.nel && !moving, zeroed);
   // worklist append into each instance's segment, one atomic per wave and
   // instance
-  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int k = s == 0 ? ia : ib;
@@ -413,6 +437,8 @@ __global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, i
       A.dtres->negvol2 = 0;
       if (!skip) ts->nstep = ts->nstep + 1;
     }
+    if (!skip)
+      atomicAdd(A.vstats, (unsigned long long)min(E.cnt[k], (uint32_t)(E.eoff[k + 1] - E.eoff[k])));
     E.cnt[k] = 0;  // E1L has read it
   }
   if (skip) return;
@@ -603,9 +629,16 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   load_nd(A.ielnd, e, nd);
   double x[4], y[4], u[4], v[4];
   load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+  // an element F1 found at rest has +0.0 viscosity forces: not read
+  const bool at_rest = (A.rest[e >> 6] >> (e & 63)) & 1ull;
   double qx[4], qy[4];
-  ld4(A.cnviscx, e, qx);
-  ld4(A.cnviscy, e, qy);
+  if (at_rest) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { qx[c] = 0.0; qy[c] = 0.0; }
+  } else {
+    ld4(A.cnviscx, e, qx);
+    ld4(A.cnviscy, e, qy);
+  }
   int ireg = 0;
   if (A.reg_tables_force) ireg = A.ielreg[e] - 1;
   // --- predictor: hydro_dr_getforce at x^n with p^n
@@ -619,7 +652,8 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
     double spm[4];
     ld4(A.spmass, e, spm);
     // elcs2 at this point holds cs2 + elvisc (hydro_dr_getcs2 in getdt)
-    forceSP(A.pmeritreg[ireg], A.rho[e], A.cs2[e] + A.elvisc[e], x, y, spm, fx, fy);
+    forceSP(A.pmeritreg[ireg], A.rho[e], A.cs2[e] + (at_rest ? 0.0 : A.elvisc[e]), x, y, spm,
+            fx, fy);
   }
   // --- predictor getstate: vertex, geometry, density; energy; EOS
   double xh[4], yh[4];
@@ -1149,6 +1183,7 @@ FusedArgs make_args(bl_handle_t* h) {
   A.ubar = h->R(BL_RSCRATCH14); A.vbar = h->R(BL_RSCRATCH15);
   A.part = h->d_part; A.parti = h->d_parti; A.dtres = h->d_dtres;
   A.work = h->d_work; A.work_count = h->d_ticket;
+  A.rest = h->d_rest; A.vstats = h->d_vstats;
   A.ts = h->d_ts;
   A.hist = h->d_hist;
   A.hist_cap = (int)h->hist_cap;
@@ -1245,6 +1280,14 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
 
 int ensure_nodesets(bl_handle_t* h) {
   if (!h->d_work) BL_HIP(hipMalloc(&h->d_work, sizeof(int32_t) * std::max(h->sz.nel, 1)));
+  if (!h->d_rest) {
+    // one word per wave of F1's grid, the partial last workgroup included
+    const size_t words = 4 * (size_t)nblocks(std::max(h->sz.nel, 1));
+    BL_HIP(hipMalloc(&h->d_rest, sizeof(uint64_t) * words));
+    BL_HIP(hipMalloc(&h->d_vstats, sizeof(unsigned long long) * 2));
+    BL_HIP(hipMemset(h->d_rest, 0, sizeof(uint64_t) * words));
+    BL_HIP(hipMemset(h->d_vstats, 0, sizeof(unsigned long long) * 2));
+  }
   if (!h->cterm[0])
     for (int k = 0; k < 4; ++k)
       BL_HIP(hipMalloc(&h->cterm[k], sizeof(double) * 4 * std::max(h->sz.nel, 1)));
@@ -1259,6 +1302,17 @@ int ensure_nodesets(bl_handle_t* h) {
     BL_HIP(hipMalloc(&h->nodeset[1][k], sizeof(double) * std::max(h->sz.nnd2, 1)));
   }
   h->cur_set = 0;
+  return BL_SUCCESS;
+}
+
+// Forget which elements' viscosity fields hold F1's zeros: at the start of
+// every run call, since between calls the host may have written them (an
+// upload, a per-driver call, a write through bl_field_device_ptr).  The
+// first step of the call writes every element's.  Outside any capture.
+int clear_rest(bl_handle_t* h) {
+  BL_HIP(hipMemsetAsync(h->d_rest, 0, sizeof(uint64_t) * 4 * (size_t)nblocks(std::max(h->sz.nel, 1)),
+                        h->stream));
+  BL_HIP(hipMemsetAsync(h->d_vstats, 0, sizeof(unsigned long long) * 2, h->stream));
   return BL_SUCCESS;
 }
 
@@ -1458,7 +1512,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   }
   fill_err(err, BL_SUCCESS, 0, "");
   if (nsteps <= 0) { if (steps_taken) *steps_taken = 0; return BL_SUCCESS; }
-  if (ensure_nodesets(h)) return BL_FAILURE;
+  if (ensure_nodesets(h) || clear_rest(h)) return BL_FAILURE;
   if (h->ehist_cap < nsteps) {
     // as bl_run_graph: room for runs of up to 1024 steps without re-capture
     const int cap = std::max(nsteps, 1024);
@@ -1639,6 +1693,7 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
   hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream);
   if (ens) hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream);
   hipMemcpyAsync(h->d_ts, &ts_saved, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream);
+  clear_rest(h);
   hipStreamSynchronize(h->stream);
   hipFree(save);
   hipEventDestroy(e0);
@@ -1662,6 +1717,19 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
     out->best_us = best;
     out->default_us = def_us;
   }
+  return BL_SUCCESS;
+}
+
+// counters of the last run call (bl_run_graph or bl_ensemble_run)
+int bl_run_stats(bl_handle_t* h, int64_t* moving, int64_t* zeroed) {
+  if (!h) return BL_FAILURE;
+  unsigned long long c[2] = {0, 0};
+  if (h->d_vstats) {
+    BL_HIP(hipMemcpyAsync(c, h->d_vstats, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    BL_HIP(hipStreamSynchronize(h->stream));
+  }
+  if (moving) *moving = (int64_t)c[0];
+  if (zeroed) *zeroed = (int64_t)c[1];
   return BL_SUCCESS;
 }
 
@@ -1724,7 +1792,7 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
                   err);
   fill_err(err, BL_SUCCESS, 0, "");
-  if (ensure_nodesets(h)) return BL_FAILURE;
+  if (ensure_nodesets(h) || clear_rest(h)) return BL_FAILURE;
   if (h->hist_cap < nsteps) {
     // the graphs bake in the history's capacity: size it generously so that
     // runs of different lengths do not re-capture

@@ -117,6 +117,8 @@ struct Handle {
   double* cterm[4] = {};
   // F1's worklist (nel): elements whose nodes do not share one velocity
   int32_t* d_work = nullptr;
+  uint64_t* d_rest = nullptr;             // F1's at-rest mask, 4 words per 256 elements
+  unsigned long long* d_vstats = nullptr;  // counters of the last run call
   // remap temporaries of the fused step (zexist): 4*nel each for the face
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[13] = {};

@@ -74,6 +74,7 @@ def load_library(path=LIBPATH):
                                                P(C.c_int32)]
         lib.bl_relax_features.argtypes = [vp, fp]
         lib.bl_autotune.argtypes = [vp, C.c_int32, P(abi.Tuning)]
+        lib.bl_run_stats.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
         lib.bl_set_tuning.argtypes = [vp, P(abi.Tuning)]
         lib.bl_get_tuning.argtypes = [vp, P(abi.Tuning)]
         lib.bl_timing_enable.argtypes = [vp, C.c_int32]
@@ -421,6 +422,14 @@ class BookLeaf(DriverState):
         out = np.zeros((self.sizes.nnd, abi.MLP_NIN), np.float32)
         self._check(self.lib.bl_relax_features(self.h, _fptr(out)), "relax_features")
         return out
+
+    def run_stats(self):
+        """(moving, zeroed) of the last graph or ensemble run call: elements
+        in motion and elements whose zero viscosity the step wrote, summed
+        over its steps (bl_run_stats)."""
+        m, z = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.bl_run_stats(self.h, C.byref(m), C.byref(z)), "run_stats")
+        return m.value, z.value
 
     def autotune(self, reps=5):
         """Pick the limiter pass's launch shape by timing (bl_autotune);

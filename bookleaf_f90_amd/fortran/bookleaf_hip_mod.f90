@@ -506,6 +506,12 @@ MODULE bookleaf_hip_mod
       CHARACTER(KIND=C_CHAR),DIMENSION(*),INTENT(OUT)   :: buf
       INTEGER(KIND=C_INT32_T),VALUE                     :: buflen
     END FUNCTION bl_timing_names
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_run_stats(h,moving,zeroed)              &
+&                                                    BIND(C,NAME="bl_run_stats")
+      IMPORT :: C_INT32_T,C_INT64_T,C_PTR
+      TYPE(C_PTR),VALUE                                 :: h
+      INTEGER(KIND=C_INT64_T),            INTENT(OUT)   :: moving,zeroed
+    END FUNCTION bl_run_stats
     INTEGER(KIND=C_INT32_T) FUNCTION bl_autotune(h,reps,out)                    &
 &                                                    BIND(C,NAME="bl_autotune")
       IMPORT :: C_INT32_T,C_PTR,bl_tuning_t

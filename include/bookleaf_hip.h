@@ -427,6 +427,15 @@ int bl_timing_query(bl_handle_t* h, const char* name, double* avg_ms,
 /* comma-separated list of the kernel classes seen so far */
 int bl_timing_names(bl_handle_t* h, char* buf, int32_t buflen);
 
+/* Counters of the last bl_run_graph or bl_ensemble_run call (no
+ * counterpart in the reference): moving = the elements whose nodes did not
+ * all share one velocity, summed over the steps (the limiter pass's work);
+ * zeroed = the elements at rest whose zero viscosity fields the step wrote,
+ * summed over the steps.  An element at rest in two consecutive steps of a
+ * call keeps the zeros of the first: neither written again nor read by the
+ * force kernel.  Either output may be NULL. */
+int bl_run_stats(bl_handle_t* h, int64_t* moving, int64_t* zeroed);
+
 /* Launch-shape autotuner (no counterpart in the reference, whose loops have
  * no launch shape).  Times the limiter pass of the handle's current state
  * -- the viscosity and CFL of the elements with moving nodes, the step's

@@ -130,3 +130,39 @@ def test_graph_without_stencil_runs_host_driven(oracle_lib):
     ho = o.run(12)
     assert all(np.array_equal(a, b) for a, b in zip(hg, ho))
     assert diff_fields(g.state(), o.state()) == []
+
+
+def _at_rest(g, nel):
+    """elements whose four nodes share one velocity, from the state the next
+    step starts from"""
+    nd = g.get("ielnd")[:4 * nel].reshape(nel, 4) - 1
+    u, v = g.get("ndu")[nd], g.get("ndv")[nd]
+    return np.all(u == u[:, :1], axis=1) & np.all(v == v[:, :1], axis=1)
+
+
+def test_run_stats_and_the_at_rest_mask(oracle_lib):
+    """bl_run_stats against the elements at rest counted on the host; a
+    12-step call (whose steps reuse the zeros of elements at rest in the step
+    before) against 12 one-step calls (each writing every element's) and the
+    oracle, bit for bit."""
+    from bookleaf_f90_amd.device import BookLeaf
+    from oracle.pyoracle import Oracle
+    p = P.sedov(n=24, nsteps=12, kappa=0.01)
+    nel = p.nel
+    a = BookLeaf(p)
+    rest = []
+    for _ in range(12):
+        r = _at_rest(a, nel)
+        a.run(1, runner="run_graph")
+        assert a.run_stats() == (int(nel - r.sum()), int(r.sum()))
+        rest.append(r)
+    b = BookLeaf(p)
+    b.run(12, runner="run_graph")
+    moving = sum(int(nel - r.sum()) for r in rest)
+    zeroed = int(rest[0].sum()) + sum(int((r & ~q).sum()) for q, r in zip(rest, rest[1:]))
+    assert b.run_stats() == (moving, zeroed)
+    assert zeroed < 12 * int(rest[-1].sum())  # the mask saved writes
+    o = Oracle(p)
+    o.run(12)
+    for g in (a, b):
+        assert diff_fields(g.state(), o.state(), compared(p)) == []
