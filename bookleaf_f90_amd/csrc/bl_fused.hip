@@ -264,19 +264,30 @@ __device__ __forceinline__ void select_dt(const FusedArgs& A, double dt_prev, in
 }
 
 // F1L: the worklist's elements, a fixed grid striding over the count F1
-// left; the CFL partials of its workgroups follow F1's (part + 3 * nb1)
+// left.  Each workgroup also folds a slice of F1's nb1 partials of each
+// control into its own, so that F1b reduces one (CFL, DIV) pair per F1L
+// workgroup: CFL at part + 3 * nb1, DIV at part + nb1 (first-index extrema
+// are associative, so the result is F1b's over all partials)
 __global__ void __launch_bounds__(kBlock)
 k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
   if (step_skipped(A, A.ts)) return;
   const uint32_t count = min(*A.work_count, (uint32_t)A.nel);
-  double best = DBL_MAX;
-  int bi = 0x7fffffff;
+  double best = DBL_MAX, dv = -1.0;
+  int bi = 0x7fffffff, di = 0x7fffffff;
+  const int per = (nb1 + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int lo = (int)blockIdx.x * per, hi = min(nb1, lo + per);
+  for (int k = lo + (int)threadIdx.x; k < hi; k += (int)blockDim.x) {
+    better_max(dv, di, A.part[k], A.parti[k]);
+    better_min(best, bi, A.part[2 * nb1 + k], A.parti[2 * nb1 + k]);
+  }
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
        k += gridDim.x * blockDim.x) {
     const int e = A.work[k];
     better_min(best, bi, f1_moving(A, N, e), e + 1);
   }
   block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
+  __syncthreads();
+  block_reduce_store<false>(dv, di, A.part + nb1, A.parti + nb1);
 }
 
 // F1b: one workgroup -- combine the partials of F1 and F1L, then the
@@ -297,7 +308,8 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
     *A.work_count = 0;  // F1L has read it
   }
   if (skip) return;
-  final_dt_reduce(nb, nb + nbl, A.part, A.parti, A.dtres);
+  final_dt_reduce(nbl, A.part + nb, A.parti + nb, nbl, A.part + 3 * nb, A.parti + 3 * nb,
+                  A.dtres);
   if (threadIdx.x != 0) return;
   const DtResult& r = *A.dtres;
   if (r.cfl_min < 0.0) {  // hydro_kn_getdt.f90:75-78

@@ -73,14 +73,14 @@ __device__ __forceinline__ void block_argmax_first(double v, int i, double* part
 // part[0,na), CFL partials in part[2na, 2na+nc); both in one pass so the
 // loads of a thread are independent
 constexpr int kFinalBlock = 1024;
-__device__ __forceinline__ void final_dt_reduce(int na, int nc, const double* part,
-                                                const int32_t* parti, DtResult* res) {
+// DIV partials (pa, qa)[0, na), CFL partials (pc, qc)[0, nc)
+__device__ __forceinline__ void final_dt_reduce(int na, const double* part, const int32_t* parti,
+                                                int nc, const double* pc, const int32_t* qc,
+                                                DtResult* res) {
   __shared__ double sa[kFinalBlock / 64], sc[kFinalBlock / 64];
   __shared__ int ia[kFinalBlock / 64], ic[kFinalBlock / 64];
   double va = -1.0, vc = DBL_MAX;
   int xa = 0x7fffffff, xc = 0x7fffffff;
-  const double* pc = part + 2 * na;
-  const int32_t* qc = parti + 2 * na;
   const int nmax = na > nc ? na : nc;
 #pragma unroll 4
   for (int k = threadIdx.x; k < nmax; k += blockDim.x) {
@@ -103,6 +103,10 @@ __device__ __forceinline__ void final_dt_reduce(int na, int nc, const double* pa
     res->cfl_idx = ic[0];
   }
   __syncthreads();
+}
+__device__ __forceinline__ void final_dt_reduce(int na, int nc, const double* part,
+                                                const int32_t* parti, DtResult* res) {
+  final_dt_reduce(na, part, parti, nc, part + 2 * na, parti + 2 * na, res);
 }
 
 }  // namespace bl
