@@ -261,17 +261,28 @@ def main():
     # the launch shape is tuned on the state the timed steps start from; the
     # last warm-up step then captures the step graphs with it
     tune = None
+    counted = [0, 0, 0]  # steps, moving, zeroed over every run call
+
+    def count(n):
+        mv, zr = g.run_stats()
+        counted[0] += n
+        counted[1] += mv
+        counted[2] += zr
+
     if args.warmup:
         run(args.warmup - 1)
+        count(args.warmup - 1)
         if not args.no_autotune:
             tune = g.autotune(reps=5)
             print("# autotune %s" % tune, file=sys.stderr, flush=True)
         run(1)
+        count(1)
     g.synchronize()
     t0 = time.perf_counter()
     run(args.steps)
     g.synchronize()
     sec = time.perf_counter() - t0
+    count(args.steps)
     value = nel * args.steps / sec
     print("# timed %d steps in %.4f s" % (args.steps, sec), file=sys.stderr, flush=True)
 
@@ -287,6 +298,7 @@ def main():
             kern[name] = {"avg_ms": ms, "launches": n, "total_ms": ms * n}
     g.timing(False)
     mv, zr = g.run_stats()
+    count(args.profile_steps)
     moving, zeroed = mv / max(args.profile_steps, 1), zr / max(args.profile_steps, 1)
     dom = max((k for k in kern if k in KERNEL_BYTES), key=lambda k: kern[k]["total_ms"],
               default=None)
@@ -345,8 +357,12 @@ def main():
         "driver_boundary_GBps": step_bytes / (sec / args.steps) / 1e9,
         "driver_boundary_hbm_frac": step_bytes / (sec / args.steps) / HBM_PEAK,
         "kernels": kern,
+        # elements in motion and zero viscosity writes per step (bl_run_stats),
+        # over the profile pass and over every run call of this command
         "profile_pass": {"steps": args.profile_steps, "moving_per_step": moving,
                          "zeroed_per_step": zeroed},
+        "all_calls": {"steps": counted[0], "moving_per_step": counted[1] / max(counted[0], 1),
+                      "zeroed_per_step": counted[2] / max(counted[0], 1)},
     }
     if args.workload == "learned":
         s_, gd, fb, rs = g.learned_stats(full=True)
