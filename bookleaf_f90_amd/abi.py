@@ -111,7 +111,8 @@ class Tuning(C.Structure):
     """The launch shape bl_autotune chose."""
     _fields_ = [("limiter_block", C.c_int32), ("limiter_grid", C.c_int32),
                 ("ens_chunk", C.c_int32), ("trials", C.c_int32),
-                ("best_us", C.c_double), ("default_us", C.c_double)]
+                ("best_us", C.c_double), ("default_us", C.c_double),
+                ("best_full_us", C.c_double), ("default_full_us", C.c_double)]
 
 
 def field_shape(sizes, fid):

@@ -161,6 +161,12 @@ __device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double
   return dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
 }
 
+// every element on the worklist (the autotuner's second trial)
+__global__ void __launch_bounds__(kBlock) k_work_all(int32_t* work, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) work[i] = i;
+}
+
 // the wave's word of the at-rest mask, and the count of elements whose
 // zeros F1 wrote (one atomic per wave that wrote any)
 __device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane, uint64_t prev,
@@ -1651,6 +1657,10 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
     A.ninst = h->d_ninst;
     A.inst_dt = h->d_inst_dt;
   }
+  // worklist lengths of the "every element" trials
+  std::vector<uint32_t> full(ens ? h->ens_n : 1, (uint32_t)nel);
+  if (ens)
+    for (int k = 0; k < h->ens_n; ++k) full[k] = h->ens_eoff[k + 1] - h->ens_eoff[k];
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BL_HIP(hipEventCreate(&e0));
   BL_HIP(hipEventCreate(&e1));
@@ -1658,45 +1668,56 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
   const int grids[5] = {128, 256, 512, 1024, 2048};
   const int chunks[5] = {1024, 2048, 4096, 8192, 16384};
   const int def_block = h->tune_lim_block, def_second = ens ? h->tune_ens_chunk : h->tune_lim_grid;
-  double best = DBL_MAX, def_us = 0.0;
-  int best_block = def_block, best_second = def_second, trials = 0;
+  double tcur[15], tfull[15];
+  int cblock[15], csecond[15], trials = 0;
   int rc = BL_SUCCESS;
   for (int b = 0; b < 3 && rc == BL_SUCCESS; ++b)
     for (int g = 0; g < 5 && rc == BL_SUCCESS; ++g) {
       const int block = blocks[b], second = ens ? chunks[g] : grids[g];
       if (ens && ens_set_chunk(h, second)) { rc = BL_FAILURE; break; }
       if (ens) E = make_ens(h);
-      std::vector<float> t;
-      for (int r = 0; r <= reps; ++r) {
-        hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream);
-        if (ens) {
-          hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream);
-          hipLaunchKernelGGL(k_e1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N, E);
+      // pass 0: the worklist F1 leaves for the current state; pass 1: every
+      // element listed (the worklist as the flow spreads)
+      for (int pass = 0; pass < 2 && rc == BL_SUCCESS; ++pass) {
+        std::vector<float> t;
+        if (pass == 1)
+          hipLaunchKernelGGL(k_work_all, dim3(nbe), dim3(kBlock), 0, h->stream, h->d_work, nel);
+        for (int r = 0; r <= reps; ++r) {
+          uint32_t* cnt = ens ? h->d_cnt : h->d_ticket;
+          if (pass == 0) {
+            hipMemsetAsync(cnt, 0, sizeof(uint32_t) * full.size(), h->stream);
+            if (ens)
+              hipLaunchKernelGGL(k_e1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N, E);
+            else
+              hipLaunchKernelGGL(k_f1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N);
+          } else {
+            hipMemcpyAsync(cnt, full.data(), sizeof(uint32_t) * full.size(),
+                           hipMemcpyHostToDevice, h->stream);
+          }
           hipEventRecord(e0, h->stream);
-          hipLaunchKernelGGL(k_e1_limiter, dim3(h->ens_lblocks), dim3(block), 0, h->stream, A, N,
-                             E);
-        } else {
-          hipLaunchKernelGGL(k_f1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N);
-          hipEventRecord(e0, h->stream);
-          hipLaunchKernelGGL(k_f1_limiter, dim3(std::min(nbe, (unsigned)second)), dim3(block), 0,
-                             h->stream, A, N, (int)nbe);
+          if (ens)
+            hipLaunchKernelGGL(k_e1_limiter, dim3(h->ens_lblocks), dim3(block), 0, h->stream, A,
+                               N, E);
+          else
+            hipLaunchKernelGGL(k_f1_limiter, dim3(std::min(nbe, (unsigned)second)), dim3(block),
+                               0, h->stream, A, N, (int)nbe);
+          hipEventRecord(e1, h->stream);
+          if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            set_error("bl_autotune: trial launch failed");
+            rc = BL_FAILURE;
+            break;
+          }
+          float ms = 0.0f;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (r > 0) t.push_back(ms);  // the first run warms the caches
         }
-        hipEventRecord(e1, h->stream);
-        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-          set_error("bl_autotune: trial launch failed");
-          rc = BL_FAILURE;
-          break;
-        }
-        float ms = 0.0f;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (r > 0) t.push_back(ms);  // the first run warms the caches
+        if (rc != BL_SUCCESS) break;
+        std::sort(t.begin(), t.end());
+        (pass == 0 ? tcur : tfull)[trials] = 1e3 * t[t.size() / 2];
       }
-      if (rc != BL_SUCCESS) break;
-      std::sort(t.begin(), t.end());
-      const double us = 1e3 * t[t.size() / 2];
+      cblock[trials] = block;
+      csecond[trials] = second;
       ++trials;
-      if (block == def_block && second == def_second) def_us = us;
-      if (us < best) { best = us; best_block = block; best_second = second; }
     }
   // put back what the trials changed
   hipMemcpyAsync(h->R(BL_CNVISCX), save, c4, hipMemcpyDeviceToDevice, h->stream);
@@ -1714,20 +1735,34 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
     if (ens) ens_set_chunk(h, def_second);
     return rc;
   }
-  h->tune_lim_block = best_block;
+  // the shape whose worse ratio to the best time of the two worklists is
+  // smallest (the first such shape on a tie)
+  double bc = DBL_MAX, bf = DBL_MAX;
+  for (int k = 0; k < trials; ++k) { bc = std::min(bc, tcur[k]); bf = std::min(bf, tfull[k]); }
+  int pick = 0, idef = -1;
+  double score = DBL_MAX;
+  for (int k = 0; k < trials; ++k) {
+    const double sk = std::max(tcur[k] / bc, tfull[k] / bf);
+    if (sk < score) { score = sk; pick = k; }
+    if (cblock[k] == def_block && csecond[k] == def_second) idef = k;
+  }
+  h->tune_lim_block = cblock[pick];
   if (ens) {
-    if (ens_set_chunk(h, best_second)) return BL_FAILURE;
+    if (ens_set_chunk(h, csecond[pick])) return BL_FAILURE;
   } else {
-    h->tune_lim_grid = best_second;
+    h->tune_lim_grid = csecond[pick];
   }
   invalidate_graph(h);
   if (out) {
-    out->limiter_block = best_block;
-    out->limiter_grid = ens ? (int32_t)h->ens_lblocks : (int32_t)std::min(nbe, (unsigned)best_second);
-    out->ens_chunk = ens ? best_second : 0;
+    out->limiter_block = cblock[pick];
+    out->limiter_grid = ens ? (int32_t)h->ens_lblocks
+                            : (int32_t)std::min(nbe, (unsigned)csecond[pick]);
+    out->ens_chunk = ens ? csecond[pick] : 0;
     out->trials = trials;
-    out->best_us = best;
-    out->default_us = def_us;
+    out->best_us = tcur[pick];
+    out->default_us = idef >= 0 ? tcur[idef] : 0.0;
+    out->best_full_us = tfull[pick];
+    out->default_full_us = idef >= 0 ? tfull[idef] : 0.0;
   }
   return BL_SUCCESS;
 }

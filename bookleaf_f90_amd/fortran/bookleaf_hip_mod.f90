@@ -84,7 +84,7 @@ MODULE bookleaf_hip_mod
   ! the launch shape bl_autotune chose (no reference counterpart)
   TYPE,BIND(C) :: bl_tuning_t
     INTEGER(KIND=C_INT32_T) :: limiter_block,limiter_grid,ens_chunk,trials
-    REAL(KIND=C_DOUBLE)     :: best_us,default_us
+    REAL(KIND=C_DOUBLE)     :: best_us,default_us,best_full_us,default_full_us
   END TYPE bl_tuning_t
 
   INTERFACE

@@ -437,13 +437,15 @@ int bl_timing_names(bl_handle_t* h, char* buf, int32_t buflen);
 int bl_run_stats(bl_handle_t* h, int64_t* moving, int64_t* zeroed);
 
 /* Launch-shape autotuner (no counterpart in the reference, whose loops have
- * no launch shape).  Times the limiter pass of the handle's current state
- * -- the viscosity and CFL of the elements with moving nodes, the step's
- * one worklist-driven kernel -- under 15 shapes: 64, 128 or 256 threads per
- * workgroup times 128..2048 workgroups (a single problem) or 1024..16384
- * elements per workgroup and instance (an ensemble), each the median of
- * reps timed launches after one warm-up, and keeps the fastest for every
- * later step.  The state is left as it was (the viscosity fields the trials
+ * no launch shape).  Times the limiter pass -- the viscosity and CFL of the
+ * elements with moving nodes, the step's one worklist-driven kernel -- under
+ * 15 shapes: 64, 128 or 256 threads per workgroup times 128..2048
+ * workgroups (a single problem) or 1024..16384 elements per workgroup and
+ * instance (an ensemble).  Each shape is timed twice, on the worklist of the
+ * handle's current state and with every element listed (the worklist as the
+ * flow spreads), each the median of reps launches after one warm-up; the
+ * shape kept for every later step is the one whose worse ratio to the best
+ * time of the two is smallest.  The state is left as it was (the viscosity fields the trials
  * write are restored) and no result depends on the choice: every reduction
  * of the pass is a first-index extremum over (value, index) pairs.  Fails
  * when the run has ended.  out (may be NULL) receives the choice, the
@@ -453,8 +455,10 @@ typedef struct {
   int32_t limiter_grid;  /* workgroups launched */
   int32_t ens_chunk;     /* ensemble: elements per workgroup and instance; 0 otherwise */
   int32_t trials;        /* shapes timed */
-  double best_us;        /* median duration of the chosen shape */
-  double default_us;     /* median duration of the shape in use before the call */
+  double best_us;        /* median duration of the chosen shape, current worklist */
+  double default_us;     /* the same for the shape in use before the call */
+  double best_full_us;   /* median duration of the chosen shape, every element listed */
+  double default_full_us; /* the same for the shape in use before the call */
 } bl_tuning_t;
 int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out);
 /* Apply a launch shape without timing (one bl_autotune chose for the same

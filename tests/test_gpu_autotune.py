@@ -41,7 +41,11 @@ def test_autotune_keeps_state_and_results(make, oracle_lib):
     assert t["trials"] == 15
     assert t["limiter_block"] in (64, 128, 256)
     assert 1 <= t["limiter_grid"] <= -(-p.nel // 256)
-    assert t["best_us"] > 0.0 and t["default_us"] >= t["best_us"]
+    assert min(t["best_us"], t["default_us"], t["best_full_us"], t["default_full_us"]) > 0.0
+    # the chosen shape is within its own best of the default on one of the
+    # two worklists at least (it minimises the worse of its two ratios)
+    assert (t["best_us"] <= t["default_us"] * 1.0001 or
+            t["best_full_us"] <= t["default_full_us"] * 1.0001)
     assert g.get_tuning()["limiter_block"] == t["limiter_block"]
     # nothing moved
     assert diff_fields(g.state(), before) == []
