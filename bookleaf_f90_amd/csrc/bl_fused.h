@@ -86,6 +86,13 @@ struct FusedArgs {
   // and elvisc hold the +0.0 F1 wrote (F1 sets it; F2 reads it)
   uint64_t* rest;
   unsigned long long* vstats;  // [0] worklist elements, [1] elements F1 zeroed
+  // F1 folded into F4 (a Lagrangian step with pure cells): F4 does F1's work
+  // for the next step at x^{n+1}, u^{n+1} and writes rest_next; F2 writes
+  // the zeros of elements at rest now (rest) but not in the step before
+  // (rest_prev).  Without the fold, F1 writes them and rest is its only mask.
+  int fold;
+  const uint64_t* rest_prev;
+  uint64_t* rest_next;
   DtResult* dtres;
   DevTimestep* ts;
   double* hist;

@@ -188,6 +188,43 @@ __device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane,
 // DBL_MAX.  The DIV term of every element is computed here.  The CFL and
 // DIV selections are first-index minima/maxima over (value, index) pairs,
 // so the split and the worklist order do not change the result.
+
+// F1's terms of element e from the state a step starts from (nodes x, y, u,
+// v; g the geometry at x; cs2e): whether its nodes move, its CFL term when
+// they do not (DBL_MAX otherwise: F1L computes it), its DIV term
+// (hydro_kn_getdtdiv)
+__device__ __forceinline__ void f1_terms(const FusedArgs& A, int e, const double x[4],
+                                         const double y[4], const double u[4], const double v[4],
+                                         const Iso& g, double cs2e, bool& moving, double& cflv,
+                                         double& divv) {
+  moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] && v[0] == v[2] &&
+             v[0] == v[3]);
+  if (!moving) cflv = f1_cfl(A, e, x, y, cs2e, 0.0);
+  divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
+}
+
+// the worklist append (one atomic per wave) and the workgroup's DIV and
+// CFL partials, at part[blockIdx.x] and part[2 * gridDim.x + blockIdx.x];
+// every thread of the workgroup takes part
+__device__ __forceinline__ void f1_collect(const FusedArgs& A, int e, int lane, bool moving,
+                                           double cflv, double divv) {
+  const uint64_t mask = __ballot(moving);
+  if (mask) {
+    const int lead = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(A.work_count, (uint32_t)__popcll(mask));
+    base = __shfl(base, lead, 64);
+    const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (moving && slot < (uint32_t)A.nel) A.work[slot] = e;
+  }
+  const int nb = gridDim.x;
+  block_argmax_first(divv, e + 1, A.part, A.parti);
+  __syncthreads();
+  block_argmin_first(cflv, e + 1, A.part + 2 * nb, A.parti + 2 * nb);
+}
+
+// F1 in a step without the fold into F4, and as the first F1 of a run call
+// with it (A.fold: F2 then writes the zeros)
 __global__ void __launch_bounds__(kBlock)
 k_f1_getdt(FusedArgs A, FusedNodes N) {
   if (step_skipped(A, A.ts)) return;
@@ -201,38 +238,20 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
     load_nd(A.ielnd, e, nd);
     double x[4], y[4], u[4], v[4];
     load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
-    moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] &&
-               v[0] == v[2] && v[0] == v[3]);
-    if (!moving) {
-      // at rest in the previous step of this run too: its zeros are in place
-      if (!((prev >> lane) & 1ull)) {
-        const double z[4] = {0.0, 0.0, 0.0, 0.0};
-        st4(A.cnviscx, e, z);
-        st4(A.cnviscy, e, z);
-        A.elvisc[e] = 0.0;
-        zeroed = true;
-      }
-      cflv = f1_cfl(A, e, x, y, A.cs2[e], 0.0);
-    }
-    // divergence (hydro_kn_getdtdiv) from the geometry at x^n
     Iso g;
     getiso(x, y, g);
-    divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
+    f1_terms(A, e, x, y, u, v, g, A.cs2[e], moving, cflv, divv);
+    // at rest in the previous step of this run too: its zeros are in place
+    if (!moving && !A.fold && !((prev >> lane) & 1ull)) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      st4(A.cnviscx, e, z);
+      st4(A.cnviscy, e, z);
+      A.elvisc[e] = 0.0;
+      zeroed = true;
+    }
   }
   rest_update(A, e, lane, prev, e < A.nel && !moving, zeroed);
-  // worklist append, one atomic per wave
-  const uint64_t mask = __ballot(moving);
-  if (mask) {
-    const int lead = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == lead) base = atomicAdd(A.work_count, (uint32_t)__popcll(mask));
-    base = __shfl(base, lead, 64);
-    if (moving) A.work[base + __popcll(mask & ((1ull << lane) - 1ull))] = e;
-  }
-  const int nb = gridDim.x;
-  block_argmax_first(divv, e + 1, A.part, A.parti);
-  __syncthreads();
-  block_argmin_first(cflv, e + 1, A.part + 2 * nb, A.parti + 2 * nb);
+  f1_collect(A, e, lane, moving, cflv, divv);
 }
 
 // viscosity, elvisc and CFL term of a worklist element (a NaN CFL term
@@ -647,7 +666,9 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   load_nd(A.ielnd, e, nd);
   double x[4], y[4], u[4], v[4];
   load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
-  // an element F1 found at rest has +0.0 viscosity forces: not read
+  // an element F1 found at rest has +0.0 viscosity forces: not read.  With
+  // F1 folded into F4, F2 writes those zeros when the element moved in the
+  // step before (or this is the first step of the run call)
   const bool at_rest = (A.rest[e >> 6] >> (e & 63)) & 1ull;
   double qx[4], qy[4];
   if (at_rest) {
@@ -720,6 +741,20 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
     A.wtT[j * nel + e] = gh.wt[j];
     A.mT[j * nel + e] = mt;
   }
+  if (A.fold) {
+    // the zeros (last, so that no load of this kernel waits behind them)
+    const bool zeroed = at_rest && !((A.rest_prev[e >> 6] >> (e & 63)) & 1ull);
+    if (zeroed) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      st4(A.cnviscx, e, z);
+      st4(A.cnviscy, e, z);
+      A.elvisc[e] = 0.0;
+    }
+    // lanes past nel left at the start, and they are the wave's last lanes
+    const uint64_t wrote = __ballot(zeroed);
+    if ((threadIdx.x & 63) == 0 && wrote)
+      atomicAdd(A.vstats + 1, (unsigned long long)__popcll(wrote));
+  }
 }
 
 // ---------------------------------------------------------------- F3
@@ -766,19 +801,15 @@ k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   O.y[n] = yh + h * v1;
 }
 
-// ---------------------------------------------------------------- F4
-__global__ void __launch_bounds__(kBlock)
-k_f4_state(FusedArgs A, FusedNodes O) {
-  const DevTimestep* ts = A.ts;
-  if (ts->skip || A.dtres->negvol) return;
-  const int e = xcd_index();
-  if (e >= A.nel) return;
+// F4's work for element e: geometry, density, energy and EOS at x^{n+1};
+// returns the new cs2 and leaves nd, x, y and the geometry g for F1's terms
+__device__ __forceinline__ double f4_element(const FusedArgs& A, const FusedNodes& O,
+                                             const DevTimestep* ts, int e, int nd[4],
+                                             double x[4], double y[4], Iso& g) {
   const double dt = A.einst ? A.inst_dt[A.einst[e]] : ts->dt;
-  int nd[4];
   load_nd(A.ielnd, e, nd);
-  double x[4], y[4], ub[4], vb[4];
+  double ub[4], vb[4];
   load4(O.x, nd, x); load4(O.y, nd, y); load4(A.ubar, nd, ub); load4(A.vbar, nd, vb);
-  Iso g;
   getiso(x, y, g);
   if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
   const double m = A.mass[e];
@@ -798,14 +829,48 @@ k_f4_state(FusedArgs A, FusedNodes O) {
   A.en[e] = en;
   A.rho[e] = rho;
   A.pre[e] = eos_pressure(et, ep, rho, en, A.pcut);
-  A.cs2[e] = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
+  const double c2 = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
+  A.cs2[e] = c2;
   if (A.write_geometry) {
     // the remap that follows reads the corner weights and the volume at
     // x^{n+1}; it rewrites the rest of the geometry on the moved mesh
     st4(A.cnwt, e, g.wt);
     A.vol[e] = g.vol;
   }
+  return c2;
 }
+
+// ---------------------------------------------------------------- F4
+__global__ void __launch_bounds__(kBlock)
+k_f4_state(FusedArgs A, FusedNodes O) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || A.dtres->negvol) return;
+  const int e = xcd_index();
+  if (!A.fold) {
+    if (e >= A.nel) return;
+    int nd[4];
+    double x[4], y[4];
+    Iso g;
+    f4_element(A, O, ts, e, nd, x, y, g);
+    return;
+  }
+  // F1 of the next step, from x^{n+1}, u^{n+1} and the cs2 just computed
+  const int lane = threadIdx.x & 63;
+  double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false;
+  if (e < A.nel) {
+    int nd[4];
+    double x[4], y[4], u[4], v[4];
+    Iso g;
+    const double c2 = f4_element(A, O, ts, e, nd, x, y, g);
+    load4(O.u, nd, u); load4(O.v, nd, v);
+    f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
+  }
+  const uint64_t now = __ballot(e < A.nel && !moving);
+  if (lane == 0) A.rest_next[e >> 6] = now;
+  f1_collect(A, e, lane, moving, cflv, divv);
+}
+
 
 // ------------------------------------------------------- remap (Eulerian)
 // The unsplit remap of ale_dr_advect (adv_type = 1) with zeul or the node
@@ -1202,6 +1267,7 @@ FusedArgs make_args(bl_handle_t* h) {
   A.part = h->d_part; A.parti = h->d_parti; A.dtres = h->d_dtres;
   A.work = h->d_work; A.work_count = h->d_ticket;
   A.rest = h->d_rest; A.vstats = h->d_vstats;
+  A.fold = 0; A.rest_prev = nullptr; A.rest_next = nullptr;
   A.ts = h->d_ts;
   A.hist = h->d_hist;
   A.hist_cap = (int)h->hist_cap;
@@ -1239,6 +1305,34 @@ AleArgs make_ale(bl_handle_t* h, const FusedNodesOut& O) {
   return B;
 }
 
+// F1 folds into F4 in a Lagrangian step of pure cells: the next step's
+// start state is then complete when F4 ends (no remap and no mixed-cell
+// update follows)
+bool fold_f1(const bl_handle_t* h) {
+  return !h->cfg.zexist && h->sz.ncp == 0 && h->ens_n == 0;
+}
+
+// the two at-rest masks of a folded run: m[p] is the mask of a step that
+// starts from node set p
+void rest_masks(bl_handle_t* h, uint64_t* m[2]) {
+  m[0] = h->d_rest;
+  m[1] = h->d_rest + 4 * (size_t)nblocks(std::max(h->sz.nel, 1));
+}
+
+// the first F1 of a folded run call, on node set p, outside the step graphs
+int launch_f1_prologue(bl_handle_t* h, int p, int stop_at_time_end) {
+  FusedArgs A = make_args(h);
+  A.stop_at_time_end = stop_at_time_end;
+  A.time_end = h->cfg.time_end;
+  uint64_t* m[2];
+  rest_masks(h, m);
+  A.fold = 1;
+  A.rest = m[p];
+  FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
+  BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nblocks(h->sz.nel), kBlock, A, N);
+  return BL_SUCCESS;
+}
+
 // one step (parity p: node set p -> node set 1-p), launched on h->stream
 int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedArgs A = make_args(h);
@@ -1251,7 +1345,17 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
   const unsigned nbl = std::min(nbe, (unsigned)h->tune_lim_grid);
-  BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
+  if (fold_f1(h)) {
+    // F1 ran in the previous step's F4 (or in the run call's prologue)
+    uint64_t* m[2];
+    rest_masks(h, m);
+    A.fold = 1;
+    A.rest = m[p];
+    A.rest_prev = m[1 - p];
+    A.rest_next = m[1 - p];
+  } else {
+    BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
+  }
   BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nbe);
   BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe, (int)nbl);
   // F2 and F5 (or R1) take a workgroup per 256 mixed cells for the
@@ -1299,8 +1403,9 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
 int ensure_nodesets(bl_handle_t* h) {
   if (!h->d_work) BL_HIP(hipMalloc(&h->d_work, sizeof(int32_t) * std::max(h->sz.nel, 1)));
   if (!h->d_rest) {
-    // one word per wave of F1's grid, the partial last workgroup included
-    const size_t words = 4 * (size_t)nblocks(std::max(h->sz.nel, 1));
+    // one word per wave of F1's grid, the partial last workgroup included,
+    // twice (the current and the previous step's with F1 folded into F4)
+    const size_t words = 8 * (size_t)nblocks(std::max(h->sz.nel, 1));
     BL_HIP(hipMalloc(&h->d_rest, sizeof(uint64_t) * words));
     BL_HIP(hipMalloc(&h->d_vstats, sizeof(unsigned long long) * 2));
     BL_HIP(hipMemset(h->d_rest, 0, sizeof(uint64_t) * words));
@@ -1327,10 +1432,13 @@ int ensure_nodesets(bl_handle_t* h) {
 // every run call, since between calls the host may have written them (an
 // upload, a per-driver call, a write through bl_field_device_ptr).  The
 // first step of the call writes every element's.  Outside any capture.
+// Also empties the worklist.
 int clear_rest(bl_handle_t* h) {
-  BL_HIP(hipMemsetAsync(h->d_rest, 0, sizeof(uint64_t) * 4 * (size_t)nblocks(std::max(h->sz.nel, 1)),
+  BL_HIP(hipMemsetAsync(h->d_rest, 0, sizeof(uint64_t) * 8 * (size_t)nblocks(std::max(h->sz.nel, 1)),
                         h->stream));
   BL_HIP(hipMemsetAsync(h->d_vstats, 0, sizeof(unsigned long long) * 2, h->stream));
+  // the worklist's length: a folded run call ends with the next step's list
+  BL_HIP(hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream));
   return BL_SUCCESS;
 }
 
@@ -1859,6 +1967,7 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   d->nstep = ts->nstep; d->idtel = ts->idtel; d->sdt = ts->sdt;
   BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
   int p = h->cur_set;
+  if (fold_f1(h) && launch_f1_prologue(h, p, stop_at_time_end)) return BL_FAILURE;
   for (int k = 0; k < nsteps; ++k) {
     // with kernel timing on, the same step kernels are launched directly so
     // each launch can be bracketed by HIP events on this stream
