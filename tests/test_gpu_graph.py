@@ -166,3 +166,22 @@ def test_run_stats_and_the_at_rest_mask(oracle_lib):
     o.run(12)
     for g in (a, b):
         assert diff_fields(g.state(), o.state(), compared(p)) == []
+
+
+@pytest.mark.parametrize("name", ["sedov_hg", "sod2mat_mixed_eul"])
+def test_timed_launches_match_graph_runs(name, oracle_lib):
+    """With kernel timing on, bl_run_graph launches the same kernels one by
+    one between HIP events (and, for a Lagrangian step of pure cells, F1 once
+    per call before the steps whose F4 does the next step's F1): the same
+    bits as the oracle, across calls of several lengths."""
+    p = PROBLEMS[name]
+    g, o = _pair(p)
+    g.timing(True)
+    for n in (1, 4, 9):
+        g.run(n, runner="run_graph")
+    names = g.timing_names()
+    g.timing(False)
+    g.run(6, runner="run_graph")
+    o.run(20)
+    assert "f2_force" in names
+    assert diff_fields(g.state(), o.state(), compared(p)) == []
