@@ -96,9 +96,10 @@ KERNEL_BYTES = {
 # (64 B) only for an element in motion
 ZEROED_BYTES = {"f1_getdt": 72, "e1_getdt": 72}
 MOVING_BYTES = {"f2_force": 64}
-# a Lagrangian step of pure cells folds F1 into F4 (F1 then runs once per
-# run call): F4 also reads u, v at x^{n+1} (16 B/node) and writes the next
-# step's at-rest mask (1/8 B/element), and F2 writes the zeros instead of F1
+# a Lagrangian step of pure cells (and every ensemble step) folds F1 (E1)
+# into F4 (F1 then runs once per run call): F4 also reads u, v at x^{n+1}
+# (16 B/node) and writes the next step's at-rest mask (1/8 B/element), and
+# F2 writes the zeros instead of F1; E1's instance id read moves into F4
 FOLD_BYTES = {"f4_state": (0.125, 16)}
 
 
@@ -310,8 +311,9 @@ def main():
     count(args.profile_steps)
     moving, zeroed = mv / max(args.profile_steps, 1), zr / max(args.profile_steps, 1)
     # F1 folded into F4: F1 launched once per run call, not once per step
-    fold = ("f1_getdt" in kern and "f4_state" in kern and
-            kern["f1_getdt"]["launches"] < kern["f4_state"]["launches"])
+    f1name = "e1_getdt" if "e1_getdt" in kern else "f1_getdt"
+    fold = (f1name in kern and "f4_state" in kern and
+            kern[f1name]["launches"] < kern["f4_state"]["launches"])
     dom = max((k for k in kern if k in KERNEL_BYTES), key=lambda k: kern[k]["total_ms"],
               default=None)
     roof = None

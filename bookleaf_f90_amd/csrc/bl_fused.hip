@@ -373,45 +373,18 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
 // extremum over its own (value, index) pairs, as a run of that instance
 // alone makes it.
 
-// E1: F1 with per-instance partial slots and worklist segments
-__global__ void __launch_bounds__(kBlock)
-k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
-  if (step_skipped(A, A.ts)) return;
+// E1's worklist appends into each instance's segment (one atomic per wave
+// and instance) and its two partial slots of each control per workgroup
+// (a workgroup spans at most two instances, a and b); every thread of the
+// workgroup takes part
+__device__ __forceinline__ void e1_collect(const FusedArgs& A, const EnsArgs& E, int e, int lane,
+                                           bool moving, double cflv, double divv) {
   const int lb = xcd_block(), nb = gridDim.x;
-  const int e = lb * kBlock + threadIdx.x;
   const int ia = A.einst[lb * kBlock];
   const int ib = A.einst[min(lb * kBlock + kBlock - 1, A.nel - 1)];
   const int my = e < A.nel ? (int)A.einst[e] : ib;
-  const int lane = threadIdx.x & 63;
-  const uint64_t prev = A.rest[e >> 6];
-  double cflv = DBL_MAX, divv = -1.0;
-  bool moving = false, zeroed = false;
-  if (e < A.nel) {
-    int nd[4];
-    load_nd(A.ielnd, e, nd);
-    double x[4], y[4], u[4], v[4];
-    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
-    moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] &&
-               v[0] == v[2] && v[0] == v[3]);
-    if (!moving) {
-      if (!((prev >> lane) & 1ull)) {
-        const double z[4] = {0.0, 0.0, 0.0, 0.0};
-        st4(A.cnviscx, e, z);
-        st4(A.cnviscy, e, z);
-        A.elvisc[e] = 0.0;
-        zeroed = true;
-      }
-      cflv = f1_cfl(A, e, x, y, A.cs2[e], 0.0);
-      if (!(cflv == cflv)) cflv = DBL_MAX;
-    }
-    Iso g;
-    getiso(x, y, g);
-    divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
-    if (!(divv == divv)) divv = -1.0;
-  }
-  rest_update(A, e, lane, prev, e < A.nel && !moving, zeroed);
-  // worklist append into each instance's segment, one atomic per wave and
-  // instance
+  if (!(cflv == cflv)) cflv = DBL_MAX;
+  if (!(divv == divv)) divv = -1.0;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int k = s == 0 ? ia : ib;
@@ -422,8 +395,9 @@ k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
       uint32_t base = 0;
       if (lane == lead) base = atomicAdd(E.cnt + k, (uint32_t)__popcll(mask));
       base = __shfl(base, lead, 64);
-      if (moving && my == k)
-        A.work[E.eoff[k] + base + __popcll(mask & ((1ull << lane) - 1ull))] = e;
+      const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (moving && my == k && slot < (uint32_t)(E.eoff[k + 1] - E.eoff[k]))
+        A.work[E.eoff[k] + slot] = e;
     }
   }
   block_reduce_at<false>(my == ia ? divv : -1.0, e + 1, E.part + lb, E.parti + lb);
@@ -435,6 +409,35 @@ k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
   __syncthreads();
   block_reduce_at<true>(my == ib ? cflv : DBL_MAX, e + 1, E.part + 3 * nb + lb,
                         E.parti + 3 * nb + lb);
+}
+
+// E1: F1 with per-instance partial slots and worklist segments
+__global__ void __launch_bounds__(kBlock)
+k_e1_getdt(FusedArgs A, FusedNodes N, EnsArgs E) {
+  if (step_skipped(A, A.ts)) return;
+  const int e = xcd_index();
+  const int lane = threadIdx.x & 63;
+  const uint64_t prev = A.rest[e >> 6];
+  double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false, zeroed = false;
+  if (e < A.nel) {
+    int nd[4];
+    load_nd(A.ielnd, e, nd);
+    double x[4], y[4], u[4], v[4];
+    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+    Iso g;
+    getiso(x, y, g);
+    f1_terms(A, e, x, y, u, v, g, A.cs2[e], moving, cflv, divv);
+    if (!moving && !A.fold && !((prev >> lane) & 1ull)) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      st4(A.cnviscx, e, z);
+      st4(A.cnviscy, e, z);
+      A.elvisc[e] = 0.0;
+      zeroed = true;
+    }
+  }
+  rest_update(A, e, lane, prev, e < A.nel && !moving, zeroed);
+  e1_collect(A, E, e, lane, moving, cflv, divv);
 }
 
 // E1L: instance k's workgroups lboff[k] .. lboff[k+1]-1 stride over its
@@ -869,6 +872,29 @@ k_f4_state(FusedArgs A, FusedNodes O) {
   const uint64_t now = __ballot(e < A.nel && !moving);
   if (lane == 0) A.rest_next[e >> 6] = now;
   f1_collect(A, e, lane, moving, cflv, divv);
+}
+
+// F4 of an ensemble step with E1 of the next step folded in (E1's terms as
+// k_f4_state's fold, E1's per-instance worklist segments and partial slots)
+__global__ void __launch_bounds__(kBlock)
+k_f4_state_ens(FusedArgs A, FusedNodes O, EnsArgs E) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || A.dtres->negvol) return;
+  const int e = xcd_index();
+  const int lane = threadIdx.x & 63;
+  double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false;
+  if (e < A.nel) {
+    int nd[4];
+    double x[4], y[4], u[4], v[4];
+    Iso g;
+    const double c2 = f4_element(A, O, ts, e, nd, x, y, g);
+    load4(O.u, nd, u); load4(O.v, nd, v);
+    f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
+  }
+  const uint64_t now = __ballot(e < A.nel && !moving);
+  if (lane == 0) A.rest_next[e >> 6] = now;
+  e1_collect(A, E, e, lane, moving, cflv, divv);
 }
 
 
@@ -1439,6 +1465,8 @@ int clear_rest(bl_handle_t* h) {
   BL_HIP(hipMemsetAsync(h->d_vstats, 0, sizeof(unsigned long long) * 2, h->stream));
   // the worklist's length: a folded run call ends with the next step's list
   BL_HIP(hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream));
+  if (h->ens_n > 0 && h->d_cnt)
+    BL_HIP(hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream));
   return BL_SUCCESS;
 }
 
@@ -1587,13 +1615,19 @@ int launch_ensemble_step(bl_handle_t* h, int p) {
   FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
-  BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nbe, kBlock, A, N, E);
+  // E1 ran in the previous step's F4 (or in the run call's prologue)
+  uint64_t* m[2];
+  rest_masks(h, m);
+  A.fold = 1;
+  A.rest = m[p];
+  A.rest_prev = m[1 - p];
+  A.rest_next = m[1 - p];
   BL_LAUNCH(h, "e1_limiter", k_e1_limiter, (unsigned)h->ens_lblocks, (unsigned)h->tune_lim_block,
             A, N, E);
   BL_LAUNCH(h, "e1b_select", k_e1b_select, (unsigned)h->ens_n, kBlock, A, E, (int)nbe);
   BL_LAUNCH(h, "f2_force", k_f2_force, nbe, kBlock, A, N, 0);
   BL_LAUNCH(h, "f3_accel", k_f3_accel, nbn, kBlock, A, N, O);
-  BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
+  BL_LAUNCH(h, "f4_state", k_f4_state_ens, nbe, kBlock, A, Oc, E);
   BL_LAUNCH(h, "f5_end", k_f5_end, 1, kBlock, A, Oc);
   return BL_SUCCESS;
 }
@@ -1658,6 +1692,22 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   d->hist_base = start;
   BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
   int p = h->cur_set;
+  {
+    // E1 of the first step; every later step's runs in the F4 before it
+    FusedArgs A = make_args(h);
+    A.stop_at_time_end = 0;
+    A.time_end = h->cfg.time_end;
+    A.einst = h->d_einst;
+    A.ninst = h->d_ninst;
+    A.inst_dt = h->d_inst_dt;
+    uint64_t* m[2];
+    rest_masks(h, m);
+    A.fold = 1;
+    A.rest = m[p];
+    const EnsArgs E = make_ens(h);
+    FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
+    BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nblocks(h->sz.nel), kBlock, A, N, E);
+  }
   for (int k = 0; k < nsteps; ++k) {
     if (h->timing) {
       if (launch_ensemble_step(h, p)) return BL_FAILURE;
