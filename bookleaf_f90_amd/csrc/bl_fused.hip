@@ -821,8 +821,10 @@ __device__ __forceinline__ double f4_element(const FusedArgs& A, const FusedNode
   const int nel = A.nel;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    fx[j] = A.fxT[j * nel + e];
-    fy[j] = A.fyT[j * nel + e];
+    // read once and never again: non-temporal loads (F4 2 µs faster at
+    // 1 M elements)
+    fx[j] = __builtin_nontemporal_load(A.fxT + j * nel + e);
+    fy[j] = __builtin_nontemporal_load(A.fyT + j * nel + e);
   }
   const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
   int mat = A.ielmat[e];
