@@ -185,3 +185,33 @@ def test_timed_launches_match_graph_runs(name, oracle_lib):
     o.run(20)
     assert "f2_force" in names
     assert diff_fields(g.state(), o.state(), compared(p)) == []
+
+
+def test_graph_negative_volume_matches(oracle_lib):
+    """A node pushed across its neighbours with the viscosity and the DIV
+    control off inverts an element in the first step's predictor: the graph
+    runner stops at the same step with the same error as the host drivers,
+    the step's viscosity fields as getdt left them.  The drivers stop after
+    the predictor moved ndx/ndy to the half step in place
+    (hydro_dr_getstate.f90:33-51); the graph runner keeps its half-step
+    positions in registers and leaves x^n."""
+    from bookleaf_f90_amd.device import BookLeafError
+    p = P.sod(nx=8, ny=3, nsteps=5, dt_initial=1.0e-2)
+    p.config.update(cvisc1=0.0, cvisc2=0.0, div_sf=1.0e30)
+    g, o = _pair(p)
+    for s in (g, o):
+        u = s.get("ndu")
+        u[12] += 100.0
+        s.set("ndu", u)
+    x0, u0 = g.get("ndx"), g.get("ndu")
+    with pytest.raises(BookLeafError) as eg:
+        g.run(5, runner="run_graph")
+    with pytest.raises(BookLeafError) as eo:
+        o.run(5)
+    assert eg.value.ierr == eo.value.ierr < 0
+    assert "negative volume" in str(eg.value) and "negative volume" in str(eo.value)
+    assert g.ts.nstep == o.ts.nstep == 1
+    assert diff_fields(g.state(), o.state(), ["ndy", "ndu", "ndv", "cnviscx", "cnviscy",
+                                              "elvisc"]) == []
+    assert np.array_equal(g.get("ndx"), x0)
+    assert np.array_equal(o.get("ndx"), x0 + (0.5 * g.ts.dt) * u0)
