@@ -295,7 +295,7 @@ int bl_ensemble_create(const bl_instance_t* inst, int32_t ninst, const bl_config
       dev_alloc(&h->d_ale_ptr, nnd + 1) || dev_alloc(&h->d_ale_idx, 4 * nel) ||
       dev_alloc(&h->d_el_ptr, nnd + 1) || dev_alloc(&h->d_el_idx, 4 * nel) ||
       dev_alloc(&h->d_outer, 8 * nel) || dev_alloc((InstTs**)&h->d_inst, ninst) ||
-      dev_alloc(&h->d_inst_dt, ninst) || dev_alloc(&h->d_cnt, ninst) ||
+      dev_alloc(&h->d_inst_dt, 2 * (size_t)ninst) || dev_alloc(&h->d_cnt, ninst) ||
       dev_alloc(&h->d_lboff, ninst + 1) || ens_set_chunk(h, h->tune_ens_chunk)) {
     hipFree(d_noff); hipFree(d_inst);
     return fail();
@@ -308,9 +308,11 @@ int bl_ensemble_create(const bl_instance_t* inst, int32_t ninst, const bl_config
     t.time = c.time_start; t.dt = c.dt_initial; t.nstep = 0; t.idtel = 0;
     t.sdt = BL_DT_INITIAL; t.pad = 0;
   }
-  std::vector<double> dts(ninst, c.dt_initial);
+  // each instance's dt, then its 1/dt (forceHG's factor, formed with the dt)
+  std::vector<double> dts(2 * (size_t)ninst, c.dt_initial);
+  for (int k = 0; k < ninst; ++k) dts[ninst + k] = 1.0 / c.dt_initial;
   hipMemcpy(h->d_inst, its.data(), sizeof(InstTs) * ninst, hipMemcpyHostToDevice);
-  hipMemcpy(h->d_inst_dt, dts.data(), sizeof(double) * ninst, hipMemcpyHostToDevice);
+  hipMemcpy(h->d_inst_dt, dts.data(), sizeof(double) * 2 * ninst, hipMemcpyHostToDevice);
   hipMemset(h->d_cnt, 0, sizeof(uint32_t) * ninst);
 
   GenArgs G{};

@@ -103,6 +103,7 @@ struct FusedArgs {
   const uint16_t* einst;
   const uint16_t* ninst;
   const double* inst_dt;
+  const double* inst_rdt;  // 1/inst_dt, formed where each instance's dt is selected
 };
 
 // the timestep of one ensemble instance
@@ -121,6 +122,7 @@ struct EnsArgs {
   const int32_t* eoff;
   InstTs* inst;
   double* inst_dt;
+  double* inst_rdt;
   uint32_t* cnt;
   double* part;
   int32_t* parti;

@@ -32,8 +32,8 @@
 // reference drivers; the intermediate half-step pressure/energy never reach
 // HBM.  Fields that hold only phase scratch in the reference (rscratch*) are
 // not written except ubar (rscratch14/15, which the ALE step reads) and the
-// corrector corner forces (rscratch23/24: by F4 when the remap follows, else
-// once at the end of a batch).
+// corrector corner forces (rscratch23/24), which k_materialize writes once at
+// the end of a batch from the last step's corner-major copies.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -346,6 +346,7 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
   select_dt(A, ts->dt, ts->nstep, r.cfl_min, r.cfl_idx, r.div_max, r.div_idx, best, bidx, bsdt);
   ts->nstep = ts->nstep + 1;
   ts->dt = best;
+  ts->rdt = 1.0 / best;  // hydro_kn_getforce.f90:88 (w4 = 1/dt, once per call)
   ts->idtel = bidx;
   ts->sdt = bsdt;
   ts->time = ts->time + best;
@@ -463,7 +464,11 @@ k_e1_limiter(FusedArgs A, FusedNodes N, EnsArgs E) {
 }
 
 // E1b: one workgroup per instance -- its partials, then its dt selection;
-// workgroup 0 also runs the step prologue of the whole ensemble
+// workgroup 0 also runs the step prologue of the whole ensemble.  No
+// workgroup writes ts->done, which every workgroup reads at its start: an
+// instance whose selection fails lowers ts->epend instead (the smallest
+// failing instance wins, whatever the workgroup order), the kernels after
+// this one skip the step on it, and the step's last kernel sets done.
 __global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, int nb) {
   DevTimestep* ts = A.ts;
   const int k = blockIdx.x, t = threadIdx.x;
@@ -471,7 +476,6 @@ __global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, i
   __syncthreads();
   if (t == 0) {
     if (k == 0) {
-      if (skip) ts->done = 1;
       ts->skip = skip ? 1 : 0;
       A.dtres->negvol = 0;
       A.dtres->negvol2 = 0;
@@ -506,8 +510,7 @@ __global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, i
   InstTs& it = E.inst[k];
   const int off = E.eoff[k];
   if (svc < 0.0) {  // hydro_kn_getdt.f90:75-78
-    ts->err = 2; ts->done = 1; ts->skip = 1; ts->err_step = it.nstep + 1;
-    ts->err_idx = k;
+    atomicMin(&ts->epend, 4 * k + 2);
     return;
   }
   double best;
@@ -527,10 +530,14 @@ __global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, i
   it.sdt = bsdt;
   it.time = it.time + best;
   E.inst_dt[k] = best;
-  if (best < A.dt_min) {  // time_dr_advance.f90:122-127
-    ts->err = 3; ts->done = 1; ts->skip = 1; ts->err_step = it.nstep;
-    ts->err_idx = k;
-  }
+  E.inst_rdt[k] = 1.0 / best;
+  if (best < A.dt_min) atomicMin(&ts->epend, 4 * k + 3);  // time_dr_advance.f90:122-127
+}
+
+// an ensemble instance's dt selection failed in this step (E1b): nothing
+// after the selection runs, as the reference halts in time_dr_end
+__device__ __forceinline__ bool ens_pending(const FusedArgs& A, const DevTimestep* ts) {
+  return A.einst != nullptr && ts->epend != kNoPending;
 }
 
 // component terms of a mixed cell in F2 (hydro_dr_getforce.f90:52-66,
@@ -652,7 +659,7 @@ __device__ __forceinline__ void f4_mix_cell(const FusedArgs& A, const FusedNodes
 __global__ void __launch_bounds__(kBlock)
 k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   const DevTimestep* ts = A.ts;
-  if (ts->skip) return;
+  if (ts->skip || ens_pending(A, ts)) return;
   // the mixed cells' workgroups come first, so that their longer serial
   // work starts with the grid rather than trailing it; nbm is a multiple of
   // the XCD count, so the element workgroups keep their XCD mapping
@@ -663,7 +670,8 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   }
   const int e = xcd_block_of((int)blockIdx.x - nbm, (int)gridDim.x - nbm) * kBlock + threadIdx.x;
   if (e >= A.nel) return;
-  const double dt = A.einst ? A.inst_dt[A.einst[e]] : ts->dt;
+  const int inst = A.einst ? (int)A.einst[e] : -1;
+  const double dt = inst >= 0 ? A.inst_dt[inst] : ts->dt;
   const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
   int nd[4];
   load_nd(A.ielnd, e, nd);
@@ -722,7 +730,9 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
     const double cs2h = eos_cs2(et, ep, rhoh, eh, A.pcut, A.ccut);
     forceSP(A.pmeritreg[ireg], rhoh, cs2h, xh, yh, spm, fx, fy);
   }
-  if (A.zhg) forceHG(A.kappareg[ireg], rhoh, gh.vol, 1.0 / dt, u, v, fx, fy);
+  if (A.zhg)
+    forceHG(A.kappareg[ireg], rhoh, gh.vol, inst >= 0 ? A.inst_rdt[inst] : ts->rdt, u, v, fx,
+            fy);
   // nodal-mass term of each corner as hydro_kn_scatteracceleration picks it
   // (hydro_kn_getacceleration.f90:58-67): cnmass, else the previous corner's
   // cnmass, else rho*cnwt at the half step
@@ -764,7 +774,7 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
 __global__ void __launch_bounds__(kBlock)
 k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   const DevTimestep* ts = A.ts;
-  if (ts->skip || A.dtres->negvol) return;
+  if (ts->skip || A.dtres->negvol || ens_pending(A, ts)) return;
   const int n = xcd_index();
   if (n >= A.nnd) return;
   const double dt = A.ninst ? A.inst_dt[A.ninst[n]] : ts->dt;
@@ -821,8 +831,9 @@ __device__ __forceinline__ double f4_element(const FusedArgs& A, const FusedNode
   const int nel = A.nel;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    // read once and never again: non-temporal loads (F4 2 µs faster at
-    // 1 M elements)
+    // read once in the step (k_materialize reads the last step's copy
+    // again after the batch): non-temporal loads, F4 2 µs faster at 1 M
+    // elements
     fx[j] = __builtin_nontemporal_load(A.fxT + j * nel + e);
     fy[j] = __builtin_nontemporal_load(A.fyT + j * nel + e);
   }
@@ -881,7 +892,7 @@ k_f4_state(FusedArgs A, FusedNodes O) {
 __global__ void __launch_bounds__(kBlock)
 k_f4_state_ens(FusedArgs A, FusedNodes O, EnsArgs E) {
   const DevTimestep* ts = A.ts;
-  if (ts->skip || A.dtres->negvol) return;
+  if (ts->skip || A.dtres->negvol || ens_pending(A, ts)) return;
   const int e = xcd_index();
   const int lane = threadIdx.x & 63;
   double cflv = DBL_MAX, divv = -1.0;
@@ -1212,6 +1223,10 @@ __global__ void __launch_bounds__(kBlock) k_f5_end(FusedArgs A, FusedNodes O) {
     return;
   }
   if (threadIdx.x != 0 || ts->skip) return;
+  if (ens_pending(A, ts)) {  // the host reads the instance and code from epend
+    ts->done = 1; ts->skip = 1;
+    return;
+  }
   if (res->negvol) {
     ts->err = 1; ts->err_idx = -res->negvol; ts->done = 1; ts->skip = 1;
     ts->err_step = ts->nstep;
@@ -1590,6 +1605,7 @@ EnsArgs make_ens(bl_handle_t* h) {
   E.eoff = h->d_eoff;
   E.inst = static_cast<InstTs*>(h->d_inst);
   E.inst_dt = h->d_inst_dt;
+  E.inst_rdt = h->d_inst_dt + h->ens_n;
   E.cnt = h->d_cnt;
   const size_t nb = nblocks(h->sz.nel);
   E.part = h->d_epart;
@@ -1610,6 +1626,7 @@ int launch_ensemble_step(bl_handle_t* h, int p) {
   A.einst = h->d_einst;
   A.ninst = h->d_ninst;
   A.inst_dt = h->d_inst_dt;
+  A.inst_rdt = h->d_inst_dt + h->ens_n;
   EnsArgs E = make_ens(h);
   const int nel = h->sz.nel, nnd = h->sz.nnd;
   const unsigned nbe = nblocks(nel), nbn = nblocks(nnd);
@@ -1692,6 +1709,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   const int start = d->nstep;
   d->done = 0; d->skip = 0; d->err = 0; d->err_step = 0; d->err_idx = 0; d->flips = 0;
   d->hist_base = start;
+  d->epend = kNoPending;
   BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
   int p = h->cur_set;
   {
@@ -1702,6 +1720,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
     A.einst = h->d_einst;
     A.ninst = h->d_ninst;
     A.inst_dt = h->d_inst_dt;
+  A.inst_rdt = h->d_inst_dt + h->ens_n;
     uint64_t* m[2];
     rest_masks(h, m);
     A.fold = 1;
@@ -1721,6 +1740,11 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));
   const int taken = d->nstep - start;
+  if (d->err == 0 && d->epend != kNoPending) {
+    // the smallest instance whose dt selection failed, and how
+    d->err = d->epend & 3;
+    d->err_idx = d->epend >> 2;
+  }
   point_fields_at(h, (h->cur_set + d->flips) & 1);
   {
     int nel = h->sz.nel;
@@ -1816,6 +1840,7 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
     A.einst = h->d_einst;
     A.ninst = h->d_ninst;
     A.inst_dt = h->d_inst_dt;
+  A.inst_rdt = h->d_inst_dt + h->ens_n;
   }
   // worklist lengths of the "every element" trials
   std::vector<uint32_t> full(ens ? h->ens_n : 1, (uint32_t)nel);
@@ -1976,6 +2001,8 @@ int bl_get_tuning(bl_handle_t* h, bl_tuning_t* t) {
   t->trials = 0;
   t->best_us = 0.0;
   t->default_us = 0.0;
+  t->best_full_us = 0.0;
+  t->default_full_us = 0.0;
   return BL_SUCCESS;
 }
 

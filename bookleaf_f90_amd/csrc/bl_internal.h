@@ -43,7 +43,8 @@ struct DtResult {
 // device-resident timestep state of the fused step (bl_run_graph): the
 // fields of timestep_t (dataAPI_types.f90:46-52) plus the batch control
 struct DevTimestep {
-  double time, dt, dts, pad_d;
+  double time, dt, dts;
+  double rdt;        // 1/dt, formed once where dt is selected (hydro_kn_getforce.f90:88)
   int32_t nstep, idtel, sdt, ale_on;
   int32_t done;      // time_end reached or an error: later steps do nothing
   int32_t skip;      // this step does nothing (set by the step prologue)
@@ -53,7 +54,14 @@ struct DevTimestep {
   int32_t hist_n;    // steps recorded in the history buffer
   int32_t flips;     // steps whose node update wrote the other node set
   int32_t hist_base;  // ensembles: instance step count at the batch's start
+  // ensembles: the smallest 4*instance + code over the instances whose dt
+  // selection failed (code 2 negative CFL, 3 dt < dt_min), kNoPending when
+  // none; written only by atomicMin in the dt selection, turned into
+  // done/skip by the step's last kernel and into err by the host
+  int32_t epend;
+  int32_t pad_i;
 };
+constexpr int32_t kNoPending = 0x7fffffff;
 
 // per-launch kernel timing with HIP events on the handle's stream
 struct TimingClass {

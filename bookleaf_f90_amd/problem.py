@@ -504,6 +504,42 @@ def sod2mat_mixed(nx=2048, ny=256, length=1.0, amplitude=0.02, seed=2024, nsteps
     return p
 
 
+# Tait (eos_kn_forms.f90, type 2): p = max(a ((rho/rho0)^b - 1), pmin),
+# params (a, b, rho0, pmin); JWL (type 3): p = w rho e + A (1 - w/t1) exp(-t1)
+# + B (1 - w/t2) exp(-t2), t1 = R1 rho0/rho, t2 = R2 rho0/rho, params
+# (w, A, B, R1, R2, rho0).  Nondimensional water-like and TNT-like values.
+TAIT_PARAM = (0.1, 7.0, 1.0, -0.05, 0.0, 0.0)
+JWL_PARAM = (0.3, 3.712, 0.0323, 4.15, 0.95, 1.63)
+
+
+def tait_jwl(nx=40, ny=4, length=1.0, nsteps=60, eulerian=False, **cfg):
+    """A JWL detonation-products column (material 1: rho 1.63, e 1) driving a
+    Tait liquid (material 2: rho 1, e 0, at its reference density) on
+    [0,L]x[0,L*ny/nx]; the interface at mid-length, reflective walls.  The
+    reference's two non-ideal EOS forms, whose pow and exp the ideal-gas
+    workloads never reach."""
+    height = length * ny / nx
+    ielnd, ndx, ndy, indtype = structured_mesh(nx, ny, 0.0, length, 0.0, height)
+    xc, _ = centroids(ielnd, ndx, ndy)
+    left = xc < 0.5 * length
+    nel = nx * ny
+    itype = np.array([0, 3, 2], np.int32)
+    param = np.zeros(abi.NEOSPARAM * 3)
+    param[abi.NEOSPARAM:2 * abi.NEOSPARAM] = JWL_PARAM
+    param[2 * abi.NEOSPARAM:] = TAIT_PARAM
+    mat = np.where(left, 1, 2).astype(np.int32)
+    config = dict(DEFAULT_CONFIG, time_end=1.0e30, dt_initial=5.0e-3)
+    config.update(cfg)
+    if eulerian:
+        config["zeul"] = 1
+    return Problem(
+        name="tait_jwl_%dx%d%s" % (nx, ny, "_eul" if eulerian else ""), ielnd=ielnd, ndx=ndx,
+        ndy=ndy, indtype=indtype, ielreg=mat.copy(), ielmat=mat,
+        density=np.where(left, JWL_PARAM[5], TAIT_PARAM[2]), energy=np.where(left, 1.0, 0.0),
+        ndu=np.zeros(len(ndx)), ndv=np.zeros(len(ndx)), nmat=2, nreg=2, eos_itype=itype,
+        eos_param=param, config=config, nsteps=nsteps, dims=(nx, ny))
+
+
 def noh(n=30, length=1.0, gamma=5.0 / 3.0, nsteps=100, **cfg):
     """Noh implosion on a Cartesian quarter [0,L]^2: rho=1, e=0, radial inward
     velocity of unit speed (the kinematic 'radial' IC of input/noh_small),

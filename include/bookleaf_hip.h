@@ -447,8 +447,10 @@ int bl_run_stats(bl_handle_t* h, int64_t* moving, int64_t* zeroed);
  * shape kept for every later step is the one whose worse ratio to the best
  * time of the two is smallest.  The state is left as it was (the viscosity fields the trials
  * write are restored) and no result depends on the choice: every reduction
- * of the pass is a first-index extremum over (value, index) pairs.  Fails
- * when the run has ended.  out (may be NULL) receives the choice, the
+ * of the pass is a first-index extremum over (value, index) pairs.  A handle
+ * whose run has ended (time_end reached, or an error) is tuned as the next
+ * run call would start it, and its device timestep is put back as it was.
+ * out (may be NULL) receives the choice, the
  * number of shapes timed, and the chosen and the default shape's times. */
 typedef struct {
   int32_t limiter_block; /* threads per workgroup */
@@ -466,7 +468,8 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out);
  * limiter_grid (>= 1, a single problem) or ens_chunk (>= 64, an ensemble);
  * the other fields are ignored.  No result depends on it. */
 int bl_set_tuning(bl_handle_t* h, const bl_tuning_t* t);
-/* the shape in use: limiter_grid is the number of workgroups launched */
+/* the shape in use: limiter_grid is the number of workgroups launched;
+ * trials and the four times are zero */
 int bl_get_tuning(bl_handle_t* h, bl_tuning_t* t);
 
 /* last error message of the library (thread-local), for diagnostics */

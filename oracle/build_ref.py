@@ -20,7 +20,15 @@ problem-specific MOD, and our own host program in place of
 inf_control_main.f90.  Flags: -O2 -ffp-contract=off (no FMA contraction),
 the same association rules as the HIP build.
 
-usage: python oracle/build_ref.py [--force]
+With --dropin it also builds oracle/_ref/bookleaf_ref_dropin: the same
+reference sources and host program, with the reference's hot-path driver
+modules (REPLACED below) left out and bookleaf_f90_amd/fortran/dropin/
+compiled in their place (same module names), linked to libbookleaf_hip.so.
+The reference's unchanged inf_solver_* then drive the GPU; the host program
+is compiled with -DBL_DROPIN so that its dumps first bring the device
+fields back.  Objects go to oracle/_ref/dropin_obj and dropin_mod.
+
+usage: python oracle/build_ref.py [--force] [--dropin]
 """
 import os
 import re
@@ -41,6 +49,14 @@ EXCLUDE_FILES = {
     "io_dr_tio.F90",             # TyphonIO only (not in makefile.IODR)
     "hydro_kn_getclosure.f90",   # not in makefile.HYDROKN (unbuildable stub)
 }
+# the driver modules the drop-in replaces (src/packages/{hydro,ale}/driver)
+REPLACED = {"hydro_dr_getdt.f90", "hydro_dr_set.f90", "hydro_dr_getforce.f90",
+            "hydro_dr_getstate.f90", "hydro_dr_getenergy.f90", "hydro_dr_geteos.f90",
+            "hydro_dr_getacceleration.f90", "ale_dr_getmeshstatus.f90",
+            "ale_dr_getfluxvolume.f90", "ale_dr_advect.f90", "ale_dr_update.f90"}
+ROOT = os.path.dirname(HERE)
+FORTRAN = os.path.join(ROOT, "bookleaf_f90_amd", "fortran")
+LIBDIR = os.path.join(ROOT, "bookleaf_f90_amd")
 INTRINSIC = {"iso_c_binding", "iso_fortran_env", "omp_lib", "mpi",
              "ieee_arithmetic", "ieee_exceptions"}
 
@@ -112,6 +128,50 @@ def run(cmd):
         raise SystemExit(1)
 
 
+def compile_all(files, objdir, moddir, force, extra=()):
+    """Compile in module order; a file is rebuilt when its object is older
+    than its source or a module it uses was rebuilt in this pass."""
+    info = {f: scan(f) for f in files}
+    rebuilt_mods, objs = set(), []
+    for f in order(files):
+        base = os.path.splitext(os.path.basename(f))[0]
+        obj = os.path.join(objdir, base + ".o")
+        defs, uses = info[f]
+        if (force or not os.path.exists(obj) or os.path.getmtime(obj) < os.path.getmtime(f)
+                or uses & rebuilt_mods):
+            run([FLANG, "-c"] + FFLAGS + list(extra) + ["-module-dir", moddir, "-I", moddir,
+                                                      f, "-o", obj])
+            rebuilt_mods |= defs
+        objs.append(obj)
+    return objs
+
+
+def build_dropin(force):
+    """oracle/_ref/bookleaf_ref_dropin: the reference with its hot-path
+    driver modules replaced by the drop-in modules (GPU)."""
+    lib = os.path.join(LIBDIR, "libbookleaf_hip.so")
+    if not os.path.exists(lib):
+        raise SystemExit("build libbookleaf_hip.so first (python -m bookleaf_f90_amd.build)")
+    objdir = os.path.join(OUT, "dropin_obj")
+    moddir = os.path.join(OUT, "dropin_mod")
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(moddir, exist_ok=True)
+    dropin = sorted(os.path.join(FORTRAN, "dropin", n)
+                    for n in os.listdir(os.path.join(FORTRAN, "dropin")) if n.endswith(".f90"))
+    files = [f for f in reference_sources() if os.path.basename(f) not in REPLACED]
+    files += [os.path.join(FORTRAN, "bookleaf_hip_mod.f90")] + dropin
+    objs = compile_all(files, objdir, moddir, force)
+    host = os.path.join(HERE, "ref", "ref_driver.f90")
+    hobj = os.path.join(objdir, "ref_driver.o")
+    run([FLANG, "-c", "-cpp", "-DBL_DROPIN"] + FFLAGS + ["-module-dir", moddir, "-I", moddir,
+                                                         host, "-o", hobj])
+    exe = os.path.join(OUT, "bookleaf_ref_dropin")
+    rpath = "-Wl,-rpath,$ORIGIN/" + os.path.relpath(LIBDIR, OUT)
+    run([FLANG, "-O2"] + objs + [hobj, "-L", LIBDIR, "-lbookleaf_hip", rpath, "-o", exe])
+    print("built", exe, "from", len(files) - len(dropin) - 1, "reference sources and",
+          len(dropin), "drop-in modules")
+
+
 def main():
     force = "--force" in sys.argv
     if not os.path.isdir(SRC):
@@ -125,23 +185,25 @@ def main():
     hosts = {"bookleaf_ref": os.path.join(HERE, "ref", "ref_driver.f90"),
              "bookleaf_meshgen": os.path.join(HERE, "ref", "ref_meshgen.f90")}
 
-    def compile_one(f):
+    def compile_one(f, extra=()):
         base = os.path.splitext(os.path.basename(f))[0]
         obj = os.path.join(objdir, base + ".o")
         if force or not os.path.exists(obj) or os.path.getmtime(obj) < os.path.getmtime(f):
-            run([FLANG, "-c"] + FFLAGS + ["-module-dir", moddir, "-I", moddir,
-                                         f, "-o", obj])
+            run([FLANG, "-c"] + list(extra) + FFLAGS + ["-module-dir", moddir, "-I", moddir,
+                                                        f, "-o", obj])
         return obj
 
     objs = [compile_one(f) for f in files]
     newest = max(os.path.getmtime(o) for o in objs)
     for name, host in hosts.items():
-        hobj = compile_one(host)
+        hobj = compile_one(host, ("-cpp",))
         exe = os.path.join(OUT, name)
         if force or not os.path.exists(exe) or os.path.getmtime(exe) < max(
                 newest, os.path.getmtime(hobj)):
             run([FLANG, "-O2"] + objs + [hobj, "-o", exe])
         print("built", exe, "from", len(files), "reference sources")
+    if "--dropin" in sys.argv:
+        build_dropin(force)
     return 0
 
 

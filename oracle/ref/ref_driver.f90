@@ -10,6 +10,12 @@
 ! inf_control_solver loop (inf_control_solver.f90:53-69) calling the
 ! reference's inf_solver_getdt / inf_solver_lagstep / inf_solver_alestep.
 !
+! Built with -DBL_DROPIN (oracle/build_ref.py --dropin), the reference's
+! hot-path driver modules are replaced by bookleaf_f90_amd/fortran/dropin/
+! (same module and procedure names), so the same inf_solver_* loop runs the
+! step on the GPU; every dump first brings the device fields back
+! (bookleaf_dropin_sync).
+!
 ! usage: bookleaf_ref <problem.bin> <dump.bin> [mode]
 !   mode 0 (default): full steps, dump every "dump_every" steps
 !   mode 1: granular -- step 1 only, dump after every driver call
@@ -64,6 +70,9 @@ PROGRAM bl_ref_driver
   USE hydro_kn_print_mod,      ONLY: hydro_kn_init_shortprint,                 &
 &                                    hydro_kn_calc_shortprint
   USE utils_kn_gather_mod,     ONLY: utils_kn_cngather,utils_kn_mxgathercncp
+#ifdef BL_DROPIN
+  USE bookleaf_dropin_mod,     ONLY: bookleaf_dropin_sync,bookleaf_dropin_detach
+#endif
 
   IMPLICIT NONE
 
@@ -294,6 +303,9 @@ PROGRAM bl_ref_driver
   ENDIF
   WRITE(iu) -9_ink
   CLOSE(iu)
+#ifdef BL_DROPIN
+  CALL bookleaf_dropin_detach()
+#endif
 
 CONTAINS
 
@@ -310,6 +322,10 @@ CONTAINS
     INTEGER(KIND=ink),INTENT(IN) :: label
     INTEGER(KIND=ink) :: id,knd
     LOGICAL(KIND=lak),DIMENSION(:),POINTER :: zp
+#ifdef BL_DROPIN
+    ! the drop-in drivers keep the fields on the GPU: bring them back first
+    CALL bookleaf_dropin_sync(dh)
+#endif
     WRITE(iu) 1_ink,label,runtime%timestep%nstep,runtime%timestep%idtel,       &
 &             runtime%timestep%dt,runtime%timestep%time,runtime%timestep%dts
     DO id=1,SIZE(dh)

@@ -14,10 +14,13 @@ from bookleaf_f90_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_EXE = os.path.join(HERE, "_ref", "bookleaf_ref")
+# the same program with the hot-path driver modules replaced by the drop-in
+# modules of bookleaf_f90_amd/fortran/dropin (runs the step on the GPU)
+DROPIN_EXE = os.path.join(HERE, "_ref", "bookleaf_ref_dropin")
 
 
-def available():
-    return os.path.exists(REF_EXE)
+def available(exe=REF_EXE):
+    return os.path.exists(exe)
 
 
 def build():
@@ -108,15 +111,16 @@ def read_dumps(path):
     return dumps, steps
 
 
-def run_reference(p, mode=0, nsteps=None, dump_every=0, timeout=3600):
-    """Run oracle/_ref/bookleaf_ref on problem p; returns (dumps, steps, stdout)."""
-    if not available():
-        raise RuntimeError("oracle/_ref/bookleaf_ref not built")
+def run_reference(p, mode=0, nsteps=None, dump_every=0, timeout=3600, exe=REF_EXE):
+    """Run oracle/_ref/bookleaf_ref (or exe) on problem p; returns (dumps,
+    steps, stdout)."""
+    if not available(exe):
+        raise RuntimeError(exe + " not built")
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "p.bin"), os.path.join(d, "dump.bin")
         write_problem(p, fin, nsteps=nsteps, dump_every=dump_every)
         env = dict(os.environ)
-        r = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec \"$0\" \"$@\"", REF_EXE,
+        r = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec \"$0\" \"$@\"", exe,
                             fin, fout, str(mode)], capture_output=True, text=True,
                            timeout=timeout, env=env)
         if r.returncode != 0:

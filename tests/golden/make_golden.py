@@ -14,7 +14,7 @@ compressed .npz holding
 Digests are taken over a canonical form (-0.0 -> +0.0, one NaN pattern), the
 same IEEE equality the parity tests use.
 
-usage: python tests/golden/make_golden.py   (needs oracle/_ref built)
+usage: python tests/golden/make_golden.py [name ...]   (needs oracle/_ref built)
 """
 import hashlib
 import json
@@ -28,12 +28,17 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from bookleaf_f90_amd import abi  # noqa: E402
+from bookleaf_f90_amd import problem as P  # noqa: E402
 from tests.helpers import small_problems  # noqa: E402
 
 # problem name -> steps of the multi-step record
 CASES = {"sod_lag": 40, "sod_eul": 40, "sedov_hg": 40, "sod_sp": 30, "sod_midlength": 30,
          "sod2mat_eul": 30, "sedov_jitter_eul": 30, "sod_cfl": 150, "noh_div": 80,
          "sod2mat_mixed_eul": 30}
+# the non-ideal EOS forms (Tait, JWL): not in small_problems(), whose GPU
+# tests are bit-exact, since the device pow and exp are not glibc's
+EXTRA = {"tait_jwl": (lambda: P.tait_jwl(nsteps=100), 100),
+         "tait_jwl_eul": (lambda: P.tait_jwl(eulerian=True, nsteps=100), 100)}
 FULL = ("eldensity", "elenergy", "elpressure", "ndx", "ndy", "ndu", "ndv")
 INPUTS = ("ielnd", "ndx", "ndy", "indtype", "ielreg", "ielmat", "density", "energy", "ndu",
           "ndv", "eos_itype", "eos_param", "kappareg", "pmeritreg", "zdtnotreg", "zmidlength")
@@ -67,8 +72,12 @@ def main():
     if not refio.available():
         sys.exit("oracle/_ref/bookleaf_ref is not built (python oracle/build_ref.py)")
     probs = small_problems()
-    for name, nsteps in CASES.items():
-        p = probs[name]
+    cases = [(name, probs[name], nsteps) for name, nsteps in CASES.items()]
+    cases += [(name, make(), nsteps) for name, (make, nsteps) in EXTRA.items()]
+    want = sys.argv[1:]
+    for name, p, nsteps in cases:
+        if want and name not in want:
+            continue
         gran, _, _ = refio.run_reference(p, mode=1)
         dumps, steps, _ = refio.run_reference(p, mode=0, nsteps=nsteps)
         last = dumps[-1]

@@ -19,9 +19,10 @@ MIXED = ("imxel", "imxfcp", "imxncp", "icpmat", "icpnext", "icpprev", "frvolume"
 
 def names():
     """The step records (make_golden.py); meshgen_*.npz are the mesh
-    generator's (make_meshgen.py)."""
+    generator's (make_meshgen.py), full_*.npz the full-size runs
+    (make_fullsize.py)."""
     return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(f).startswith("meshgen_"))
+                  if not os.path.basename(f).startswith(("meshgen_", "full_")))
 
 
 def load_meshgen(name):

@@ -96,3 +96,57 @@ def test_ensemble_argument_errors():
     cfg = dict(P.ENSEMBLE_CONFIG, zexist=1)
     with pytest.raises(BookLeafError):
         Ensemble(_small(), config=cfg)
+
+
+def _dt_sequences(params, nsteps):
+    from oracle.pyoracle import Oracle
+    return np.array([Oracle(P.ensemble_instance(q, config=CFG)).run(nsteps)[0] for q in params])
+
+
+@pytest.mark.parametrize("fail_step", [3, 8])
+def test_ensemble_dt_min_failure_is_deterministic(fail_step, oracle_lib):
+    """An instance whose dt falls below dt_min (time_dr_advance.f90:122-127)
+    stops the whole ensemble at that step, whatever order the per-instance
+    selections run in: every instance's dt of the failing step is selected
+    (its nstep and time advance, as time_dr_end advances them before its
+    check), no instance's state advances, and the error names the smallest
+    failing instance.  dt_min is placed, from the oracle's dt sequences,
+    below every dt of the steps before fail_step and above at least one of
+    that step's (two instances at step 3, one at step 8).  Two runs give the
+    same bits, and each instance holds the oracle's state after
+    fail_step - 1 steps of it alone (the viscosity fields aside: the failing
+    step's limiter pass has written those of its moving elements)."""
+    from bookleaf_f90_amd.device import BookLeafError, Ensemble
+    from oracle.pyoracle import Oracle
+    params = _small(seed=11, n=6)
+    D = _dt_sequences(params, fail_step)
+    s = fail_step - 1
+    dt_min = D[:, :s].min()
+    failing = [k for k in range(len(params)) if D[k, s] < dt_min]
+    assert failing, "no instance falls below dt_min at this step"
+    cfg = dict(CFG, dt_min=dt_min)
+    fields = [f for f in STATE_FIELDS if f not in ("cnviscx", "cnviscy", "elvisc")]
+    runs = []
+    for rep in range(2):
+        print("ensemble dt_min failure at step", fail_step, "rep", rep, flush=True)
+        ens = Ensemble(params, config=cfg)
+        with pytest.raises(BookLeafError) as ei:
+            ens.run(20)
+        assert "dt < dt_min (instance %d)" % failing[0] in str(ei.value)
+        dt, idt, ctl = ens.last_history
+        times, last_dt, nstep = ens.times()
+        runs.append((dt, idt, ctl, times, nstep, {f: ens.get(f) for f in fields}))
+    for a, b in zip(runs[0][:5], runs[1][:5]):
+        assert np.array_equal(a, b)
+    assert diff_fields(runs[0][5], runs[1][5]) == []
+    dt, idt, ctl, times, nstep, state = runs[0]
+    assert dt.shape[0] == fail_step
+    assert np.all(nstep == fail_step)
+    assert np.array_equal(dt[s], D[:, s])
+    for k, q in enumerate(params):
+        o = Oracle(P.ensemble_instance(q, config=CFG))
+        odt, _, _ = o.run(s)
+        assert np.array_equal(dt[:s, k], odt), k
+        assert times[k] == o.ts.time + D[k, s], k
+        mine = {f: ens.instance_field(f, k, state[f]) for f in fields}
+        assert diff_fields(mine, o.state(), [f for f in fields if f not in LAG_SCRATCH]) == [], k
