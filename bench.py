@@ -21,10 +21,19 @@ Reported next to the throughput:
   cpu_baseline  the reference's own Fortran (oracle/_ref, flang -O2, serial)
                 timed on this host, one core, on the first --cpu-steps steps of
                 the same workload ("port": the C oracle when _ref is absent)
+  cpu_baseline_throughput  the same on every host core at once: --cpu-procs
+                independent copies of that sample, their element-steps/s summed
+  profile_pass.at_rest_fraction  the share of elements whose four nodes share
+                one velocity (they skip the limiter pass) over the profile pass
   driver_boundary_GBps  the throughput expressed in the per-driver
                 algorithmic bytes of the reference's call graph
                 (1664 B/element + 272 B/node per Lagrangian step, +84 B/element
                 with the anti-hourglass force)
+
+--workload tgv runs a Taylor-Green vortex on the same 1024x1024 size with
+every element in motion from the first step (no element skips the limiter),
+beside the Sedov headline where the blast has reached a small share of the
+mesh by step 500.
 
 One GPU only: --gpus N with N > 1 exits.
 """
@@ -89,7 +98,15 @@ KERNEL_BYTES = {
     "mlp_nodes": (0, 80),
     # the check: ielnd 16 | x, y, mesh velocity 32
     "learned_check": (16, 32),
+    # the limiter pass: only the elements on F1's worklist (LIMITER_BYTES)
+    "f1_limiter": (0, 0),
 }
+# the limiter pass, per listed element: worklist entry 4, ielnd 16, ielel
+# 16, outer-node table 32, own rho, cs2 16; writes cnviscx/y 64, elvisc 8
+# (the neighbours' rho, cs2 and the far nodes come through the cache); and
+# the x, y, u, v of the nodes it touches, 32 B/node in the proportion of
+# elements listed
+LIMITER_BYTES = (156, 32)
 # data-dependent parts, from the run's own counters (bl_run_stats): F1 and
 # E1 write cnviscx/y and elvisc (72 B) for an element at rest only when it
 # moved in the step before (or on a call's first step); F2 reads cnviscx/y
@@ -106,6 +123,8 @@ FOLD_BYTES = {"f4_state": (0.125, 16)}
 def kernel_bytes(k, nel, nnd, moving, zeroed, fold=False):
     """algorithmic bytes of one launch of kernel k; moving and zeroed are the
     per-step averages of bl_run_stats"""
+    if k == "f1_limiter":
+        return LIMITER_BYTES[0] * moving + LIMITER_BYTES[1] * nnd * min(1.0, moving / nel)
     be, bn = KERNEL_BYTES[k]
     if fold:
         fe, fn = FOLD_BYTES.get(k, (0, 0))
@@ -130,9 +149,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--runner", choices=["run", "run_graph"], default="run_graph")
     ap.add_argument("--n", type=int, default=1024, help="mesh is n x n (sedov)")
-    ap.add_argument("--workload", choices=["sedov", "sod2mat", "relax", "ensemble", "learned"],
+    ap.add_argument("--workload", choices=["sedov", "tgv", "sod2mat", "relax", "ensemble",
+                                           "learned"],
                     default="sedov",
-                    help="sedov: Lagrangian 1024^2 Sedov (the headline); sod2mat: Eulerian "
+                    help="sedov: Lagrangian 1024^2 Sedov (the headline); tgv: Lagrangian "
+                         "1024^2 Taylor-Green vortex, every element in motion from the first "
+                         "step; sod2mat: Eulerian "
                          "(ALE remap every step) two-material Sod on 2048x256 with mixed "
                          "interface cells; relax: two-material Sod 1024x512 with ALE node "
                          "relaxation; ensemble: 512 Sod/Sedov instances of 64^2..512^2 "
@@ -148,13 +170,16 @@ def parse():
                     help="keep the default launch shape of the limiter pass instead of "
                          "timing the candidates (bl_autotune) before the last warm-up step")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="processes of the throughput CPU baseline (one per host core, at "
+                         "most the host's core count); 0 skips it")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="launch the plain nel-double copy kernel (hydro_setpredictor) this "
                          "many times, to scale profiler byte counters")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"sedov": 490, "ensemble": 190}.get(args.workload, 290)
+        args.steps = {"sedov": 490, "tgv": 490, "ensemble": 190}.get(args.workload, 290)
     return args
 
 
@@ -184,6 +209,37 @@ def cpu_baseline(prob, steps):
     return {"value": prob.nel * steps / sec, "unit": "element-steps/s", "cores": 1,
             "kind": "port", "sample": "C oracle, steps 1-%d of the same run (%.2f s)"
             % (steps, sec)}
+
+
+def cpu_baseline_throughput(prob, steps, nproc):
+    """The reference's Fortran (oracle/_ref, serial) as nproc independent
+    processes at once on nproc host cores, each running the same bounded
+    sample of the workload: the sum of their own element-steps/s (the host's
+    throughput on independent copies of the problem)."""
+    from oracle import refio
+    nproc = min(nproc, os.cpu_count() or 1)
+    if nproc < 1 or not refio.available() or prob.relax:
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "p.bin")
+        refio.write_problem(prob, fin, nsteps=steps)
+        procs = [subprocess.Popen(["bash", "-c", 'ulimit -s unlimited; exec "$0" "$@"',
+                                   refio.REF_EXE, fin, os.path.join(d, "o%d.bin" % k), "2"],
+                                  stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+                 for k in range(nproc)]
+        rates = []
+        for pr in procs:
+            out, _ = pr.communicate(timeout=900)
+            m = re.search(r"elsteps_per_s=\s*(\S+)", out)
+            if pr.returncode != 0 or not m:
+                return None
+            rates.append(float(m.group(1)))
+    return {"value": sum(rates), "unit": "element-steps/s", "cores": nproc, "kind": "reference",
+            "sample": "oracle/_ref (the reference's Fortran, flang -O2 -ffp-contract=off, "
+                      "serial), %d processes at once on %d host cores, each on steps 1-%d of "
+                      "the same %s run; the sum of their own element-steps/s (slowest %.3g, "
+                      "fastest %.3g)" % (nproc, nproc, steps, prob.name, min(rates),
+                                          max(rates))}
 
 
 def cpu_baseline_ensemble(params, steps):
@@ -236,6 +292,10 @@ def main():
     elif args.workload == "sedov":
         prob = P.sedov(n=args.n, nsteps=args.warmup + args.steps, kappa=KAPPA)
         wname = "sedov_lagrangian_%dx%d_jitter0.1_seed12345_kappa0.01_%dsteps" % (
+            args.n, args.n, args.warmup + args.steps)
+    elif args.workload == "tgv":
+        prob = P.taylor_green(n=args.n, nsteps=args.warmup + args.steps)
+        wname = "taylor_green_lagrangian_%dx%d_u1_p10_%dsteps" % (
             args.n, args.n, args.warmup + args.steps)
     elif args.workload == "sod2mat":
         prob = P.sod2mat_mixed(nx=2048, ny=256, nsteps=args.warmup + args.steps)
@@ -334,13 +394,16 @@ def main():
     for _ in range(args.calibrate_copy):
         g.hydro_setpredictor()  # k_copy_r: reads and writes 8*nel bytes
     g.synchronize()
+    cpu_tp = None
     if args.no_cpu_baseline:
         cpu = None
     elif prob is None:
         cpu = cpu_baseline_ensemble(params, min(args.cpu_steps, 10))
     else:
         cpu = cpu_baseline(prob, args.cpu_steps)
-    dbe, dbn = (DRIVER_BOUNDARY if args.workload in ("sedov", "ensemble")
+        if args.cpu_procs > 0:
+            cpu_tp = cpu_baseline_throughput(prob, args.cpu_steps, args.cpu_procs)
+    dbe, dbn = (DRIVER_BOUNDARY if args.workload in ("sedov", "tgv", "ensemble")
                 else DRIVER_BOUNDARY_EUL)
     step_bytes = dbe * nel + dbn * nnd
     # algorithmic bytes of the kernels one step launches (from the profile pass)
@@ -367,6 +430,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        # the same reference on every host core at once (independent copies)
+        "cpu_baseline_throughput": cpu_tp,
+        "speedup_vs_cpu_throughput": (value / cpu_tp["value"]) if cpu_tp else None,
         "step_hbm_frac": fused_bytes / (sec / args.steps) / HBM_PEAK,
         "driver_boundary_GBps": step_bytes / (sec / args.steps) / 1e9,
         "driver_boundary_hbm_frac": step_bytes / (sec / args.steps) / HBM_PEAK,
@@ -374,8 +440,10 @@ def main():
         # elements in motion and zero viscosity writes per step (bl_run_stats),
         # over the profile pass and over every run call of this command
         "profile_pass": {"steps": args.profile_steps, "moving_per_step": moving,
+                         "at_rest_fraction": 1.0 - moving / nel,
                          "zeroed_per_step": zeroed, "f1_folded_into_f4": fold},
         "all_calls": {"steps": counted[0], "moving_per_step": counted[1] / max(counted[0], 1),
+                      "at_rest_fraction": 1.0 - counted[1] / max(counted[0], 1) / nel,
                       "zeroed_per_step": counted[2] / max(counted[0], 1)},
     }
     if args.workload == "learned":

@@ -15,9 +15,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbookleaf_hip.so")
 SOURCES = ["bl_api.hip", "bl_ale.hip", "bl_step.hip", "bl_fused.hip", "bl_mix.hip", "bl_diag.hip",
-           "bl_relax.hip", "bl_ensemble.hip", "bl_mlp.hip"]
+           "bl_relax.hip", "bl_ensemble.hip", "bl_mlp.hip", "bl_tiles.hip"]
 HEADERS = ["bl_internal.h", "bl_math.h", "bl_kernels.h", "bl_step.h", "bl_fused.h", "bl_mix.h", "bl_ale.h",
-           "bl_relax.h"]
+           "bl_relax.h", "bl_tiles.h"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "bl_internal.h"
+#include "bl_tiles.h"
 #include "bl_kernels.h"
 #include "bl_mix.h"
 #include "bl_math.h"
@@ -603,13 +604,20 @@ int bl_destroy(bl_handle_t* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   invalidate_graph(h);
   const bool sets = h->nodeset[1][0] != nullptr;
+  const bool elsets = h->elset[0][0] != nullptr;
   for (int id = 1; id <= BL_NFIELD; ++id) {
     bool node = id == BL_NDX || id == BL_NDY || id == BL_NDU || id == BL_NDV;
-    if (h->f[id] && !(sets && node)) hipFree(h->f[id]);
+    bool state = id == BL_ELDENSITY || id == BL_ELENERGY || id == BL_ELPRESSURE ||
+                 id == BL_ELCS2;
+    if (h->f[id] && !(sets && node) && !(elsets && state)) hipFree(h->f[id]);
   }
   if (sets)
     for (int s = 0; s < 2; ++s)
       for (int k = 0; k < 4; ++k) hipFree(h->nodeset[s][k]);
+  if (elsets)
+    for (int s = 0; s < 2; ++s)
+      for (int k = 0; k < 4; ++k) hipFree(h->elset[s][k]);
+  free_tiles(h);
   void* bufs[] = {h->d_kappareg, h->d_pmeritreg, h->d_zdtnotreg, h->d_zmidlength,
                   h->d_eos_itype, h->d_eos_param, h->d_hyd_ptr, h->d_hyd_idx,
                   h->d_ale_ptr, h->d_ale_idx, h->d_el_ptr, h->d_el_idx, h->d_part,
@@ -828,6 +836,7 @@ int bl_build_maps(bl_handle_t* h) {
   if (build_outer(h, ielnd)) return BL_FAILURE;
   if (h->relax_type != BL_RELAX_OFF && relax_rebuild(h)) return BL_FAILURE;
   h->nbfc_stale = true;
+  h->tile_state = 0;  // the tiling follows the connectivity
   h->maps_built = true;
   invalidate_graph(h);
   return BL_SUCCESS;

@@ -45,11 +45,17 @@ struct FusedArgs {
   const double* pmeritreg;
   const int32_t* eos_itype;
   const double* eos_param;
-  // element fields
+  // element fields (rho, en, pre, cs2 as the step starts; the corrector
+  // writes rho_o, en_o, pre_o, cs2_o: the same arrays, or the other element
+  // set in the tile step)
   double* rho;
   double* en;
   double* pre;
   double* cs2;
+  double* rho_o;
+  double* en_o;
+  double* pre_o;
+  double* cs2_o;
   double* vol;
   const double* mass;
   const double* cnmass;
@@ -80,8 +86,9 @@ struct FusedArgs {
   // reductions and device timestep
   double* part;
   int32_t* parti;
-  int32_t* work;          // F1's worklist of elements whose nodes move unequally
-  uint32_t* work_count;   // its length (reset by F1b)
+  // an ensemble's worklist of elements whose nodes move unequally, one
+  // segment per instance (a single problem's F1L reads the at-rest mask)
+  int32_t* work;
   // bit e of rest[e / 64]: element e is at rest this step and its cnviscx/y
   // and elvisc hold the +0.0 F1 wrote (F1 sets it; F2 reads it)
   uint64_t* rest;
@@ -91,6 +98,10 @@ struct FusedArgs {
   // the zeros of elements at rest now (rest) but not in the step before
   // (rest_prev).  Without the fold, F1 writes them and rest is its only mask.
   int fold;
+  // no F5: the next step's F1b records this step's end (step_end), and the
+  // host the last step's (a pure-cell Lagrangian batch without per-step
+  // diagnostics)
+  int end_in_f1b;
   const uint64_t* rest_prev;
   uint64_t* rest_next;
   DtResult* dtres;
@@ -98,6 +109,23 @@ struct FusedArgs {
   double* hist;
   int hist_cap;
   MixArgs mx;  // mixed cells (mx.ncp == 0: none)
+  // the tile step (bl_tiles.h): tiles, their element entries (element |
+  // flags << 30, four 8-bit local node slots), the boundary list and its
+  // corner forces, and the at-rest bytes of the step (rest8: now, rest8_prev:
+  // the step before, rest8_next: written for the next step); worklist
+  // segments and partial slots are the ntile tiles then the boundary blocks
+  int ntile, nbnd, nseg;
+  const int32_t* tile_eptr;
+  const int32_t* tile_nptr;
+  const int32_t* tile_nodes;
+  const int2* tile_el;
+  const int4* tile_nd;  // the entry's four nodes (0-based), beside tile_el
+  const int32_t* bnd_el;
+  const int32_t* el_bslot;
+  double* bnd_f;
+  const uint8_t* rest8;
+  const uint8_t* rest8_prev;
+  uint8_t* rest8_next;
   // ensembles (null otherwise): the instance of each element and node, and
   // each instance's dt, which F2-F4 use in place of the timestep's
   const uint16_t* einst;

@@ -45,6 +45,7 @@
 #include "bl_internal.h"
 #include "bl_kernels.h"
 #include "bl_math.h"
+#include "bl_tiles.h"
 
 namespace bl {
 
@@ -161,12 +162,6 @@ __device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double
   return dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
 }
 
-// every element on the worklist (the autotuner's second trial)
-__global__ void __launch_bounds__(kBlock) k_work_all(int32_t* work, int n) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) work[i] = i;
-}
-
 // the wave's word of the at-rest mask, and the count of elements whose
 // zeros F1 wrote (one atomic per wave that wrote any)
 __device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane, uint64_t prev,
@@ -203,20 +198,11 @@ __device__ __forceinline__ void f1_terms(const FusedArgs& A, int e, const double
   divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
 }
 
-// the worklist append (one atomic per wave) and the workgroup's DIV and
-// CFL partials, at part[blockIdx.x] and part[2 * gridDim.x + blockIdx.x];
-// every thread of the workgroup takes part
+// the workgroup's DIV and CFL partials, at part[blockIdx.x] and
+// part[2 * gridDim.x + blockIdx.x] (the elements in motion are the at-rest
+// mask's zeros, which F1L reads); every thread of the workgroup takes part
 __device__ __forceinline__ void f1_collect(const FusedArgs& A, int e, int lane, bool moving,
                                            double cflv, double divv) {
-  const uint64_t mask = __ballot(moving);
-  if (mask) {
-    const int lead = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == lead) base = atomicAdd(A.work_count, (uint32_t)__popcll(mask));
-    base = __shfl(base, lead, 64);
-    const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (moving && slot < (uint32_t)A.nel) A.work[slot] = e;
-  }
   const int nb = gridDim.x;
   block_argmax_first(divv, e + 1, A.part, A.parti);
   __syncthreads();
@@ -254,6 +240,13 @@ k_f1_getdt(FusedArgs A, FusedNodes N) {
   f1_collect(A, e, lane, moving, cflv, divv);
 }
 
+// every element on an ensemble's worklist (the autotuner's second trial):
+// element i at place i, which lies in its instance's segment
+__global__ void __launch_bounds__(kBlock) k_work_all(int32_t* work, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) work[i] = i;
+}
+
 // viscosity, elvisc and CFL term of a worklist element (a NaN CFL term
 // becomes DBL_MAX: it never wins the strict '<' scan)
 __device__ __forceinline__ double f1_moving(const FusedArgs& A, const FusedNodes& N, int e) {
@@ -288,15 +281,44 @@ __device__ __forceinline__ void select_dt(const FusedArgs& A, double dt_prev, in
   if (c < best) { best = c; bidx = div_idx; bsdt = BL_DT_DIV; }
 }
 
-// F1L: the worklist's elements, a fixed grid striding over the count F1
-// left.  Each workgroup also folds a slice of F1's nb1 partials of each
+// The end of a step that ran (k_f5_end's work, when no F5 follows F4): a
+// negative volume of the predictor (F2: the nodes were not moved) or of the
+// corrector (F4: they were) stops the batch at that step, as
+// geometry_kn_checkvolume's error halts the reference; a step that moved
+// the nodes wrote the other node set.  Host and device form of the same
+// record (the host runs it for a batch's last step).
+__host__ __device__ inline void step_end(DevTimestep* ts, const DtResult* res) {
+  if (!ts->open) return;
+  ts->open = 0;
+  if (res->negvol) {
+    ts->err = 1; ts->err_idx = -res->negvol; ts->done = 1; ts->skip = 1;
+    ts->err_step = ts->nstep;
+    return;
+  }
+  ts->flips = ts->flips + 1;
+  if (res->negvol2) {
+    ts->err = 1; ts->err_idx = -res->negvol2; ts->done = 1; ts->skip = 1;
+    ts->err_step = ts->nstep;
+  }
+}
+
+// F1L: the elements in motion, found from the step's at-rest mask (bits
+// in A.rest, or the tile step's bytes in A.rest8) over this workgroup's run
+// of element blocks (its XCD's part of the mesh, as xcd_block deals them),
+// compacted chunk by chunk in LDS so that every lane of a pass has an
+// element, in element order.  Each workgroup also folds a slice of the nb1
+// partials of F1 (or of the step kernels that did F1's work) of each
 // control into its own, so that F1b reduces one (CFL, DIV) pair per F1L
 // workgroup: CFL at part + 3 * nb1, DIV at part + nb1 (first-index extrema
 // are associative, so the result is F1b's over all partials)
 __global__ void __launch_bounds__(kBlock)
 k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
   if (step_skipped(A, A.ts)) return;
-  const uint32_t count = min(*A.work_count, (uint32_t)A.nel);
+  // the step before found a negative volume that F1b has not recorded yet:
+  // this step does nothing (F1b makes it so)
+  if (A.end_in_f1b && (A.dtres->negvol || A.dtres->negvol2)) return;
+  __shared__ int32_t list[kBlock];
+  __shared__ uint32_t wave_n[kBlock / 64];
   double best = DBL_MAX, dv = -1.0;
   int bi = 0x7fffffff, di = 0x7fffffff;
   const int per = (nb1 + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -305,11 +327,34 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
     better_max(dv, di, A.part[k], A.parti[k]);
     better_min(best, bi, A.part[2 * nb1 + k], A.parti[2 * nb1 + k]);
   }
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
-       k += gridDim.x * blockDim.x) {
-    const int e = A.work[k];
-    better_min(best, bi, f1_moving(A, N, e), e + 1);
+  const int nbe = (A.nel + kBlock - 1) / kBlock;
+  const int lb = xcd_block();
+  const int e0 = (int)(((long long)lb * nbe) / gridDim.x) * kBlock;
+  const int e1 = min(A.nel, (int)(((long long)(lb + 1) * nbe) / gridDim.x) * kBlock);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t listed = 0;
+  for (int c = e0; c < e1; c += blockDim.x) {
+    const int e = c + threadIdx.x;
+    bool mv = false;
+    if (e < e1) mv = A.rest8 ? A.rest8[e] == 0 : !((A.rest[e >> 6] >> (e & 63)) & 1ull);
+    const uint64_t mask = __ballot(mv);
+    if (lane == 0) wave_n[wid] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (int w = 0; w < nw; ++w) {
+      base += w < wid ? wave_n[w] : 0u;
+      total += wave_n[w];
+    }
+    if (mv) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = e;
+    __syncthreads();
+    if (threadIdx.x < total) {
+      const int el = list[threadIdx.x];
+      better_min(best, bi, f1_moving(A, N, el), el + 1);
+    }
+    listed += total;
+    __syncthreads();  // the list is refilled
   }
+  if (threadIdx.x == 0 && listed) atomicAdd(A.vstats, (unsigned long long)listed);
   block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
   __syncthreads();
   block_reduce_store<false>(dv, di, A.part + nb1, A.parti + nb1);
@@ -322,6 +367,10 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
 // far more than the launch with an L2 per XCD.)
 __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb, int nbl) {
   DevTimestep* ts = A.ts;
+  if (A.end_in_f1b) {
+    if (threadIdx.x == 0) step_end(ts, A.dtres);
+    __syncthreads();
+  }
   const bool skip = step_skipped(A, ts);
   __syncthreads();  // every thread has read ts before thread 0 writes it
   if (threadIdx.x == 0) {
@@ -329,8 +378,7 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
     ts->skip = skip ? 1 : 0;
     A.dtres->negvol = 0;
     A.dtres->negvol2 = 0;
-    if (!skip) A.vstats[0] += min(*A.work_count, (uint32_t)A.nel);
-    *A.work_count = 0;  // F1L has read it
+    if (A.end_in_f1b && !skip) ts->open = 1;
   }
   if (skip) return;
   final_dt_reduce(nbl, A.part + nb, A.parti + nb, nbl, A.part + 3 * nb, A.parti + 3 * nb,
@@ -351,6 +399,8 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
   ts->sdt = bsdt;
   ts->time = ts->time + best;
   ts->dts = best;
+  // the batch's last step (the one after it would be skipped)
+  ts->last = (ts->nstep == ts->last_step) || (A.stop_at_time_end && ts->time >= A.time_end);
   ts->ale_on = A.zexist && (ts->time >= A.mintime) && (ts->time <= A.maxtime);
   if (A.hist && ts->hist_n < A.hist_cap) {
     int k = ts->hist_n++;
@@ -656,31 +706,21 @@ __device__ __forceinline__ void f4_mix_cell(const FusedArgs& A, const FusedNodes
 }
 
 // ---------------------------------------------------------------- F2
-__global__ void __launch_bounds__(kBlock)
-k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
-  const DevTimestep* ts = A.ts;
-  if (ts->skip || ens_pending(A, ts)) return;
-  // the mixed cells' workgroups come first, so that their longer serial
-  // work starts with the grid rather than trailing it; nbm is a multiple of
-  // the XCD count, so the element workgroups keep their XCD mapping
-  if ((int)blockIdx.x < nbm) {
-    const int imx = (int)blockIdx.x * kBlock + threadIdx.x;
-    if (imx < A.mx.nmx) f2_mix_cell(A, N, ts, imx);
-    return;
-  }
-  const int e = xcd_block_of((int)blockIdx.x - nbm, (int)gridDim.x - nbm) * kBlock + threadIdx.x;
-  if (e >= A.nel) return;
-  const int inst = A.einst ? (int)A.einst[e] : -1;
-  const double dt = inst >= 0 ? A.inst_dt[inst] : ts->dt;
+// F2's work for element e: the predictor at x^n (force with p^n, half-step
+// geometry, density, energy and EOS in registers) and the corrector's corner
+// forces at the half step P+Q(+SP)(+HG) in fx, fy, with the half-step
+// corner weights wt and each corner's nodal-mass term mt as
+// hydro_kn_scatteracceleration picks it (hydro_kn_getacceleration.f90:58-67).
+// at_rest: the element's viscosity forces are +0.0 and are not read.  With
+// check_volume the element reports a negative half-step volume.  (A mixed
+// cell's corner force comes from its components, F2m.)
+__device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes& N, int e,
+                                           const int nd[4], double dt, double rdt,
+                                           bool at_rest, bool check_volume, double fx[4],
+                                           double fy[4], double wt[4], double mt[4]) {
   const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
-  int nd[4];
-  load_nd(A.ielnd, e, nd);
   double x[4], y[4], u[4], v[4];
   load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
-  // an element F1 found at rest has +0.0 viscosity forces: not read.  With
-  // F1 folded into F4, F2 writes those zeros when the element moved in the
-  // step before (or this is the first step of the run call)
-  const bool at_rest = (A.rest[e >> 6] >> (e & 63)) & 1ull;
   double qx[4], qy[4];
   if (at_rest) {
 #pragma unroll
@@ -694,7 +734,6 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   // --- predictor: hydro_dr_getforce at x^n with p^n
   Iso g;
   getiso(x, y, g);
-  double fx[4], fy[4];
   forceP(A.pre[e], g.a1, g.a3, g.b1, g.b3, fx, fy);
 #pragma unroll
   for (int c = 0; c < 4; ++c) { fx[c] = fx[c] + qx[c]; fy[c] = fy[c] + qy[c]; }
@@ -711,7 +750,7 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   for (int c = 0; c < 4; ++c) { xh[c] = x[c] + h * u[c]; yh[c] = y[c] + h * v[c]; }
   Iso gh;
   getiso(xh, yh, gh);
-  if (gh.vol < 0.0) atomicMax(&A.dtres->negvol, e + 1);
+  if (check_volume && gh.vol < 0.0) atomicMax(&A.dtres->negvol, e + 1);
   const double m = A.mass[e];
   const double rhoh = m / gh.vol;
   const double eh = energy_update(A.en[e], m, fx, fy, u, v, h, A.zerocut);
@@ -730,39 +769,70 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
     const double cs2h = eos_cs2(et, ep, rhoh, eh, A.pcut, A.ccut);
     forceSP(A.pmeritreg[ireg], rhoh, cs2h, xh, yh, spm, fx, fy);
   }
-  if (A.zhg)
-    forceHG(A.kappareg[ireg], rhoh, gh.vol, inst >= 0 ? A.inst_rdt[inst] : ts->rdt, u, v, fx,
-            fy);
-  // nodal-mass term of each corner as hydro_kn_scatteracceleration picks it
-  // (hydro_kn_getacceleration.f90:58-67): cnmass, else the previous corner's
+  if (A.zhg) forceHG(A.kappareg[ireg], rhoh, gh.vol, rdt, u, v, fx, fy);
+  // nodal-mass term of each corner: cnmass, else the previous corner's
   // cnmass, else rho*cnwt at the half step
   double cm[4];
   ld4(A.cnmass, e, cm);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (cm[j] > A.zerocut) mt[j] = cm[j];
+    else if (cm[(j + 3) & 3] > A.zerocut) mt[j] = cm[(j + 3) & 3];
+    else mt[j] = rhoh * gh.wt[j];
+    wt[j] = gh.wt[j];
+  }
+}
+
+// the zero viscosity of an element at rest now but not in the step before
+// (or on a run call's first step), written after the element's loads
+__device__ __forceinline__ void write_rest_zeros(const FusedArgs& A, int e) {
+  const double z[4] = {0.0, 0.0, 0.0, 0.0};
+  st4(A.cnviscx, e, z);
+  st4(A.cnviscy, e, z);
+  A.elvisc[e] = 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || ens_pending(A, ts)) return;
+  // the mixed cells' workgroups come first, so that their longer serial
+  // work starts with the grid rather than trailing it; nbm is a multiple of
+  // the XCD count, so the element workgroups keep their XCD mapping
+  if ((int)blockIdx.x < nbm) {
+    const int imx = (int)blockIdx.x * kBlock + threadIdx.x;
+    if (imx < A.mx.nmx) f2_mix_cell(A, N, ts, imx);
+    return;
+  }
+  const int e = xcd_block_of((int)blockIdx.x - nbm, (int)gridDim.x - nbm) * kBlock + threadIdx.x;
+  if (e >= A.nel) return;
+  const int inst = A.einst ? (int)A.einst[e] : -1;
+  const double dt = inst >= 0 ? A.inst_dt[inst] : ts->dt;
+  const double rdt = inst >= 0 ? A.inst_rdt[inst] : ts->rdt;
+  // an element F1 found at rest has +0.0 viscosity forces: not read.  With
+  // F1 folded into F4, F2 writes those zeros when the element moved in the
+  // step before (or this is the first step of the run call)
+  const bool at_rest = (A.rest[e >> 6] >> (e & 63)) & 1ull;
+  int nd[4];
+  load_nd(A.ielnd, e, nd);
+  double fx[4], fy[4], wt[4], mt[4];
+  f2_element(A, N, e, nd, dt, rdt, at_rest, true, fx, fy, wt, mt);
   const int nel = A.nel;
   // a mixed cell's force comes from its components (F2m)
   const bool own_force = !(A.mx.ncp > 0 && A.ielmat[e] < 0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    double mt;
-    if (cm[j] > A.zerocut) mt = cm[j];
-    else if (cm[(j + 3) & 3] > A.zerocut) mt = cm[(j + 3) & 3];
-    else mt = rhoh * gh.wt[j];
     if (own_force) {
       A.fxT[j * nel + e] = fx[j];
       A.fyT[j * nel + e] = fy[j];
     }
-    A.wtT[j * nel + e] = gh.wt[j];
-    A.mT[j * nel + e] = mt;
+    A.wtT[j * nel + e] = wt[j];
+    A.mT[j * nel + e] = mt[j];
   }
   if (A.fold) {
     // the zeros (last, so that no load of this kernel waits behind them)
     const bool zeroed = at_rest && !((A.rest_prev[e >> 6] >> (e & 63)) & 1ull);
-    if (zeroed) {
-      const double z[4] = {0.0, 0.0, 0.0, 0.0};
-      st4(A.cnviscx, e, z);
-      st4(A.cnviscy, e, z);
-      A.elvisc[e] = 0.0;
-    }
+    if (zeroed) write_rest_zeros(A, e);
     // lanes past nel left at the start, and they are the wave's last lanes
     const uint64_t wrote = __ballot(zeroed);
     if ((threadIdx.x & 63) == 0 && wrote)
@@ -771,6 +841,34 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
 }
 
 // ---------------------------------------------------------------- F3
+// F3's work for a node from its sums over its corners (mass m, area a,
+// force fu, fv in the (corner, element) order of the reference's scatter),
+// its position and velocity at step start and its type t:
+// hydro_kn_getacceleration, utils_kn_bc, hydro_kn_applyacceleration and the
+// corrector's geometry_kn_getvertex.  out = x^{n+1}, y^{n+1}, u^{n+1},
+// v^{n+1}, ubar, vbar.
+__device__ __forceinline__ void f3_node(const FusedArgs& A, double dt, double m, double a,
+                                        double fu, double fv, double x0, double y0, double u0,
+                                        double v0, int t, double out[6]) {
+  const double h = 0.5 * dt;
+  const double w1 = A.dencut * a;
+  if (m > w1) { fu = fu / m; fv = fv / m; }
+  else { fu = 0.0; fv = 0.0; }
+  if (t == -1) fu = 0.0;
+  else if (t == -2) fv = 0.0;
+  else if (t == -3) { fu = 0.0; fv = 0.0; }
+  if (fu * fu + fv * fv < A.accut2) { fu = 0.0; fv = 0.0; }
+  const double u1 = u0 + dt * fu, v1 = v0 + dt * fv;
+  const double dt05 = 0.5 * dt;
+  out[4] = u0 + dt05 * fu;
+  out[5] = v0 + dt05 * fv;
+  out[2] = u1;
+  out[3] = v1;
+  const double xh = x0 + h * u0, yh = y0 + h * v0;
+  out[0] = xh + h * u1;
+  out[1] = yh + h * v1;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   const DevTimestep* ts = A.ts;
@@ -778,7 +876,6 @@ k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   const int n = xcd_index();
   if (n >= A.nnd) return;
   const double dt = A.ninst ? A.inst_dt[A.ninst[n]] : ts->dt;
-  const double h = 0.5 * dt;
   double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
   const int b = A.hyd_ptr[n], end = A.hyd_ptr[n + 1];
   const int nel = A.nel;
@@ -792,30 +889,50 @@ k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
     fu = fu + A.fxT[off];
     fv = fv + A.fyT[off];
   }
-  // hydro_kn_getacceleration + utils_kn_bc
-  const double w1 = A.dencut * a;
-  if (m > w1) { fu = fu / m; fv = fv / m; }
-  else { fu = 0.0; fv = 0.0; }
-  const int t = A.indtype[n];
-  if (t == -1) fu = 0.0;
-  else if (t == -2) fv = 0.0;
-  else if (t == -3) { fu = 0.0; fv = 0.0; }
-  if (fu * fu + fv * fv < A.accut2) { fu = 0.0; fv = 0.0; }
-  // hydro_kn_applyacceleration, then the corrector's geometry_kn_getvertex
-  const double u0 = N.u[n], v0 = N.v[n];
-  const double u1 = u0 + dt * fu, v1 = v0 + dt * fv;
-  const double dt05 = 0.5 * dt;
-  A.ubar[n] = u0 + dt05 * fu;
-  A.vbar[n] = v0 + dt05 * fv;
-  O.u[n] = u1;
-  O.v[n] = v1;
-  const double xh = N.x[n] + h * u0, yh = N.y[n] + h * v0;
-  O.x[n] = xh + h * u1;
-  O.y[n] = yh + h * v1;
+  double out[6];
+  f3_node(A, dt, m, a, fu, fv, N.x[n], N.y[n], N.u[n], N.v[n], A.indtype[n], out);
+  O.x[n] = out[0];
+  O.y[n] = out[1];
+  O.u[n] = out[2];
+  O.v[n] = out[3];
+  A.ubar[n] = out[4];
+  A.vbar[n] = out[5];
 }
 
-// F4's work for element e: geometry, density, energy and EOS at x^{n+1};
-// returns the new cs2 and leaves nd, x, y and the geometry g for F1's terms
+// F4's work for element e from its nodes at x^{n+1} (x, y), their ubar
+// (ub, vb) and its corrector corner forces (fx, fy): geometry g, density,
+// energy with ubar, EOS, written to the output element fields (en_o ...,
+// the same arrays as en ... unless the step writes the other element set);
+// returns the new cs2
+__device__ __forceinline__ double f4_state_of(const FusedArgs& A, int e, double dt,
+                                              const double x[4], const double y[4],
+                                              const double ub[4], const double vb[4],
+                                              const double fx[4], const double fy[4], Iso& g) {
+  getiso(x, y, g);
+  if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
+  const double m = A.mass[e];
+  const double rho = m / g.vol;
+  const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
+  int mat = A.ielmat[e];
+  mat = mat > 0 ? mat : 0;
+  const int et = A.eos_itype[mat];
+  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
+  A.en_o[e] = en;
+  A.rho_o[e] = rho;
+  A.pre_o[e] = eos_pressure(et, ep, rho, en, A.pcut);
+  const double c2 = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
+  A.cs2_o[e] = c2;
+  if (A.write_geometry) {
+    // the remap that follows reads the corner weights and the volume at
+    // x^{n+1}; it rewrites the rest of the geometry on the moved mesh
+    st4(A.cnwt, e, g.wt);
+    A.vol[e] = g.vol;
+  }
+  return c2;
+}
+
+// F4's work for element e with its inputs from F3's node set and F2's
+// corner-major forces; leaves nd, x, y and the geometry g for F1's terms
 __device__ __forceinline__ double f4_element(const FusedArgs& A, const FusedNodes& O,
                                              const DevTimestep* ts, int e, int nd[4],
                                              double x[4], double y[4], Iso& g) {
@@ -823,10 +940,6 @@ __device__ __forceinline__ double f4_element(const FusedArgs& A, const FusedNode
   load_nd(A.ielnd, e, nd);
   double ub[4], vb[4];
   load4(O.x, nd, x); load4(O.y, nd, y); load4(A.ubar, nd, ub); load4(A.vbar, nd, vb);
-  getiso(x, y, g);
-  if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
-  const double m = A.mass[e];
-  const double rho = m / g.vol;
   double fx[4], fy[4];
   const int nel = A.nel;
 #pragma unroll
@@ -837,23 +950,7 @@ __device__ __forceinline__ double f4_element(const FusedArgs& A, const FusedNode
     fx[j] = __builtin_nontemporal_load(A.fxT + j * nel + e);
     fy[j] = __builtin_nontemporal_load(A.fyT + j * nel + e);
   }
-  const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
-  int mat = A.ielmat[e];
-  mat = mat > 0 ? mat : 0;
-  const int et = A.eos_itype[mat];
-  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
-  A.en[e] = en;
-  A.rho[e] = rho;
-  A.pre[e] = eos_pressure(et, ep, rho, en, A.pcut);
-  const double c2 = eos_cs2(et, ep, rho, en, A.pcut, A.ccut);
-  A.cs2[e] = c2;
-  if (A.write_geometry) {
-    // the remap that follows reads the corner weights and the volume at
-    // x^{n+1}; it rewrites the rest of the geometry on the moved mesh
-    st4(A.cnwt, e, g.wt);
-    A.vol[e] = g.vol;
-  }
-  return c2;
+  return f4_state_of(A, e, dt, x, y, ub, vb, fx, fy, g);
 }
 
 // ---------------------------------------------------------------- F4
@@ -910,6 +1007,217 @@ k_f4_state_ens(FusedArgs A, FusedNodes O, EnsArgs E) {
   e1_collect(A, E, e, lane, moving, cflv, divv);
 }
 
+
+// ------------------------------------------------------------ the tile step
+// F2, F3 and F4 of a pure-cell Lagrangian step in one kernel per tile of
+// nodes (bl_tiles.hip): the corner terms of F2 stay in LDS for the tile's
+// node sums, the node results stay in LDS for the corrector state of the
+// elements the tile holds whole.  The arithmetic is F2's, F3's and F4's own
+// (f2_element, f3_node, f4_state_of), so the results are the same bits.  A
+// node's corner terms are summed over jj = 1..4 in that order: on a mesh
+// where no node is corner jj of two elements (build_tiles checks it) this is
+// the (corner, element) order of hydro_kn_scatteracceleration.  The element
+// state of the step is written to the other element set, so a step that
+// stops on a negative volume leaves the state it started from, as the
+// three-kernel step does (F4 does not run after F2's negative volume there).
+
+// The partial slots of one workgroup of F1's terms: its DIV maximum and
+// CFL minimum to slots seg and 2 * nseg + seg.  Inactive lanes carry the
+// neutral values and an index that never wins a tie.
+__device__ __forceinline__ void seg_collect(const FusedArgs& A, int seg, int nseg, bool active,
+                                            int e, double cflv, double divv) {
+  const int idx = active ? e + 1 : 0x7fffffff;
+  if (!active || !(divv == divv)) divv = -1.0;      // a NaN never wins the strict '>' scan
+  if (!active || !(cflv == cflv)) cflv = DBL_MAX;   // nor the strict '<' scan
+  block_reduce_at<false>(divv, idx, A.part + seg, A.parti + seg);
+  __syncthreads();
+  block_reduce_at<true>(cflv, idx, A.part + 2 * nseg + seg, A.parti + 2 * nseg + seg);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip) return;
+  // slot-fastest, so that a wave's lanes of one (quantity, corner) are
+  // consecutive doubles: (quantity m, area, fx, fy; corner jj; node slot)
+  __shared__ double acc[4][4][kTileNodes + 1];
+  __shared__ double nout[6][kTileNodes + 1];  // x, y, u, v at n+1, ubar, vbar
+  __shared__ uint8_t pres[4][kTileNodes + 1];
+  const int t = xcd_block();
+  const int tid = threadIdx.x;
+  const int eb = A.tile_eptr[t], ne = A.tile_eptr[t + 1] - eb;
+  const int nb = A.tile_nptr[t], nn = A.tile_nptr[t + 1] - nb;
+  // the owned node's start-of-step values, loaded now for F3 below
+  const bool own = tid < nn;
+  int n = 0, ntype = 0;
+  double x0 = 0.0, y0 = 0.0, u0 = 0.0, v0 = 0.0;
+  if (own) {
+    n = A.tile_nodes[nb + tid];
+    x0 = N.x[n]; y0 = N.y[n]; u0 = N.u[n]; v0 = N.v[n];
+    ntype = A.indtype[n];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pres[j][tid] = 0;
+  }
+  __syncthreads();
+  const double dt = ts->dt;
+  // --- F2: every element around the tile's nodes, one per thread
+  const bool active = tid < ne;
+  int e = 0;
+  uint32_t flags = 0, slots = 0xFFFFFFFFu;
+  double fx[4], fy[4];
+  bool zeroed = false;
+  if (active) {
+    const int2 ent = A.tile_el[eb + tid];
+    const int4 q = A.tile_nd[eb + tid];
+    const int nd[4] = {q.x, q.y, q.z, q.w};
+    e = ent.x & 0x3fffffff;
+    flags = (uint32_t)ent.x >> 30;
+    slots = (uint32_t)ent.y;
+    const bool writer = (flags & kTileWriter) != 0;
+    const bool at_rest = A.rest8[e] != 0;
+    double wt[4], mt[4];
+    f2_element(A, N, e, nd, dt, ts->rdt, at_rest, writer, fx, fy, wt, mt);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t sl = (slots >> (8 * j)) & 0xFFu;
+      if (sl != kNoSlot) {
+        acc[0][j][sl] = mt[j];
+        acc[1][j][sl] = wt[j];
+        acc[2][j][sl] = fx[j];
+        acc[3][j][sl] = fy[j];
+        pres[j][sl] = 1;
+      }
+    }
+    if (writer) {
+      // a boundary element's corner forces for k_f4_boundary; the batch's
+      // last step leaves them in rscratch23/24 (lagstep cnfx/cnfy)
+      if (!(flags & kTileInterior)) {
+        double* bf = A.bnd_f + 8 * (size_t)A.el_bslot[e];
+        st4(bf, 0, fx);
+        st4(bf, 1, fy);
+      }
+      if (ts->last) {
+        st4(A.cnfx, e, fx);
+        st4(A.cnfy, e, fy);
+      }
+      zeroed = at_rest && !A.rest8_prev[e];
+      if (zeroed) write_rest_zeros(A, e);
+    }
+  }
+  const uint64_t wrote = __ballot(zeroed);
+  if ((tid & 63) == 0 && wrote) atomicAdd(A.vstats + 1, (unsigned long long)__popcll(wrote));
+  __syncthreads();
+  // --- F3: the tile's nodes, their corners in jj order
+  if (own) {
+    double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (pres[j][tid]) {
+        m = m + acc[0][j][tid];
+        a = a + acc[1][j][tid];
+        fu = fu + acc[2][j][tid];
+        fv = fv + acc[3][j][tid];
+      }
+    double out[6];
+    f3_node(A, dt, m, a, fu, fv, x0, y0, u0, v0, ntype, out);
+    O.x[n] = out[0];
+    O.y[n] = out[1];
+    O.u[n] = out[2];
+    O.v[n] = out[3];
+    A.ubar[n] = out[4];
+    A.vbar[n] = out[5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) nout[k][tid] = out[k];
+  }
+  __syncthreads();
+  // --- F4 and the next step's F1 terms: the elements held whole
+  const bool interior = active && (flags & kTileInterior);
+  double cflv = DBL_MAX, divv = -1.0;
+  if (interior) {
+    double x[4], y[4], u[4], v[4], ub[4], vb[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t sl = (slots >> (8 * j)) & 0xFFu;
+      x[j] = nout[0][sl];
+      y[j] = nout[1][sl];
+      u[j] = nout[2][sl];
+      v[j] = nout[3][sl];
+      ub[j] = nout[4][sl];
+      vb[j] = nout[5][sl];
+    }
+    Iso g;
+    bool moving;
+    const double c2 = f4_state_of(A, e, dt, x, y, ub, vb, fx, fy, g);
+    f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
+    A.rest8_next[e] = moving ? 0 : 1;
+  }
+  seg_collect(A, t, A.nseg, interior, e, cflv, divv);
+}
+
+// F4 and the next step's F1 terms of the boundary list's elements
+__global__ void __launch_bounds__(kBlock)
+k_f4_boundary(FusedArgs A, FusedNodes O) {
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || A.dtres->negvol) return;
+  const int b = xcd_block();
+  const int k = b * kBlock + threadIdx.x;
+  const bool active = k < A.nbnd;
+  int e = 0;
+  double cflv = DBL_MAX, divv = -1.0;
+  if (active) {
+    bool moving;
+    e = A.bnd_el[k];
+    int nd[4];
+    load_nd(A.ielnd, e, nd);
+    double x[4], y[4], u[4], v[4], ub[4], vb[4], fx[4], fy[4];
+    load4(O.x, nd, x); load4(O.y, nd, y); load4(A.ubar, nd, ub); load4(A.vbar, nd, vb);
+    ld4(A.bnd_f, 2 * k, fx);
+    ld4(A.bnd_f, 2 * k + 1, fy);
+    Iso g;
+    const double c2 = f4_state_of(A, e, ts->dt, x, y, ub, vb, fx, fy, g);
+    load4(O.u, nd, u); load4(O.v, nd, v);
+    f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
+    A.rest8_next[e] = moving ? 0 : 1;
+  }
+  seg_collect(A, A.ntile + b, A.nseg, active, e, cflv, divv);
+}
+
+// F1 of a tile-step run call's first step, in the tile step's segments:
+// one workgroup per tile over its interior elements, then (boundary) one
+// per boundary block; the at-rest bytes of that step go to rest8_next
+__global__ void __launch_bounds__(kBlock)
+k_f1_seg(FusedArgs A, FusedNodes N, int boundary) {
+  if (step_skipped(A, A.ts)) return;
+  const int tid = threadIdx.x;
+  const int b = xcd_block();
+  int e = 0, seg;
+  bool active;
+  if (!boundary) {
+    const int eb = A.tile_eptr[b], ne = A.tile_eptr[b + 1] - eb;
+    const int2 ent = tid < ne ? A.tile_el[eb + tid] : make_int2(0, 0);
+    active = tid < ne && (((uint32_t)ent.x >> 30) & kTileInterior);
+    e = ent.x & 0x3fffffff;
+    seg = b;
+  } else {
+    const int k = b * kBlock + tid;
+    active = k < A.nbnd;
+    if (active) e = A.bnd_el[k];
+    seg = A.ntile + b;
+  }
+  double cflv = DBL_MAX, divv = -1.0;
+  if (active) {
+    bool moving;
+    int nd[4];
+    load_nd(A.ielnd, e, nd);
+    double x[4], y[4], u[4], v[4];
+    load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+    Iso g;
+    getiso(x, y, g);
+    f1_terms(A, e, x, y, u, v, g, A.cs2[e], moving, cflv, divv);
+    A.rest8_next[e] = moving ? 0 : 1;
+  }
+  seg_collect(A, seg, A.nseg, active, e, cflv, divv);
+}
 
 // ------------------------------------------------------- remap (Eulerian)
 // The unsplit remap of ale_dr_advect (adv_type = 1) with zeul or the node
@@ -1299,6 +1607,7 @@ FusedArgs make_args(bl_handle_t* h) {
   A.eos_itype = h->d_eos_itype; A.eos_param = h->d_eos_param;
   A.rho = h->R(BL_ELDENSITY); A.en = h->R(BL_ELENERGY); A.pre = h->R(BL_ELPRESSURE);
   A.cs2 = h->R(BL_ELCS2); A.vol = h->R(BL_ELVOLUME); A.mass = h->R(BL_ELMASS);
+  A.rho_o = A.rho; A.en_o = A.en; A.pre_o = A.pre; A.cs2_o = A.cs2;
   A.cnmass = h->R(BL_CNMASS); A.spmass = h->R(BL_SPMASS); A.cnwt = h->R(BL_CNWT);
   A.cnx = h->R(BL_CNX); A.cny = h->R(BL_CNY);
   A.a1 = h->R(BL_A1); A.a2 = h->R(BL_A2); A.a3 = h->R(BL_A3);
@@ -1308,7 +1617,7 @@ FusedArgs make_args(bl_handle_t* h) {
   A.fxT = h->cterm[0]; A.fyT = h->cterm[1]; A.wtT = h->cterm[2]; A.mT = h->cterm[3];
   A.ubar = h->R(BL_RSCRATCH14); A.vbar = h->R(BL_RSCRATCH15);
   A.part = h->d_part; A.parti = h->d_parti; A.dtres = h->d_dtres;
-  A.work = h->d_work; A.work_count = h->d_ticket;
+  A.work = h->d_work;
   A.rest = h->d_rest; A.vstats = h->d_vstats;
   A.fold = 0; A.rest_prev = nullptr; A.rest_next = nullptr;
   A.ts = h->d_ts;
@@ -1355,6 +1664,44 @@ bool fold_f1(const bl_handle_t* h) {
   return !h->cfg.zexist && h->sz.ncp == 0 && h->ens_n == 0;
 }
 
+// no F5 in a folded Lagrangian step without per-step diagnostics (whose
+// records need the step's skip flag before the next step): F1b records the
+// step before's end, the host the batch's last
+bool end_in_f1b(const bl_handle_t* h) {
+  return fold_f1(h) && !h->diag_on;
+}
+
+// the tile step runs a folded Lagrangian step with no F5 when the mesh has
+// a tiling (bl_tiles.hip) and tiling is on
+bool use_tiles(const bl_handle_t* h) {
+  return end_in_f1b(h) && h->tiles_on && h->tile_state == 1;
+}
+
+// the tile step's arguments for a step from node and element set p: it
+// reads element set p and writes set 1 - p
+void tile_args(bl_handle_t* h, FusedArgs& A, int p) {
+  const int nel = h->sz.nel;
+  A.ntile = h->ntile;
+  A.nbnd = h->nbnd;
+  A.nseg = h->ntile + (int)nblocks(h->nbnd);
+  A.tile_eptr = h->d_tile_eptr;
+  A.tile_nptr = h->d_tile_nptr;
+  A.tile_nodes = h->d_tile_nodes;
+  A.tile_el = h->d_tile_el;
+  A.tile_nd = h->d_tile_nd;
+  A.bnd_el = h->d_bnd_el;
+  A.el_bslot = h->d_el_bslot;
+  A.bnd_f = h->d_bnd_f;
+  A.rho = h->elset[p][0]; A.en = h->elset[p][1]; A.pre = h->elset[p][2]; A.cs2 = h->elset[p][3];
+  A.rho_o = h->elset[1 - p][0]; A.en_o = h->elset[1 - p][1];
+  A.pre_o = h->elset[1 - p][2]; A.cs2_o = h->elset[1 - p][3];
+  A.rest8 = h->d_rest8 + (size_t)p * nel;
+  A.rest8_prev = h->d_rest8 + (size_t)(1 - p) * nel;
+  A.rest8_next = h->d_rest8 + (size_t)(1 - p) * nel;
+  A.fold = 1;
+  A.end_in_f1b = 1;
+}
+
 // the two at-rest masks of a folded run: m[p] is the mask of a step that
 // starts from node set p
 void rest_masks(bl_handle_t* h, uint64_t* m[2]) {
@@ -1367,6 +1714,14 @@ int launch_f1_prologue(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedArgs A = make_args(h);
   A.stop_at_time_end = stop_at_time_end;
   A.time_end = h->cfg.time_end;
+  if (use_tiles(h)) {
+    tile_args(h, A, p);
+    A.rest8_next = h->d_rest8 + (size_t)p * h->sz.nel;  // the first step's own bytes
+    FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
+    BL_LAUNCH(h, "f1_getdt", k_f1_seg, (unsigned)h->ntile, kBlock, A, N, 0);
+    if (h->nbnd > 0) BL_LAUNCH(h, "f1_getdt", k_f1_seg, nblocks(h->nbnd), kBlock, A, N, 1);
+    return BL_SUCCESS;
+  }
   uint64_t* m[2];
   rest_masks(h, m);
   A.fold = 1;
@@ -1387,7 +1742,19 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
+  if (use_tiles(h)) {
+    tile_args(h, A, p);
+    const unsigned nseg = (unsigned)A.nseg;
+    const unsigned nbl = std::min(nseg, (unsigned)h->tune_lim_grid);
+    BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nseg);
+    BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nseg, (int)nbl);
+    BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O);
+    if (h->nbnd > 0)
+      BL_LAUNCH(h, "f4_boundary", k_f4_boundary, nblocks(h->nbnd), kBlock, A, Oc);
+    return BL_SUCCESS;
+  }
   const unsigned nbl = std::min(nbe, (unsigned)h->tune_lim_grid);
+  A.end_in_f1b = end_in_f1b(h) ? 1 : 0;
   if (fold_f1(h)) {
     // F1 ran in the previous step's F4 (or in the run call's prologue)
     uint64_t* m[2];
@@ -1410,7 +1777,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   BL_LAUNCH(h, "f4_state", k_f4_state, nbe, kBlock, A, Oc);
   // Eulerian ALE: F5 runs inside R1 and F4m inside R4
   const bool fold_f5 = h->cfg.zexist && h->cfg.zeul;
-  if (!fold_f5) BL_LAUNCH(h, "f5_end", k_f5_end, 1 + nbm, kBlock, A, Oc);
+  if (!fold_f5 && !A.end_in_f1b) BL_LAUNCH(h, "f5_end", k_f5_end, 1 + nbm, kBlock, A, Oc);
   if (h->cfg.zexist) {
     // the remap on the node set this step wrote
     AleArgs B = make_ale(h, O);
@@ -1444,7 +1811,10 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
 }
 
 int ensure_nodesets(bl_handle_t* h) {
-  if (!h->d_work) BL_HIP(hipMalloc(&h->d_work, sizeof(int32_t) * std::max(h->sz.nel, 1)));
+  if (!h->d_work) {
+    // an ensemble's per-instance worklist segments
+    BL_HIP(hipMalloc(&h->d_work, sizeof(int32_t) * std::max(h->sz.nel, 1)));
+  }
   if (!h->d_rest) {
     // one word per wave of F1's grid, the partial last workgroup included,
     // twice (the current and the previous step's with F1 folded into F4)
@@ -1477,11 +1847,16 @@ int ensure_nodesets(bl_handle_t* h) {
 // first step of the call writes every element's.  Outside any capture.
 // Also empties the worklist.
 int clear_rest(bl_handle_t* h) {
+  // the negative-volume flags the last call's last step may have left (F1L
+  // and F1b read them as the step before's outcome)
+  BL_HIP(hipMemsetAsync(h->d_dtres, 0, sizeof(DtResult), h->stream));
   BL_HIP(hipMemsetAsync(h->d_rest, 0, sizeof(uint64_t) * 8 * (size_t)nblocks(std::max(h->sz.nel, 1)),
                         h->stream));
   BL_HIP(hipMemsetAsync(h->d_vstats, 0, sizeof(unsigned long long) * 2, h->stream));
-  // the worklist's length: a folded run call ends with the next step's list
-  BL_HIP(hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream));
+  if (h->d_rest8)
+    BL_HIP(hipMemsetAsync(h->d_rest8, 0, 2 * (size_t)std::max(h->sz.nel, 1), h->stream));
+  // an ensemble's worklist lengths: a folded run call ends with the next
+  // step's lists
   if (h->ens_n > 0 && h->d_cnt)
     BL_HIP(hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream));
   return BL_SUCCESS;
@@ -1493,6 +1868,34 @@ void point_fields_at(bl_handle_t* h, int s) {
   h->f[BL_NDU] = h->nodeset[s][2];
   h->f[BL_NDV] = h->nodeset[s][3];
   h->cur_set = s;
+}
+
+// the element state after a tile-step batch: set s, like the nodes
+void point_elements_at(bl_handle_t* h, int s) {
+  h->f[BL_ELDENSITY] = h->elset[s][0];
+  h->f[BL_ELENERGY] = h->elset[s][1];
+  h->f[BL_ELPRESSURE] = h->elset[s][2];
+  h->f[BL_ELCS2] = h->elset[s][3];
+  h->el_cur = s;
+}
+
+// Before a tile-step batch: the tiling (built when the mesh's connectivity
+// changed), and element set cur_set holding the registry's element state
+// (a batch of another step kind moved the node sets but kept the elements
+// in place).  Outside any capture.
+int prepare_tiles(bl_handle_t* h) {
+  if (!h->tiles_on || !end_in_f1b(h)) return BL_SUCCESS;
+  if (h->tile_state == 0) {
+    if (build_tiles(h)) return BL_FAILURE;
+    invalidate_graph(h);
+  }
+  if (h->tile_state != 1) return BL_SUCCESS;
+  if (h->el_cur != h->cur_set) {
+    for (int k = 0; k < 4; ++k) std::swap(h->elset[0][k], h->elset[1][k]);
+    h->el_cur = h->cur_set;
+    invalidate_graph(h);
+  }
+  return BL_SUCCESS;
 }
 
 // the packed neighbour/face table of the fused remap, from ielel and ielfc
@@ -1552,6 +1955,7 @@ int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
 // the registry's geometry fields after a Lagrangian batch: F4 keeps them in
 // registers, so they are rebuilt once from the final coordinates; the last
 // step's corrector corner forces go back to the element-major rscratch23/24
+// (fxT null: the tile step's last step wrote them there itself)
 __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
                               const double* __restrict__ ndx, const double* __restrict__ ndy,
                               double* __restrict__ cnx, double* __restrict__ cny,
@@ -1563,14 +1967,16 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
                               double* __restrict__ cnfx, double* __restrict__ cnfy) {
   int e = xcd_index();
   if (e >= nel) return;
-  double fx[4], fy[4];
+  if (fxT) {
+    double fx[4], fy[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    fx[j] = fxT[j * nel + e];
-    fy[j] = fyT[j * nel + e];
+    for (int j = 0; j < 4; ++j) {
+      fx[j] = fxT[j * nel + e];
+      fy[j] = fyT[j * nel + e];
+    }
+    st4(cnfx, e, fx);
+    st4(cnfy, e, fy);
   }
-  st4(cnfx, e, fx);
-  st4(cnfy, e, fy);
   int nd[4];
   load_nd(ielnd, e, nd);
   double x[4], y[4];
@@ -1720,7 +2126,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
     A.einst = h->d_einst;
     A.ninst = h->d_ninst;
     A.inst_dt = h->d_inst_dt;
-  A.inst_rdt = h->d_inst_dt + h->ens_n;
+    A.inst_rdt = h->d_inst_dt + h->ens_n;
     uint64_t* m[2];
     rest_masks(h, m);
     A.fold = 1;
@@ -1840,9 +2246,9 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
     A.einst = h->d_einst;
     A.ninst = h->d_ninst;
     A.inst_dt = h->d_inst_dt;
-  A.inst_rdt = h->d_inst_dt + h->ens_n;
+    A.inst_rdt = h->d_inst_dt + h->ens_n;
   }
-  // worklist lengths of the "every element" trials
+  // an ensemble's worklist lengths of the "every element" trials
   std::vector<uint32_t> full(ens ? h->ens_n : 1, (uint32_t)nel);
   if (ens)
     for (int k = 0; k < h->ens_n; ++k) full[k] = h->ens_eoff[k + 1] - h->ens_eoff[k];
@@ -1865,17 +2271,21 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
       // element listed (the worklist as the flow spreads)
       for (int pass = 0; pass < 2 && rc == BL_SUCCESS; ++pass) {
         std::vector<float> t;
-        if (pass == 1)
+        // every element in motion: the single problem's mask all zero, an
+        // ensemble's worklist every element (and its counts below)
+        if (pass == 1 && !ens)
+          hipMemsetAsync(h->d_rest, 0, sizeof(uint64_t) * 4 * (size_t)nbe, h->stream);
+        if (pass == 1 && ens)
           hipLaunchKernelGGL(k_work_all, dim3(nbe), dim3(kBlock), 0, h->stream, h->d_work, nel);
         for (int r = 0; r <= reps; ++r) {
-          uint32_t* cnt = ens ? h->d_cnt : h->d_ticket;
+          uint32_t* cnt = h->d_cnt;
           if (pass == 0) {
-            hipMemsetAsync(cnt, 0, sizeof(uint32_t) * full.size(), h->stream);
+            if (ens) hipMemsetAsync(cnt, 0, sizeof(uint32_t) * full.size(), h->stream);
             if (ens)
               hipLaunchKernelGGL(k_e1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N, E);
             else
               hipLaunchKernelGGL(k_f1_getdt, dim3(nbe), dim3(kBlock), 0, h->stream, A, N);
-          } else {
+          } else if (ens) {
             hipMemcpyAsync(cnt, full.data(), sizeof(uint32_t) * full.size(),
                            hipMemcpyHostToDevice, h->stream);
           }
@@ -1908,7 +2318,6 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
   hipMemcpyAsync(h->R(BL_CNVISCX), save, c4, hipMemcpyDeviceToDevice, h->stream);
   hipMemcpyAsync(h->R(BL_CNVISCY), save + 4 * (size_t)nel, c4, hipMemcpyDeviceToDevice, h->stream);
   hipMemcpyAsync(h->R(BL_ELVISC), save + 8 * (size_t)nel, c1, hipMemcpyDeviceToDevice, h->stream);
-  hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t), h->stream);
   if (ens) hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream);
   hipMemcpyAsync(h->d_ts, &ts_saved, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream);
   clear_rest(h);
@@ -1987,6 +2396,33 @@ int bl_set_tuning(bl_handle_t* h, const bl_tuning_t* t) {
   return BL_SUCCESS;
 }
 
+// the tile step on (the default) or off
+int bl_set_tiling(bl_handle_t* h, int32_t on) {
+  if (!h) {
+    set_error("bl_set_tiling: null handle");
+    return BL_FAILURE;
+  }
+  if ((on != 0) != h->tiles_on) {
+    h->tiles_on = on != 0;
+    invalidate_graph(h);
+  }
+  return BL_SUCCESS;
+}
+
+int bl_get_tiling(bl_handle_t* h, int32_t* on, int32_t* ntile, int32_t* nboundary,
+                  int32_t* nentries) {
+  if (!h) {
+    set_error("bl_get_tiling: null handle");
+    return BL_FAILURE;
+  }
+  const bool built = h->tile_state == 1;
+  if (on) *on = (h->tiles_on && built && end_in_f1b(h)) ? 1 : 0;
+  if (ntile) *ntile = built ? h->ntile : 0;
+  if (nboundary) *nboundary = built ? h->nbnd : 0;
+  if (nentries) *nentries = built ? h->nentries : 0;
+  return BL_SUCCESS;
+}
+
 // the launch shape in use
 int bl_get_tuning(bl_handle_t* h, bl_tuning_t* t) {
   if (!h || !t) {
@@ -2026,7 +2462,7 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
                   err);
   fill_err(err, BL_SUCCESS, 0, "");
-  if (ensure_nodesets(h) || clear_rest(h)) return BL_FAILURE;
+  if (ensure_nodesets(h) || prepare_tiles(h) || clear_rest(h)) return BL_FAILURE;
   if (h->hist_cap < nsteps) {
     // the graphs bake in the history's capacity: size it generously so that
     // runs of different lengths do not re-capture
@@ -2044,7 +2480,9 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   std::memset(d, 0, sizeof(DevTimestep));
   d->time = ts->time; d->dt = ts->dt; d->dts = ts->dts;
   d->nstep = ts->nstep; d->idtel = ts->idtel; d->sdt = ts->sdt;
+  d->last_step = ts->nstep + nsteps;
   BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
+  const bool tiles = use_tiles(h);
   int p = h->cur_set;
   if (fold_f1(h) && launch_f1_prologue(h, p, stop_at_time_end)) return BL_FAILURE;
   for (int k = 0; k < nsteps; ++k) {
@@ -2058,19 +2496,29 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     p ^= 1;
   }
   BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
+  BL_HIP(hipMemcpyAsync(h->h_dtres, h->d_dtres, sizeof(DtResult), hipMemcpyDeviceToHost,
+                        h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));
+  if (d->open) {
+    // the batch's last step ended with no F1b after it
+    step_end(d, h->h_dtres);
+    BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
+  }
   // steps that selected a dt (a failing step included)
   const int taken = d->hist_n;
   // the node set holding the latest state: every step whose F3 ran wrote the
   // other set
   point_fields_at(h, (h->cur_set + d->flips) & 1);
+  if (tiles) point_elements_at(h, h->cur_set);
   {
+    // (the tile step's last step wrote rscratch23/24 itself)
     int nel = h->sz.nel;
     BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
               h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
               h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
-              h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
-              h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
+              h->R(BL_ELVOLUME), tiles ? nullptr : (const double*)h->cterm[0],
+              tiles ? nullptr : (const double*)h->cterm[1], h->R(BL_RSCRATCH23),
+              h->R(BL_RSCRATCH24));
     if (!h->cfg.zexist && mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
   }
   if (taken > 0) {

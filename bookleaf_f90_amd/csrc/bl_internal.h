@@ -59,7 +59,15 @@ struct DevTimestep {
   // none; written only by atomicMin in the dt selection, turned into
   // done/skip by the step's last kernel and into err by the host
   int32_t epend;
-  int32_t pad_i;
+  // the tile step: the nstep of the batch's last step, and whether the step
+  // running is the batch's last (it leaves its corner forces in
+  // rscratch23/24, which its kernels otherwise keep in LDS)
+  int32_t last_step;
+  int32_t last;
+  // a step of a pure-cell Lagrangian batch whose end (the negative-volume
+  // outcome and the node-set flip) the next step's F1b, or the host after
+  // the batch, still has to record
+  int32_t open;
 };
 constexpr int32_t kNoPending = 0x7fffffff;
 
@@ -123,7 +131,8 @@ struct Handle {
   // corrector corner terms of the fused step, corner-major (4*nel each):
   // force x, force y, half-step corner weight, nodal-mass term
   double* cterm[4] = {};
-  // F1's worklist (nel): elements whose nodes do not share one velocity
+  // an ensemble's worklist (nel): elements whose nodes do not share one
+  // velocity, one segment per instance
   int32_t* d_work = nullptr;
   uint64_t* d_rest = nullptr;             // F1's at-rest mask, 4 words per 256 elements
   unsigned long long* d_vstats = nullptr;  // counters of the last run call
@@ -171,6 +180,26 @@ struct Handle {
   int32_t tune_lim_block = 256, tune_lim_grid = 512, tune_ens_chunk = 4096;
   int32_t ens_lblocks_cap = 0;    // limiter partial slots allocated
   bool nbfc_stale = true;         // rebuild it before the next capture
+  // the tile step of a pure-cell Lagrangian problem (bl_tiles.hip): the
+  // nodes cut into tiles of at most kTileNodes, each with the elements
+  // around its nodes (at most kTileEls); elements with every node in one
+  // tile are that tile's interior, the rest the boundary list
+  int32_t tile_state = 0;         // 0 not built, 1 built, -1 the mesh has no tiling
+  bool tiles_on = true;           // bl_set_tiling
+  int32_t ntile = 0, nbnd = 0, nentries = 0;
+  int32_t* d_tile_eptr = nullptr;  // ntile + 1
+  int32_t* d_tile_nptr = nullptr;  // ntile + 1
+  int32_t* d_tile_nodes = nullptr; // owned nodes, 0-based
+  int2* d_tile_el = nullptr;       // (element | flags << 30, four local node slots)
+  int4* d_tile_nd = nullptr;       // the entries' four nodes, 0-based
+  int32_t* d_bnd_el = nullptr;     // boundary elements, 0-based
+  int32_t* d_el_bslot = nullptr;   // nel: an element's place in the boundary list, or -1
+  double* d_bnd_f = nullptr;       // 8 per boundary element: its corrector corner forces
+  uint8_t* d_rest8 = nullptr;      // 2 x nel at-rest bytes of the tile step
+  // the element state of the tile step in two sets (eldensity, elenergy,
+  // elpressure, elcs2); el_cur names the one the registry points at
+  double* elset[2][4] = {};
+  int el_cur = 0;
   float* d_ld1 = nullptr;         // (nnd, 2): one-sweep displacement of the node
   // per-step diagnostics of the fused step: region table + Sod L1 records
   bool diag_on = false;

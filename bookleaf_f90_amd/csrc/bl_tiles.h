@@ -1,0 +1,22 @@
+// bl_tiles.h -- the node tiling of the tile step (bl_tiles.hip, bl_fused.hip).
+#pragma once
+#include <stdint.h>
+
+namespace bl {
+
+struct Handle;
+
+constexpr int kTileNodes = 255;   // owned nodes per tile (an 8-bit slot, kNoSlot free)
+constexpr int kTileEls = 256;     // elements around a tile's nodes: one per thread
+constexpr int kTileTarget = 200;  // the node count the bisection aims at
+constexpr uint32_t kNoSlot = 0xFF;
+constexpr uint32_t kTileWriter = 1;    // this tile writes the element's side outputs
+constexpr uint32_t kTileInterior = 2;  // all four nodes in this tile
+
+// Cut the nodes of h's mesh into tiles (h->tile_state 1), or find that the
+// mesh has a node that is the same corner of two elements (h->tile_state
+// -1, no tiling); ielnd and the node positions come from the registry.
+int build_tiles(Handle* h);
+void free_tiles(Handle* h);
+
+}  // namespace bl

@@ -1,0 +1,237 @@
+// bl_tiles.hip -- the node tiling of the tile step (host side).
+//
+// The tile step fuses the fused step's F2, F3 and F4 for a pure-cell
+// Lagrangian problem: one workgroup owns a tile of nodes, computes the
+// predictor and the corrector corner terms of every element around them
+// (F2's work, kept in LDS), sums them per owned node in the reference's
+// scatter order (F3's work), and then the corrector state of the elements
+// whose four nodes it owns (F4's work).  The elements with nodes in two or
+// more tiles (the boundary list) get their corrector state from a second
+// kernel.  The F2->F3->F4 corner terms then never reach HBM.
+//
+// The reference sums a node's corner terms in (corner jj, element in
+// ielsort1 order) order (hydro_kn_getacceleration.f90:53-72).  When no node
+// is corner jj of two elements, that order is simply jj = 1..4 over the
+// node's incident corners, which a tile reproduces from one LDS slot per
+// (node, jj).  A mesh where some node is corner jj of two elements keeps the
+// three-kernel step.
+//
+// Tiles come from recursive coordinate bisection of the node positions
+// when the tiling is built (setup_rcb.f90 cuts the reference's partitions
+// the same way): each cut is along the longer side of the node set's
+// bounding box, at the share of the nodes that makes the tile counts of the
+// two sides proportional, down to about kTileTarget nodes; a tile whose
+// nodes or incident elements do not fit one workgroup is cut once more.
+// The positions only shape the tiles: any tiling gives the same results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "bl_internal.h"
+#include "bl_tiles.h"
+
+namespace bl {
+
+namespace {
+
+struct Rcb {
+  const std::vector<double>& x;
+  const std::vector<double>& y;
+  const std::vector<int32_t>& adj_ptr;  // node -> incident elements
+  const std::vector<int32_t>& adj;
+  std::vector<int32_t> idx;              // node ids, cut in place
+  std::vector<int32_t> stamp;            // per element: last tile counted
+  std::vector<std::pair<int, int>> leaves;
+  int counter = 0;
+
+  int incident(int lo, int hi) {
+    ++counter;
+    int n = 0;
+    for (int k = lo; k < hi; ++k) {
+      const int v = idx[k];
+      for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q)
+        if (stamp[adj[q]] != counter) { stamp[adj[q]] = counter; ++n; }
+    }
+    return n;
+  }
+
+  void cut(int lo, int hi, int k) {
+    if (k <= 1) {
+      if ((hi - lo <= kTileNodes && incident(lo, hi) <= kTileEls) || hi - lo < 2) {
+        leaves.emplace_back(lo, hi);
+        return;
+      }
+      k = 2;
+    }
+    double x0 = x[idx[lo]], x1 = x0, y0 = y[idx[lo]], y1 = y0;
+    for (int q = lo + 1; q < hi; ++q) {
+      const int v = idx[q];
+      x0 = std::min(x0, x[v]); x1 = std::max(x1, x[v]);
+      y0 = std::min(y0, y[v]); y1 = std::max(y1, y[v]);
+    }
+    const std::vector<double>& c = (x1 - x0 >= y1 - y0) ? x : y;
+    const int k1 = k / 2;
+    int m = lo + (int)((int64_t)(hi - lo) * k1 / k);
+    m = std::max(lo + 1, std::min(hi - 1, m));
+    std::nth_element(idx.begin() + lo, idx.begin() + m, idx.begin() + hi,
+                     [&c](int32_t a, int32_t b) { return c[a] < c[b] || (c[a] == c[b] && a < b); });
+    cut(lo, m, k1);
+    cut(m, hi, k - k1);
+  }
+};
+
+template <typename T>
+int upload_vec(T** d, const std::vector<T>& v) {
+  if (*d) { hipFree(*d); *d = nullptr; }
+  BL_HIP(hipMalloc(d, sizeof(T) * std::max<size_t>(v.size(), 1)));
+  if (!v.empty()) BL_HIP(hipMemcpy(*d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return BL_SUCCESS;
+}
+
+}  // namespace
+
+void free_tiles(Handle* h) {
+  void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_nodes, h->d_tile_el, h->d_tile_nd,
+                  h->d_bnd_el, h->d_el_bslot, h->d_bnd_f, h->d_rest8};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  h->d_tile_eptr = h->d_tile_nptr = h->d_tile_nodes = nullptr;
+  h->d_tile_el = nullptr;
+  h->d_tile_nd = nullptr;
+  h->d_bnd_el = h->d_el_bslot = nullptr;
+  h->d_bnd_f = nullptr;
+  h->d_rest8 = nullptr;
+  h->ntile = h->nbnd = h->nentries = 0;
+  h->tile_state = 0;
+}
+
+int build_tiles(Handle* h) {
+  free_tiles(h);
+  const int nel = h->sz.nel, nnd = h->sz.nnd;
+  std::vector<int32_t> ielnd(4 * (size_t)nel);
+  std::vector<double> x(nnd), y(nnd);
+  bl_handle_t* hh = static_cast<bl_handle_t*>(h);
+  if (bl_download(hh, BL_IELND, ielnd.data(), 4 * (int64_t)nel) ||
+      bl_download(hh, BL_NDX, x.data(), nnd) || bl_download(hh, BL_NDY, y.data(), nnd))
+    return BL_FAILURE;
+  // every node is corner jj of at most one element, for each jj
+  std::vector<uint8_t> seen(4 * (size_t)nnd, 0);
+  std::vector<int32_t> adj_ptr(nnd + 1, 0);
+  for (int e = 0; e < nel; ++e)
+    for (int j = 0; j < 4; ++j) {
+      const int v = ielnd[4 * (size_t)e + j] - 1;
+      if (v < 0 || v >= nnd || seen[4 * (size_t)v + j]) {
+        h->tile_state = -1;  // not a tiling this step can use
+        return BL_SUCCESS;
+      }
+      seen[4 * (size_t)v + j] = 1;
+      ++adj_ptr[v + 1];
+    }
+  for (int v = 0; v < nnd; ++v) adj_ptr[v + 1] += adj_ptr[v];
+  // (an element with a repeated node is listed once per corner: the counts
+  // below take each element once)
+  std::vector<int32_t> adj(adj_ptr[nnd]), fill(adj_ptr.begin(), adj_ptr.end() - 1);
+  for (int e = 0; e < nel; ++e)
+    for (int j = 0; j < 4; ++j) adj[fill[ielnd[4 * (size_t)e + j] - 1]++] = e;
+  if (nel >= (1 << 30)) { h->tile_state = -1; return BL_SUCCESS; }
+  Rcb r{x, y, adj_ptr, adj, {}, std::vector<int32_t>(nel, 0), {}, 0};
+  r.idx.resize(nnd);
+  for (int v = 0; v < nnd; ++v) r.idx[v] = v;
+  r.cut(0, nnd, std::max(1, (nnd + kTileTarget - 1) / kTileTarget));
+
+  const int ntile = (int)r.leaves.size();
+  std::vector<int32_t> tile_of(nnd), slot_of(nnd), nptr(ntile + 1, 0), nodes(nnd);
+  int pos = 0;
+  for (int t = 0; t < ntile; ++t) {
+    const auto lr = r.leaves[t];
+    std::sort(r.idx.begin() + lr.first, r.idx.begin() + lr.second);
+    if (lr.second - lr.first > kTileNodes) { h->tile_state = -1; return BL_SUCCESS; }
+    nptr[t] = pos;
+    for (int k = lr.first; k < lr.second; ++k) {
+      const int v = r.idx[k];
+      tile_of[v] = t;
+      slot_of[v] = k - lr.first;
+      nodes[pos++] = v;
+    }
+  }
+  nptr[ntile] = pos;
+  // each tile's incident elements, ascending; the boundary list in tile order
+  std::vector<int32_t> eptr(ntile + 1, 0), bnd, bslot(nel, -1);
+  std::vector<int2> ents;
+  std::vector<int4> ent_nd;
+  ents.reserve((size_t)nel + nel / 4);
+  ent_nd.reserve((size_t)nel + nel / 4);
+  std::vector<int32_t> els;
+  std::vector<int32_t> stamp(nel, -1);
+  for (int t = 0; t < ntile; ++t) {
+    els.clear();
+    for (int k = nptr[t]; k < nptr[t + 1]; ++k) {
+      const int v = nodes[k];
+      for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q)
+        if (stamp[adj[q]] != t) { stamp[adj[q]] = t; els.push_back(adj[q]); }
+    }
+    if ((int)els.size() > kTileEls) { h->tile_state = -1; return BL_SUCCESS; }
+    std::sort(els.begin(), els.end());
+    eptr[t] = (int)ents.size();
+    for (int e : els) {
+      uint32_t slots = 0;
+      bool interior = true;
+      for (int j = 0; j < 4; ++j) {
+        const int v = ielnd[4 * (size_t)e + j] - 1;
+        const bool own = tile_of[v] == t;
+        interior = interior && own;
+        slots |= (uint32_t)(own ? slot_of[v] : kNoSlot) << (8 * j);
+      }
+      const bool writer = tile_of[ielnd[4 * (size_t)e] - 1] == t;
+      if (writer && !interior) {
+        bslot[e] = (int32_t)bnd.size();
+        bnd.push_back(e);
+      }
+      const uint32_t flags = (writer ? kTileWriter : 0u) | (interior ? kTileInterior : 0u);
+      ents.push_back(make_int2((int)((uint32_t)e | (flags << 30)), (int)slots));
+      const int32_t* q = &ielnd[4 * (size_t)e];
+      ent_nd.push_back(make_int4(q[0] - 1, q[1] - 1, q[2] - 1, q[3] - 1));
+    }
+  }
+  eptr[ntile] = (int)ents.size();
+  const int nbnd = (int)bnd.size();
+  if (upload_vec(&h->d_tile_eptr, eptr) || upload_vec(&h->d_tile_nptr, nptr) ||
+      upload_vec(&h->d_tile_nodes, nodes) || upload_vec(&h->d_tile_el, ents) ||
+      upload_vec(&h->d_tile_nd, ent_nd) ||
+      upload_vec(&h->d_bnd_el, bnd) || upload_vec(&h->d_el_bslot, bslot))
+    return BL_FAILURE;
+  BL_HIP(hipMalloc(&h->d_bnd_f, sizeof(double) * 8 * std::max(nbnd, 1)));
+  BL_HIP(hipMalloc(&h->d_rest8, 2 * (size_t)std::max(nel, 1)));
+  BL_HIP(hipMemset(h->d_rest8, 0, 2 * (size_t)std::max(nel, 1)));
+  h->ntile = ntile;
+  h->nbnd = nbnd;
+  h->nentries = (int32_t)ents.size();
+  // the partial slots of the tile step: one per tile, then one per boundary
+  // block
+  const int nseg = ntile + (int)nblocks(nbnd);
+  if (nseg + 16 > h->npart) {
+    hipFree(h->d_part);
+    hipFree(h->d_parti);
+    h->d_part = nullptr;
+    h->d_parti = nullptr;
+    h->npart = nseg + 16;
+    BL_HIP(hipMalloc(&h->d_part, sizeof(double) * 4 * h->npart));
+    BL_HIP(hipMalloc(&h->d_parti, sizeof(int32_t) * 4 * h->npart));
+  }
+  // the second element set
+  if (!h->elset[0][0]) {
+    const int ids[4] = {BL_ELDENSITY, BL_ELENERGY, BL_ELPRESSURE, BL_ELCS2};
+    for (int k = 0; k < 4; ++k) {
+      h->elset[h->cur_set][k] = h->R(ids[k]);
+      BL_HIP(hipMalloc(&h->elset[1 - h->cur_set][k],
+                       sizeof(double) * std::max<int64_t>(h->count[ids[k]], 1)));
+    }
+    h->el_cur = h->cur_set;
+  }
+  h->tile_state = 1;
+  return BL_SUCCESS;
+}
+
+}  // namespace bl

@@ -77,6 +77,9 @@ def load_library(path=LIBPATH):
         lib.bl_run_stats.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
         lib.bl_set_tuning.argtypes = [vp, P(abi.Tuning)]
         lib.bl_get_tuning.argtypes = [vp, P(abi.Tuning)]
+        lib.bl_set_tiling.argtypes = [vp, C.c_int32]
+        lib.bl_get_tiling.argtypes = [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                      P(C.c_int32)]
         lib.bl_timing_enable.argtypes = [vp, C.c_int32]
         lib.bl_timing_reset.argtypes = [vp]
         lib.bl_timing_query.argtypes = [vp, C.c_char_p, P(C.c_double), P(C.c_int64)]
@@ -449,6 +452,17 @@ class BookLeaf(DriverState):
         t = abi.Tuning()
         self._check(self.lib.bl_get_tuning(self.h, C.byref(t)), "get_tuning")
         return {k: getattr(t, k) for k, _ in abi.Tuning._fields_[:3]}
+
+    def set_tiling(self, on=True):
+        """The tile step on or off (bl_set_tiling)."""
+        self._check(self.lib.bl_set_tiling(self.h, int(bool(on))), "set_tiling")
+
+    def get_tiling(self):
+        """(tile step in use, tiles, boundary elements, element entries)
+        (bl_get_tiling)."""
+        v = [C.c_int32(0) for _ in range(4)]
+        self._check(self.lib.bl_get_tiling(self.h, *(C.byref(x) for x in v)), "get_tiling")
+        return bool(v[0].value), v[1].value, v[2].value, v[3].value
 
     def timing(self, on=True):
         self._check(self.lib.bl_timing_enable(self.h, int(on)), "timing_enable")

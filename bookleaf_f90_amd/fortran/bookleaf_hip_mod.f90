@@ -531,6 +531,17 @@ MODULE bookleaf_hip_mod
       TYPE(C_PTR),VALUE                                 :: h
       TYPE(bl_tuning_t),                  INTENT(OUT)   :: t
     END FUNCTION bl_get_tuning
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_set_tiling(h,on) BIND(C,NAME="bl_set_tiling")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE             :: h
+      INTEGER(KIND=C_INT32_T),VALUE :: on
+    END FUNCTION bl_set_tiling
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_get_tiling(h,on,ntile,nboundary,nentries)&
+&                                    BIND(C,NAME="bl_get_tiling")
+      IMPORT :: C_INT32_T,C_PTR
+      TYPE(C_PTR),VALUE                   :: h
+      INTEGER(KIND=C_INT32_T),INTENT(OUT) :: on,ntile,nboundary,nentries
+    END FUNCTION bl_get_tiling
     TYPE(C_PTR) FUNCTION bl_last_error_c() BIND(C,NAME="bl_last_error")
       IMPORT :: C_PTR
     END FUNCTION bl_last_error_c

@@ -562,6 +562,29 @@ def noh(n=30, length=1.0, gamma=5.0 / 3.0, nsteps=100, **cfg):
         eos_param=param, config=config, nsteps=nsteps, dims=(n, n))
 
 
+def taylor_green(n=1024, u0=1.0, p0=10.0, gamma=5.0 / 3.0, nsteps=500, **cfg):
+    """A Taylor-Green vortex on [0,1]^2: node velocity u = u0 sin(pi x)
+    cos(pi y), v = -u0 cos(pi x) sin(pi y), rho = 1, element pressure
+    p0 + (u0^2/4)(cos 2 pi x + cos 2 pi y) at the element centroid (e =
+    p/((gamma-1) rho)); reflecting walls (SLIPY bottom/top, SLIPX left/right:
+    the field's own normal velocity is zero there).  The velocity differs
+    between every pair of neighbouring nodes, so every element is in motion
+    from the first step and goes through the limiter every step."""
+    ielnd, ndx, ndy, indtype = structured_mesh(n, n, 0.0, 1.0, 0.0, 1.0)
+    xc, yc = centroids(ielnd, ndx, ndy)
+    pre = p0 + 0.25 * u0 * u0 * (np.cos(2.0 * math.pi * xc) + np.cos(2.0 * math.pi * yc))
+    nel = n * n
+    itype, param = ideal_eos([gamma])
+    config = dict(DEFAULT_CONFIG, time_end=1.0e30)
+    config.update(cfg)
+    return Problem(
+        name="taylor_green_%dx%d" % (n, n), ielnd=ielnd, ndx=ndx, ndy=ndy, indtype=indtype,
+        ielreg=np.ones(nel, np.int32), ielmat=np.ones(nel, np.int32), density=np.ones(nel),
+        energy=pre / (gamma - 1.0), ndu=u0 * np.sin(math.pi * ndx) * np.cos(math.pi * ndy),
+        ndv=-u0 * np.cos(math.pi * ndx) * np.sin(math.pi * ndy), nmat=1, nreg=1,
+        eos_itype=itype, eos_param=param, config=config, nsteps=nsteps, dims=(n, n))
+
+
 def sod_deck_small(eulerian=False):
     """The reference deck input/sod_sm_lag_small (or _eul_small): 100x25 on
     [0,100]x[0,25], dt_initial 1e-3, time_end 20.  The mesh is a uniform

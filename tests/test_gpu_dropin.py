@@ -21,6 +21,9 @@ from tests.helpers import small_problems, diff_fields
 
 pytestmark = pytest.mark.gpu
 PROBLEMS = small_problems()
+# the component forces, unset in the reference until the first getforce
+# (driver 201) writes them: both builds hold whatever their memory held
+UNSET_BEFORE_FORCE = ("rcpscratch21", "rcpscratch22")
 
 
 @pytest.fixture(scope="module")
@@ -48,7 +51,8 @@ def test_dropin_every_driver_call(name, refio):
     for a, b in zip(gpu, ref):
         assert (a["nstep"], a["dt"], a["time"], a["idtel"]) == (b["nstep"], b["dt"], b["time"],
                                                               b["idtel"]), a["label"]
-        assert diff_fields(a, b, ncp=p.ncp) == [], "after driver %d" % a["label"]
+        skip = UNSET_BEFORE_FORCE if a["label"] < 201 else ()
+        assert diff_fields(a, b, ncp=p.ncp, skip=skip) == [], "after driver %d" % a["label"]
 
 
 @pytest.mark.parametrize("name", sorted(PROBLEMS))
@@ -59,6 +63,7 @@ def test_dropin_run(name, refio):
     assert gs == rs  # (nstep, idtel, dt, time, dt control) of every step
     assert [d["label"] for d in gpu] == [d["label"] for d in ref]
     for a, b in zip(gpu, ref):
-        assert diff_fields(a, b, ncp=p.ncp) == [], "dump %d" % a["label"]
+        skip = UNSET_BEFORE_FORCE if a["label"] == 0 else ()
+        assert diff_fields(a, b, ncp=p.ncp, skip=skip) == [], "dump %d" % a["label"]
     for t in ("region", "material"):
         assert np.array_equal(gpu[-1]["shortprint"][t], ref[-1]["shortprint"][t])

@@ -13,7 +13,13 @@ from tests.test_gpu_graph import excluded
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", G.names())
+# the Tait and JWL records call pow and exp, which the device computes
+# within an ulp of glibc but not bit for bit: test_gpu_eos.py holds them to
+# stated tolerances
+BITEXACT = [n for n in G.names() if not n.startswith("tait_jwl")]
+
+
+@pytest.mark.parametrize("name", BITEXACT)
 def test_graph_runner_matches_reference_record(name):
     from bookleaf_f90_amd.device import BookLeaf
     p, meta, z = G.load(name)
