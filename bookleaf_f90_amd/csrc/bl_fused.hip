@@ -750,7 +750,6 @@ __device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes&
   for (int c = 0; c < 4; ++c) { xh[c] = x[c] + h * u[c]; yh[c] = y[c] + h * v[c]; }
   Iso gh;
   getiso(xh, yh, gh);
-  if (check_volume && gh.vol < 0.0) atomicMax(&A.dtres->negvol, e + 1);
   const double m = A.mass[e];
   const double rhoh = m / gh.vol;
   const double eh = energy_update(A.en[e], m, fx, fy, u, v, h, A.zerocut);
@@ -781,6 +780,8 @@ __device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes&
     else mt[j] = rhoh * gh.wt[j];
     wt[j] = gh.wt[j];
   }
+  // last: no load of the element passes this atomic
+  if (check_volume && gh.vol < 0.0) atomicMax(&A.dtres->negvol, e + 1);
 }
 
 // the zero viscosity of an element at rest now but not in the step before
@@ -909,7 +910,6 @@ __device__ __forceinline__ double f4_state_of(const FusedArgs& A, int e, double 
                                               const double ub[4], const double vb[4],
                                               const double fx[4], const double fy[4], Iso& g) {
   getiso(x, y, g);
-  if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
   const double m = A.mass[e];
   const double rho = m / g.vol;
   const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
@@ -928,6 +928,8 @@ __device__ __forceinline__ double f4_state_of(const FusedArgs& A, int e, double 
     st4(A.cnwt, e, g.wt);
     A.vol[e] = g.vol;
   }
+  // last: no load of the element passes this atomic
+  if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
   return c2;
 }
 
