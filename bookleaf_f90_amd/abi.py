@@ -11,6 +11,8 @@ SUCCESS, FAILURE = 0, -1
 HALT_ALL, HALT_SINGLE = 1, 2
 SLEN = 130
 NEOSPARAM = 6
+# largest mesh of bl_hydro_getdt_noalloc (BL_NOALLOC_MAX_EL)
+NOALLOC_MAX_EL = 1 << 24
 
 KIND_R64, KIND_I32, KIND_BOOL = 1, 2, 3
 DT_MAXIMUM, DT_GROWTH, DT_INITIAL, DT_CFL, DT_DIV = 1, 2, 3, 4, 5

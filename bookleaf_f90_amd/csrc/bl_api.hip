@@ -13,6 +13,7 @@
 #include <cfloat>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -424,7 +425,7 @@ __global__ void k_getdt_b(int nel, double zerocut, double cvisc1, double cvisc2,
                           const double* __restrict__ dv, const double* __restrict__ dx,
                           const double* __restrict__ dy, const double* __restrict__ cnx,
                           const double* __restrict__ cny, const double* __restrict__ cnu,
-                          const double* __restrict__ cnv, double* __restrict__ scratch,
+                          const double* __restrict__ cnv, double* __restrict__ rs27,
                           double* __restrict__ cnviscx, double* __restrict__ cnviscy,
                           double* __restrict__ elvisc, double* __restrict__ ellength) {
   int e = xcd_index();
@@ -479,7 +480,7 @@ __global__ void k_getdt_b(int nel, double zerocut, double cvisc1, double cvisc2,
                            cvisc2, zerocut, ev);
     sc[0] = s1; sc[1] = s2; sc[2] = s3; sc[3] = s4;
   }
-  reinterpret_cast<double4*>(scratch)[e] = make_double4(sc[0], sc[1], sc[2], sc[3]);
+  reinterpret_cast<double4*>(rs27)[e] = make_double4(sc[0], sc[1], sc[2], sc[3]);
   elvisc[e] = ev;
   double4 X = reinterpret_cast<const double4*>(cnx)[e];
   double4 Y = reinterpret_cast<const double4*>(cny)[e];
@@ -522,6 +523,52 @@ __global__ void k_getdt_c(int nel, const int32_t* __restrict__ ielreg,
 __global__ void k_dt_final(int nblk_a, int nblk_c, const double* __restrict__ part,
                            const int32_t* __restrict__ parti, DtResult* __restrict__ res) {
   final_dt_reduce(nblk_a, nblk_c, part, parti, res);
+}
+
+// --- the allocation-free timestep (bl_hydro_getdt_noalloc) ------------------
+// Its workspace, the partials of k_getdt_a/k_getdt_c and k_dt_final's
+// result, lives in this file-scope array: one copy per GPU, held until the
+// process ends.  Layout: the DtResult in the first 256 bytes, then the
+// partial values part[3 * kNaBlocks] (DIV in [0, nb), CFL in [2nb, 3nb), as
+// the handle's partials), then their indices parti[3 * kNaBlocks].  Every
+// workgroup writes its own slots and k_dt_final writes every field the host
+// reads, so nothing in it needs clearing between calls.
+constexpr int kNaBlocks = BL_NOALLOC_MAX_EL / kBlock;
+constexpr size_t kNaPart = 256;
+constexpr size_t kNaParti = kNaPart + sizeof(double) * 3 * (size_t)kNaBlocks;
+constexpr size_t kNaBytes = kNaParti + sizeof(int32_t) * 3 * (size_t)kNaBlocks;
+static_assert(sizeof(DtResult) <= kNaPart, "DtResult does not fit its slot");
+static_assert(BL_NOALLOC_MAX_EL % kBlock == 0, "whole workgroups");
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// scratch's device address on each GPU (hipGetSymbolAddress with that GPU
+// current; the host-side name is no device address)
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mutex;
+static unsigned char* g_na_ptr[kNaMaxDevices] = {};
+static bool g_na_looked[kNaMaxDevices] = {};
+
+// Looks scratch up on the current device (once per device); called by
+// bl_create, so the entry point itself makes no runtime look-up.  Returns
+// the address, or nullptr with the error set.
+static unsigned char* noalloc_scratch(int device) {
+  if (device < 0 || device >= kNaMaxDevices) {
+    set_error("bl_hydro_getdt_noalloc: device ordinal beyond the workspace table");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  if (!g_na_looked[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      set_error(std::string("bl_hydro_getdt_noalloc: hipGetSymbolAddress: ") +
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_na_ptr[device] = static_cast<unsigned char*>(ptr);
+    g_na_looked[device] = true;
+  }
+  return g_na_ptr[device];
 }
 
 }  // namespace bl
@@ -579,6 +626,9 @@ int bl_create(const bl_sizes_t* sizes, bl_handle_t** out) {
       hipHostMalloc(&h->h_ts, sizeof(DevTimestep)) != hipSuccess) {
     set_error("bl_create: out of device memory"); bl_destroy(h); return BL_FAILURE;
   }
+  // the allocation-free timestep's workspace address on this device (a
+  // failed look-up is reported by that entry point, not here)
+  if (!noalloc_scratch(h->device)) g_last_error.clear();
   hipMemsetAsync(h->d_ticket, 0, sizeof(uint32_t) * 16, h->stream);
   hipMemsetAsync(h->d_dtres, 0, sizeof(DtResult), h->stream);
   hipMemsetAsync(h->d_ts, 0, sizeof(DevTimestep), h->stream);
@@ -983,8 +1033,10 @@ int bl_hydro_init(bl_handle_t* h) {
 }
 
 // ----------------------------------------------------------- Lagrangian step
-int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) {
-  if (!h || !h->d_zdtnotreg) { set_error("bl_hydro_getdt: regions not set"); return BL_FAILURE; }
+// hydro_dr_getdt with the partials at (part, parti), each holding 3 * nb
+// slots for nb = nblocks(nel), and the result at dres
+static int getdt_with(bl_handle_t* h, double* part, int32_t* parti, DtResult* dres,
+                      bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) {
   int nel = h->sz.nel;
   unsigned nb = nblocks(nel);
   const bl_config_t& c = h->cfg;
@@ -992,7 +1044,7 @@ int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) 
             h->R(BL_NDV), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1), h->R(BL_A3), h->R(BL_B1),
             h->R(BL_B3), h->R(BL_ELVOLUME), h->R(BL_RSCRATCH21), h->R(BL_RSCRATCH22),
             h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24), h->R(BL_RSCRATCH25), h->R(BL_RSCRATCH26),
-            h->d_part, h->d_parti);
+            part, parti);
   BL_LAUNCH(h, "getdt_b", k_getdt_b, nb, kBlock, nel, c.zerocut, c.cvisc1, c.cvisc2, c.zcut,
             h->I(BL_INDTYPE), h->I(BL_IELEL), h->I(BL_IELND), h->I(BL_IELFC), h->I(BL_IELREG),
             h->d_zdtnotreg, h->d_zmidlength, h->R(BL_ELDENSITY), h->R(BL_ELCS2),
@@ -1002,10 +1054,9 @@ int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) 
             h->R(BL_RSCRATCH12));
   BL_LAUNCH(h, "getdt_c", k_getdt_c, nb, kBlock, nel, h->I(BL_IELREG), h->d_zdtnotreg,
             h->R(BL_ELVISC), h->R(BL_ELCS2), h->R(BL_RSCRATCH12), h->R(BL_RSCRATCH11),
-            h->d_part, h->d_parti, (int)nb, (const int32_t*)h->I(BL_IELMAT), make_mix(h));
-  BL_LAUNCH(h, "dt_final", k_dt_final, 1, kFinalBlock, (int)nb, (int)nb, h->d_part, h->d_parti,
-            h->d_dtres);
-  BL_HIP(hipMemcpyAsync(h->h_dtres, h->d_dtres, sizeof(DtResult), hipMemcpyDeviceToHost, h->stream));
+            part, parti, (int)nb, (const int32_t*)h->I(BL_IELMAT), make_mix(h));
+  BL_LAUNCH(h, "dt_final", k_dt_final, 1, kFinalBlock, (int)nb, (int)nb, part, parti, dres);
+  BL_HIP(hipMemcpyAsync(h->h_dtres, dres, sizeof(DtResult), hipMemcpyDeviceToHost, h->stream));
   BL_HIP(hipStreamSynchronize(h->stream));
   const DtResult& r = *h->h_dtres;
   if (r.cfl_min < 0.0) {
@@ -1022,6 +1073,34 @@ int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) 
   if (div) { div->rdt = c.div_sf / r.div_max; div->idt = r.div_idx; div->sdt = BL_DT_DIV; }
   if (err) { err->ierr = BL_SUCCESS; err->iout = 0; std::memset(err->serr, ' ', BL_SLEN); }
   return BL_SUCCESS;
+}
+
+int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) {
+  if (!h || !h->d_zdtnotreg) { set_error("bl_hydro_getdt: regions not set"); return BL_FAILURE; }
+  return getdt_with(h, h->d_part, h->d_parti, h->d_dtres, cfl, div, err);
+}
+
+int bl_hydro_getdt_noalloc(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) {
+  if (!h || !h->d_zdtnotreg) {
+    set_error("bl_hydro_getdt_noalloc: regions not set");
+    return BL_FAILURE;
+  }
+  if (h->sz.nel > BL_NOALLOC_MAX_EL) {
+    set_error("bl_hydro_getdt_noalloc: " + std::to_string(h->sz.nel) +
+              " elements exceed the workspace maximum of " +
+              std::to_string(BL_NOALLOC_MAX_EL) + " (use bl_hydro_getdt)");
+    return BL_FAILURE;
+  }
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != h->device) {
+    set_error("bl_hydro_getdt_noalloc: the handle's device is not the current device");
+    return BL_FAILURE;
+  }
+  unsigned char* ws = noalloc_scratch(h->device);
+  if (!ws) return BL_FAILURE;
+  return getdt_with(h, reinterpret_cast<double*>(ws + kNaPart),
+                    reinterpret_cast<int32_t*>(ws + kNaParti), reinterpret_cast<DtResult*>(ws),
+                    cfl, div, err);
 }
 
 int bl_hydro_setpredictor(bl_handle_t* h) {

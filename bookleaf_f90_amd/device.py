@@ -156,6 +156,16 @@ class DriverState:
                     "hydro_getdt", err)
         return cfl, div
 
+    def hydro_getdt_noalloc(self):
+        """hydro_getdt through bl_hydro_getdt_noalloc: the same kernels and
+        results, with no device allocation in the call (meshes up to
+        abi.NOALLOC_MAX_EL elements)."""
+        cfl, div, err = abi.Dt(), abi.Dt(), abi.Error()
+        self._check(self._f("hydro_getdt_noalloc")(self.h, C.byref(cfl), C.byref(div),
+                                                   C.byref(err)),
+                    "hydro_getdt_noalloc", err)
+        return cfl, div
+
     def hydro_setpredictor(self):
         self._check(self._f("hydro_setpredictor")(self.h), "hydro_setpredictor")
 

@@ -177,6 +177,15 @@ MODULE bookleaf_hip_mod
       TYPE(bl_dt_t),   INTENT(OUT)   :: cfl,div
       TYPE(bl_error_t),INTENT(OUT)   :: err
     END FUNCTION bl_hydro_getdt
+    ! the same with no device allocation in the call (meshes of at most
+    ! 2**24 elements; calls that overlap on one GPU race)
+    INTEGER(KIND=C_INT32_T) FUNCTION bl_hydro_getdt_noalloc(h,cfl,div,err)      &
+&                                                   BIND(C,NAME="bl_hydro_getdt_noalloc")
+      IMPORT :: C_INT32_T,C_PTR,bl_dt_t,bl_error_t
+      TYPE(C_PTR),VALUE              :: h
+      TYPE(bl_dt_t),   INTENT(OUT)   :: cfl,div
+      TYPE(bl_error_t),INTENT(OUT)   :: err
+    END FUNCTION bl_hydro_getdt_noalloc
     INTEGER(KIND=C_INT32_T) FUNCTION bl_hydro_setpredictor(h)                   &
 &                                                   BIND(C,NAME="bl_hydro_setpredictor")
       IMPORT :: C_INT32_T,C_PTR

@@ -183,6 +183,21 @@ int bl_hydro_init(bl_handle_t* h);
  * edge viscosity, cs2 += visc, CFL and divergence controls. */
 int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div,
                    bl_error_t* err);
+/* bl_hydro_getdt without the handle's reduction buffers: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc): the call allocates and frees
+ * nothing.  Same kernels, same results and the same fields written as
+ * bl_hydro_getdt; its partial minima/maxima and their result live in a
+ * library-wide device array (one per GPU, about 2.3 MiB, held until the
+ * process ends) whose address bl_create looks up.  Limit: a mesh of at most
+ * BL_NOALLOC_MAX_EL elements, else BL_FAILURE with the reason in
+ * bl_last_error; the handle's device must be the current device.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (two host
+ * threads, two handles' streams) race and must be serialised by the
+ * caller. */
+#define BL_NOALLOC_MAX_EL (1 << 24)
+int bl_hydro_getdt_noalloc(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div,
+                           bl_error_t* err);
 /* hydro_dr_setpredictor / setcorrector (hydro_dr_set.f90:31-51) */
 int bl_hydro_setpredictor(bl_handle_t* h);
 int bl_hydro_setcorrector(bl_handle_t* h);

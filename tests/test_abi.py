@@ -33,6 +33,16 @@ def test_header_declares_the_driver_boundary():
         assert n in names
 
 
+def test_noalloc_limit_matches_the_python_mirror():
+    """BL_NOALLOC_MAX_EL (the largest mesh of bl_hydro_getdt_noalloc) and its
+    mirror in abi.py agree, and the limit is whole workgroups of 256."""
+    from bookleaf_f90_amd import abi
+    m = re.search(r"#define BL_NOALLOC_MAX_EL \(1 << (\d+)\)", open(HEADER).read())
+    assert m and (1 << int(m.group(1))) == abi.NOALLOC_MAX_EL
+    assert abi.NOALLOC_MAX_EL % 256 == 0
+    assert "bl_hydro_getdt_noalloc" in declared_functions()
+
+
 def test_library_exports_every_declared_symbol(libpath):
     lib = ctypes.CDLL(libpath)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
