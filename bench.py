@@ -120,12 +120,55 @@ MOVING_BYTES = {"f2_force": 64}
 FOLD_BYTES = {"f4_state": (0.125, 16)}
 
 
-def kernel_bytes(k, nel, nnd, moving, zeroed, fold=False):
+# The tile step (a pure-cell Lagrangian step: F2, F3 and F4 per tile of
+# nodes, bl_tiles.hip), priced from bl_get_tiling's counts: nent entries
+# (an element around a tile's nodes, once per tile that holds one of its
+# nodes), nbnd boundary elements (nodes in two or more tiles).
+#  lag_tile: per entry its element id and node slots 8, node ids 16; per
+#    element once its at-rest byte 1, p, m, e 24, ielmat, ielreg 8, cnmass
+#    32, and the next at-rest byte written 1; per interior element rho, e, p,
+#    cs2 written 32; per boundary element its corner forces written 64 (for
+#    f4_boundary) | per node the tile's node list 4, x, y, u, v 32, indtype
+#    4; x, y, u, v, ubar, vbar written 48; per element in motion cnviscx/y
+#    64; per element whose zero viscosity it writes 72.  Halo entries read
+#    their elements' values a second time through the cache: not counted.
+#  f4_boundary: per boundary element its id 4, ielnd 16, corner forces 64,
+#    m, e 16, ielmat 4, x, y, u, v, ubar, vbar of its nodes 96 (a strip of
+#    elements, each node shared by two), rho, e, p, cs2 and the at-rest byte
+#    written 33.
+#  f1_seg (F1 of a run call's first step, once per call): per entry 8 and per
+#    element ielnd 16, cs2 8, the at-rest byte written 1 | x, y, u, v 32.
+TILE_KERNELS = ("lag_tile", "f4_boundary", "f1_seg")
+
+
+def tile_bytes(k, nel, nnd, moving, zeroed, tiles):
+    _, nbnd, nent = tiles
+    if k == "lag_tile":
+        return (24 * nent + 66 * nel + 32 * (nel - nbnd) + 64 * nbnd + 88 * nnd +
+                64 * moving + 72 * zeroed)
+    if k == "f4_boundary":
+        return 233 * nbnd
+    # two launches a call (the tiles' interior elements, then the boundary
+    # list): half of the bytes each on average, as the timing averages both
+    return (8 * nent + 25 * nel + 32 * nnd) / 2
+
+
+# before a remap F4 also writes the corner weights and the volume at x^{n+1}
+# (cnwt 32, elvolume 8), which the remap reads
+REMAP_F4_BYTES = 40
+
+
+def kernel_bytes(k, nel, nnd, moving, zeroed, fold=False, tiles=None, remap=False):
     """algorithmic bytes of one launch of kernel k; moving and zeroed are the
-    per-step averages of bl_run_stats"""
+    per-step averages of bl_run_stats; tiles = (tiles, boundary elements,
+    entries) of the tile step; remap: the step ends with the remap"""
+    if k in TILE_KERNELS:
+        return tile_bytes(k, nel, nnd, moving, zeroed, tiles)
     if k == "f1_limiter":
         return LIMITER_BYTES[0] * moving + LIMITER_BYTES[1] * nnd * min(1.0, moving / nel)
     be, bn = KERNEL_BYTES[k]
+    if remap and k == "f4_state":
+        be += REMAP_F4_BYTES
     if fold:
         fe, fn = FOLD_BYTES.get(k, (0, 0))
         be, bn = be + fe, bn + fn
@@ -372,13 +415,20 @@ def main():
     moving, zeroed = mv / max(args.profile_steps, 1), zr / max(args.profile_steps, 1)
     # F1 folded into F4: F1 launched once per run call, not once per step
     f1name = "e1_getdt" if "e1_getdt" in kern else "f1_getdt"
-    fold = (f1name in kern and "f4_state" in kern and
-            kern[f1name]["launches"] < kern["f4_state"]["launches"])
-    dom = max((k for k in kern if k in KERNEL_BYTES), key=lambda k: kern[k]["total_ms"],
+    tiles = None
+    if "lag_tile" in kern:
+        on, ntile, nbnd, nent = g.get_tiling()
+        tiles = (ntile, nbnd, nent)
+    # (the tile step always folds F1 into it: f1_seg runs once per call)
+    fold = tiles is not None or (f1name in kern and "f4_state" in kern and
+                                 kern[f1name]["launches"] < kern["f4_state"]["launches"])
+    priced = set(KERNEL_BYTES) | (set(TILE_KERNELS) if tiles else set())
+    remap = "r1_volflux" in kern
+    dom = max((k for k in kern if k in priced), key=lambda k: kern[k]["total_ms"],
               default=None)
     roof = None
     if dom:
-        alg = kernel_bytes(dom, nel, nnd, moving, zeroed, fold)
+        alg = kernel_bytes(dom, nel, nnd, moving, zeroed, fold, tiles, remap)
         avg_s = kern[dom]["avg_ms"] * 1e-3
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
@@ -388,8 +438,8 @@ def main():
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg / avg_s / HBM_PEAK,
                 "traffic": traffic, "algorithmic_bytes": alg, "avg_ms": kern[dom]["avg_ms"]}
         for k in kern:
-            if k in KERNEL_BYTES:
-                kern[k]["alg_GBps"] = (kernel_bytes(k, nel, nnd, moving, zeroed, fold) /
+            if k in priced:
+                kern[k]["alg_GBps"] = (kernel_bytes(k, nel, nnd, moving, zeroed, fold, tiles, remap) /
                                        (kern[k]["avg_ms"] * 1e-3) / 1e9)
     for _ in range(args.calibrate_copy):
         g.hydro_setpredictor()  # k_copy_r: reads and writes 8*nel bytes
@@ -407,9 +457,9 @@ def main():
                 else DRIVER_BOUNDARY_EUL)
     step_bytes = dbe * nel + dbn * nnd
     # algorithmic bytes of the kernels one step launches (from the profile pass)
-    fused_bytes = sum(kernel_bytes(k, nel, nnd, moving, zeroed, fold) *
+    fused_bytes = sum(kernel_bytes(k, nel, nnd, moving, zeroed, fold, tiles, remap) *
                       kern[k]["launches"] / max(args.profile_steps, 1)
-                      for k in kern if k in KERNEL_BYTES)
+                      for k in kern if k in priced)
     out = {
         "metric": "element-steps/s FP64 at 1M cells, %HBM peak; learned vs plain ALE at "
                   "equal L1(rho)",
@@ -426,6 +476,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": wname, "nel": nel, "nnd": nnd, "runner": args.runner,
                    "diagnostics": bool(args.diagnostics),
+                   "tile_step": (None if tiles is None else
+                                 {"tiles": tiles[0], "boundary_elements": tiles[1],
+                                  "entries": tiles[2]}),
                    "launch": tune if tune is not None else g.get_tuning()},
         "roofline": roof,
         "cpu_baseline": cpu,

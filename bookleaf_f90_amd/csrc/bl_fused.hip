@@ -1720,8 +1720,8 @@ int launch_f1_prologue(bl_handle_t* h, int p, int stop_at_time_end) {
     tile_args(h, A, p);
     A.rest8_next = h->d_rest8 + (size_t)p * h->sz.nel;  // the first step's own bytes
     FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
-    BL_LAUNCH(h, "f1_getdt", k_f1_seg, (unsigned)h->ntile, kBlock, A, N, 0);
-    if (h->nbnd > 0) BL_LAUNCH(h, "f1_getdt", k_f1_seg, nblocks(h->nbnd), kBlock, A, N, 1);
+    BL_LAUNCH(h, "f1_seg", k_f1_seg, (unsigned)h->ntile, kBlock, A, N, 0);
+    if (h->nbnd > 0) BL_LAUNCH(h, "f1_seg", k_f1_seg, nblocks(h->nbnd), kBlock, A, N, 1);
     return BL_SUCCESS;
   }
   uint64_t* m[2];
