@@ -8,6 +8,16 @@
 
 namespace bl {
 
+// The run call's counters (bl_run_stats): the elements the limiter pass
+// listed at vstats[0], and the count of elements whose zero viscosity a
+// step wrote spread over kStatSlots counters one 128-byte line apart
+// (vstats[kStatStride * (1 + s)]): a wave adds to its workgroup's slot, so
+// that a step where every wave writes zeros does not queue all its atomics
+// on one address.
+constexpr int kStatSlots = 64;
+constexpr int kStatStride = 16;
+constexpr int kStatWords = kStatStride * (1 + kStatSlots);
+
 struct FusedNodes {
   const double* x;
   const double* y;
@@ -92,7 +102,7 @@ struct FusedArgs {
   // bit e of rest[e / 64]: element e is at rest this step and its cnviscx/y
   // and elvisc hold the +0.0 F1 wrote (F1 sets it; F2 reads it)
   uint64_t* rest;
-  unsigned long long* vstats;  // [0] worklist elements, [1] elements F1 zeroed
+  unsigned long long* vstats;  // kStatWords counters (kStatSlots above)
   // F1 folded into F4 (a Lagrangian step with pure cells): F4 does F1's work
   // for the next step at x^{n+1}, u^{n+1} and writes rest_next; F2 writes
   // the zeros of elements at rest now (rest) but not in the step before

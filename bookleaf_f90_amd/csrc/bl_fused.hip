@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <vector>
 
 #include "bl_fused.h"
 #include "bl_ale.h"
@@ -55,6 +56,13 @@ namespace bl {
 // F1b then records it in ts->skip for the kernels that follow.
 __device__ __forceinline__ bool step_skipped(const FusedArgs& A, const DevTimestep* ts) {
   return ts->done || (A.stop_at_time_end && ts->time >= A.time_end);
+}
+
+// a wave's count of elements whose zero viscosity it wrote, to its
+// workgroup's counter slot
+__device__ __forceinline__ void count_zeroed(const FusedArgs& A, uint64_t wrote) {
+  atomicAdd(A.vstats + kStatStride * (1 + (blockIdx.x & (kStatSlots - 1))),
+            (unsigned long long)__popcll(wrote));
 }
 
 __device__ __forceinline__ void load4(const double* __restrict__ a, const int nd[4],
@@ -170,7 +178,7 @@ __device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane,
   const uint64_t wrote = __ballot(zeroed);
   if (lane == 0) {
     if (now != prev) A.rest[e >> 6] = now;
-    if (wrote) atomicAdd(A.vstats + 1, (unsigned long long)__popcll(wrote));
+    if (wrote) count_zeroed(A, wrote);
   }
 }
 
@@ -837,7 +845,7 @@ k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
     // lanes past nel left at the start, and they are the wave's last lanes
     const uint64_t wrote = __ballot(zeroed);
     if ((threadIdx.x & 63) == 0 && wrote)
-      atomicAdd(A.vstats + 1, (unsigned long long)__popcll(wrote));
+      count_zeroed(A, wrote);
   }
 }
 
@@ -1107,7 +1115,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
     }
   }
   const uint64_t wrote = __ballot(zeroed);
-  if ((tid & 63) == 0 && wrote) atomicAdd(A.vstats + 1, (unsigned long long)__popcll(wrote));
+  if ((tid & 63) == 0 && wrote) count_zeroed(A, wrote);
   __syncthreads();
   // --- F3: the tile's nodes, their corners in jj order
   if (own) {
@@ -1822,9 +1830,9 @@ int ensure_nodesets(bl_handle_t* h) {
     // twice (the current and the previous step's with F1 folded into F4)
     const size_t words = 8 * (size_t)nblocks(std::max(h->sz.nel, 1));
     BL_HIP(hipMalloc(&h->d_rest, sizeof(uint64_t) * words));
-    BL_HIP(hipMalloc(&h->d_vstats, sizeof(unsigned long long) * 2));
+    BL_HIP(hipMalloc(&h->d_vstats, sizeof(unsigned long long) * kStatWords));
     BL_HIP(hipMemset(h->d_rest, 0, sizeof(uint64_t) * words));
-    BL_HIP(hipMemset(h->d_vstats, 0, sizeof(unsigned long long) * 2));
+    BL_HIP(hipMemset(h->d_vstats, 0, sizeof(unsigned long long) * kStatWords));
   }
   if (!h->cterm[0])
     for (int k = 0; k < 4; ++k)
@@ -1847,20 +1855,43 @@ int ensure_nodesets(bl_handle_t* h) {
 // every run call, since between calls the host may have written them (an
 // upload, a per-driver call, a write through bl_field_device_ptr).  The
 // first step of the call writes every element's.  Outside any capture.
-// Also empties the worklist.
-int clear_rest(bl_handle_t* h) {
-  // the negative-volume flags the last call's last step may have left (F1L
-  // and F1b read them as the step before's outcome)
-  BL_HIP(hipMemsetAsync(h->d_dtres, 0, sizeof(DtResult), h->stream));
-  BL_HIP(hipMemsetAsync(h->d_rest, 0, sizeof(uint64_t) * 8 * (size_t)nblocks(std::max(h->sz.nel, 1)),
-                        h->stream));
-  BL_HIP(hipMemsetAsync(h->d_vstats, 0, sizeof(unsigned long long) * 2, h->stream));
-  if (h->d_rest8)
-    BL_HIP(hipMemsetAsync(h->d_rest8, 0, 2 * (size_t)std::max(h->sz.nel, 1), h->stream));
-  // an ensemble's worklist lengths: a folded run call ends with the next
-  // step's lists
-  if (h->ens_n > 0 && h->d_cnt)
-    BL_HIP(hipMemsetAsync(h->d_cnt, 0, sizeof(uint32_t) * h->ens_n, h->stream));
+// Everything a run call starts from, in one launch: the at-rest masks (bits
+// and bytes), the counters, the negative-volume flags the last call's last
+// step may have left (F1L and F1b read them as the step before's outcome),
+// an ensemble's worklist lengths (a folded run call ends with the next
+// step's lists) and, with set_ts, the device timestep.
+__global__ void k_call_start(DevTimestep init, int set_ts, DevTimestep* __restrict__ ts,
+                             DtResult* __restrict__ dtres, uint64_t* __restrict__ rest,
+                             int64_t nrest, unsigned long long* __restrict__ vstats,
+                             uint8_t* __restrict__ rest8, int64_t nrest8,
+                             uint32_t* __restrict__ cnt, int ncnt) {
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (i0 == 0) {
+    *dtres = DtResult{};
+    if (set_ts) *ts = init;
+  }
+  for (int64_t i = i0; i < nrest; i += stride) rest[i] = 0;
+  for (int64_t i = i0; i < kStatWords; i += stride) vstats[i] = 0;
+  for (int64_t i = i0; i < ncnt; i += stride) cnt[i] = 0;
+  // the bytes: whole 8-byte words (hipMalloc'd, so aligned), then the tail
+  uint64_t* r8 = reinterpret_cast<uint64_t*>(rest8);
+  for (int64_t i = i0; i < nrest8 / 8; i += stride) r8[i] = 0;
+  for (int64_t i = (nrest8 / 8) * 8 + i0; i < nrest8; i += stride) rest8[i] = 0;
+}
+
+// Forget which elements' viscosity fields hold F1's zeros (and the rest of
+// what k_call_start clears); with init, also write the device timestep
+int clear_rest(bl_handle_t* h, const DevTimestep* init = nullptr) {
+  const int64_t nrest = 8 * (int64_t)nblocks(std::max(h->sz.nel, 1));
+  const int64_t nrest8 = h->d_rest8 ? 2 * (int64_t)std::max(h->sz.nel, 1) : 0;
+  const int ncnt = (h->ens_n > 0 && h->d_cnt) ? h->ens_n : 0;
+  const int64_t most = std::max(std::max(nrest, nrest8 / 8), (int64_t)std::max(kStatWords, ncnt));
+  const unsigned grid = (unsigned)std::min<int64_t>(1024, (most + kBlock - 1) / kBlock);
+  DevTimestep none{};
+  BL_LAUNCH(h, "call_start", k_call_start, std::max(grid, 1u), kBlock, init ? *init : none,
+            init ? 1 : 0, h->d_ts, h->d_dtres, h->d_rest, nrest, h->d_vstats, h->d_rest8,
+            nrest8, h->d_cnt, ncnt);
   return BL_SUCCESS;
 }
 
@@ -1925,15 +1956,25 @@ int build_nbfc(bl_handle_t* h) {
   return BL_SUCCESS;
 }
 
+// Steps per multi-step graph: consecutive graph launches leave a gap of
+// several microseconds on the device between them, kernels inside one graph
+// none, so a run launches kGraphSteps steps at a time (an even count: the
+// node set parity is the same after them).
+constexpr int kGraphSteps = 8;
+
 int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
   invalidate_graph(h);
   if (h->cfg.zexist && (h->nbfc_stale || !h->d_nbfc) && build_nbfc(h)) return BL_FAILURE;
   bool timing = h->timing;
   h->timing = false;  // no event records inside a capture
-  for (int p = 0; p < 2; ++p) {
+  for (int q = 0; q < 4; ++q) {
+    // q < 2: one step from node set q; q >= 2: kGraphSteps from set q - 2
+    const int p = q & 1, steps = q < 2 ? 1 : kGraphSteps;
     hipGraph_t g = nullptr;
     BL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_step(h, p, stop_at_time_end);
+    int rc = BL_SUCCESS;
+    for (int k = 0; k < steps && rc == BL_SUCCESS; ++k)
+      rc = launch_step(h, p ^ (k & 1), stop_at_time_end);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc != BL_SUCCESS || e != hipSuccess) {
       if (g) hipGraphDestroy(g);
@@ -1941,7 +1982,7 @@ int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
       set_error("step graph capture failed");
       return BL_FAILURE;
     }
-    e = hipGraphInstantiate(&h->graph_exec[p], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&h->graph_exec[q], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) {
       h->timing = timing;
@@ -1954,12 +1995,25 @@ int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
   return BL_SUCCESS;
 }
 
-// the registry's geometry fields after a Lagrangian batch: F4 keeps them in
+// The end of a batch whose last step left its end open (no F1b after it),
+// on the device, so that the host reads the timestep once after all the
+// call's work
+__global__ void k_batch_end(DevTimestep* ts, const DtResult* res) {
+  if (threadIdx.x == 0) step_end(ts, res);
+}
+
+// The registry's geometry fields after a Lagrangian batch: F4 keeps them in
 // registers, so they are rebuilt once from the final coordinates; the last
 // step's corrector corner forces go back to the element-major rscratch23/24
-// (fxT null: the tile step's last step wrote them there itself)
+// from the corner-major copy fxT, fyT (fxT null: the tile step's last step
+// wrote them there itself).  The final node set is (cur + ts->flips) & 1 of
+// (x0, y0), (x1, y1): read on the device, after the batch; with ts null it
+// is set cur.  (Writing the geometry in the batch's last tile step instead
+// slowed every step's tile kernels by more than this pass costs once.)
 __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
-                              const double* __restrict__ ndx, const double* __restrict__ ndy,
+                              const double* __restrict__ x0, const double* __restrict__ y0,
+                              const double* __restrict__ x1, const double* __restrict__ y1,
+                              const DevTimestep* __restrict__ ts, int cur,
                               double* __restrict__ cnx, double* __restrict__ cny,
                               double* __restrict__ a1, double* __restrict__ a2,
                               double* __restrict__ a3, double* __restrict__ b1,
@@ -1969,6 +2023,9 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
                               double* __restrict__ cnfx, double* __restrict__ cnfy) {
   int e = xcd_index();
   if (e >= nel) return;
+  const bool other = ((ts ? cur + ts->flips : cur) & 1) != 0;
+  const double* __restrict__ ndx = other ? x1 : x0;
+  const double* __restrict__ ndy = other ? y1 : y0;
   if (fxT) {
     double fx[4], fy[4];
 #pragma unroll
@@ -2157,7 +2214,9 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   {
     int nel = h->sz.nel;
     BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
-              h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
+              h->nodeset[0][0], h->nodeset[0][1], h->nodeset[1][0], h->nodeset[1][1],
+              (const DevTimestep*)nullptr, h->cur_set, h->R(BL_CNX), h->R(BL_CNY),
+              h->R(BL_A1),
               h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
               h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
               h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
@@ -2366,13 +2425,16 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
 // counters of the last run call (bl_run_graph or bl_ensemble_run)
 int bl_run_stats(bl_handle_t* h, int64_t* moving, int64_t* zeroed) {
   if (!h) return BL_FAILURE;
-  unsigned long long c[2] = {0, 0};
+  std::vector<unsigned long long> c(kStatWords, 0ull);
   if (h->d_vstats) {
-    BL_HIP(hipMemcpyAsync(c, h->d_vstats, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    BL_HIP(hipMemcpyAsync(c.data(), h->d_vstats, sizeof(unsigned long long) * kStatWords,
+                          hipMemcpyDeviceToHost, h->stream));
     BL_HIP(hipStreamSynchronize(h->stream));
   }
+  unsigned long long z = 0;
+  for (int s = 0; s < kStatSlots; ++s) z += c[kStatStride * (1 + s)];
   if (moving) *moving = (int64_t)c[0];
-  if (zeroed) *zeroed = (int64_t)c[1];
+  if (zeroed) *zeroed = (int64_t)z;
   return BL_SUCCESS;
 }
 
@@ -2464,7 +2526,7 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     return bl_run(h, nsteps, stop_at_time_end, ts, dt_hist, idt_hist, ctl_hist, steps_taken,
                   err);
   fill_err(err, BL_SUCCESS, 0, "");
-  if (ensure_nodesets(h) || prepare_tiles(h) || clear_rest(h)) return BL_FAILURE;
+  if (ensure_nodesets(h) || prepare_tiles(h)) return BL_FAILURE;
   if (h->hist_cap < nsteps) {
     // the graphs bake in the history's capacity: size it generously so that
     // runs of different lengths do not re-capture
@@ -2483,58 +2545,59 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   d->time = ts->time; d->dt = ts->dt; d->dts = ts->dts;
   d->nstep = ts->nstep; d->idtel = ts->idtel; d->sdt = ts->sdt;
   d->last_step = ts->nstep + nsteps;
-  BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
+  if (clear_rest(h, d)) return BL_FAILURE;
   const bool tiles = use_tiles(h);
   int p = h->cur_set;
   if (fold_f1(h) && launch_f1_prologue(h, p, stop_at_time_end)) return BL_FAILURE;
-  for (int k = 0; k < nsteps; ++k) {
+  for (int k = 0; k < nsteps;) {
     // with kernel timing on, the same step kernels are launched directly so
     // each launch can be bracketed by HIP events on this stream
     if (h->timing) {
       if (launch_step(h, p, stop_at_time_end)) return BL_FAILURE;
+    } else if (nsteps - k >= kGraphSteps) {
+      BL_HIP(hipGraphLaunch(h->graph_exec[2 + p], h->stream));
+      k += kGraphSteps;  // even: the parity after them is p again
+      continue;
     } else {
       BL_HIP(hipGraphLaunch(h->graph_exec[p], h->stream));
     }
     p ^= 1;
+    ++k;
   }
-  BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
-  BL_HIP(hipMemcpyAsync(h->h_dtres, h->d_dtres, sizeof(DtResult), hipMemcpyDeviceToHost,
-                        h->stream));
-  BL_HIP(hipStreamSynchronize(h->stream));
-  if (d->open) {
-    // the batch's last step ended with no F1b after it
-    step_end(d, h->h_dtres);
-    BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
-  }
-  // steps that selected a dt (a failing step included)
-  const int taken = d->hist_n;
-  // the node set holding the latest state: every step whose F3 ran wrote the
-  // other set
-  point_fields_at(h, (h->cur_set + d->flips) & 1);
-  if (tiles) point_elements_at(h, h->cur_set);
+  // everything after the batch is queued behind it with no host round trip:
+  // the open end of the last step, the geometry at the final node set (every
+  // step whose F3 ran wrote the other set; the device knows how many did),
+  // then one read of the timestep, the dt history (at most one entry per
+  // step) and the error state
+  if (end_in_f1b(h))
+    BL_LAUNCH(h, "batch_end", k_batch_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
   {
     // (the tile step's last step wrote rscratch23/24 itself)
     int nel = h->sz.nel;
     BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
-              h->R(BL_NDX), h->R(BL_NDY), h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1),
-              h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
-              h->R(BL_ELVOLUME), tiles ? nullptr : (const double*)h->cterm[0],
+              h->nodeset[0][0], h->nodeset[0][1], h->nodeset[1][0], h->nodeset[1][1],
+              (const DevTimestep*)h->d_ts, h->cur_set, h->R(BL_CNX), h->R(BL_CNY),
+              h->R(BL_A1), h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3),
+              h->R(BL_CNWT), h->R(BL_ELVOLUME), tiles ? nullptr : (const double*)h->cterm[0],
               tiles ? nullptr : (const double*)h->cterm[1], h->R(BL_RSCRATCH23),
               h->R(BL_RSCRATCH24));
     if (!h->cfg.zexist && mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
   }
-  if (taken > 0) {
-    std::vector<double> hist(3 * (size_t)taken);
-    BL_HIP(hipMemcpyAsync(hist.data(), h->d_hist, sizeof(double) * 3 * taken,
+  std::vector<double> hist(3 * (size_t)std::max(nsteps, 1));
+  BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
+  if (nsteps > 0)
+    BL_HIP(hipMemcpyAsync(hist.data(), h->d_hist, sizeof(double) * 3 * (size_t)nsteps,
                           hipMemcpyDeviceToHost, h->stream));
-    BL_HIP(hipStreamSynchronize(h->stream));
-    for (int k = 0; k < taken; ++k) {
-      if (dt_hist) dt_hist[k] = hist[3 * k];
-      if (idt_hist) idt_hist[k] = (int32_t)hist[3 * k + 1];
-      if (ctl_hist) ctl_hist[k] = (int32_t)hist[3 * k + 2];
-    }
-  }
   BL_HIP(hipStreamSynchronize(h->stream));
+  // steps that selected a dt (a failing step included)
+  const int taken = d->hist_n;
+  for (int k = 0; k < taken; ++k) {
+    if (dt_hist) dt_hist[k] = hist[3 * k];
+    if (idt_hist) idt_hist[k] = (int32_t)hist[3 * k + 1];
+    if (ctl_hist) ctl_hist[k] = (int32_t)hist[3 * k + 2];
+  }
+  point_fields_at(h, (h->cur_set + d->flips) & 1);
+  if (tiles) point_elements_at(h, h->cur_set);
   ts->time = d->time; ts->dt = d->dt; ts->dts = d->dts;
   ts->nstep = d->nstep; ts->idtel = d->idtel; ts->sdt = d->sdt;
   ts->zcorrector = 1;

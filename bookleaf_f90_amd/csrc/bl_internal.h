@@ -139,7 +139,9 @@ struct Handle {
   // remap temporaries of the fused step (zexist): 4*nel each for the face
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[13] = {};
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step per parity
+  // [p]: one step from node set p; [2 + p]: kGraphSteps steps from node set
+  // p (bl_run_graph launches those while at least that many steps remain)
+  hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   // ensembles (bl_ensemble_create): instances stepped together
   int32_t ens_n = 0;
   std::vector<int32_t> ens_eoff, ens_noff;  // host copies, ens_n + 1 each
