@@ -503,7 +503,10 @@ def main():
         s_, gd, fb, rs = g.learned_stats(full=True)
         out["learned"] = {"training": train_info, "timed_run": {"guarded_steps": gd,
                                                               "fallback_steps": fb},
-                          "equal_time": None if args.no_compare else L.compare(net)}
+                          "equal_time": None if args.no_compare else L.compare(net),
+                          # plain EQUI (om 0.25) at 1..8 sweeps and the network
+                          # against the 8-sweep run: time and L1 of each
+                          "pareto": None if args.no_compare else L.pareto(net)}
     print(json.dumps(out))
 
 

@@ -63,27 +63,38 @@ def l1(st, ref):
     return out
 
 
-def run_to(mode, net=None, tend=T_END, nx=1024, ny=512):
-    """One run from t = 0 to tend: mode "plain" (the plain relaxation),
-    "learned" (net) or "one_sweep".  Returns (final fields, info); info has
-    "error" when the run stopped on one."""
+def run_to(mode, net=None, tend=T_END, nx=1024, ny=512, nsweep=None, reps=1):
+    """One run from t = 0 to tend: mode "plain" (the plain relaxation, with
+    nsweep sweeps in place of the method's 8 when given), "learned" (net) or
+    "one_sweep".  With reps > 1 the run is made reps times from the same
+    start and "seconds" is the median wall time (every repeat gives the same
+    bits).  Returns (final fields, info); info has "error" when the run
+    stopped on one."""
     from .device import BookLeaf, BookLeafError
     p = relax_problem(1_000_000, nx=nx, ny=ny, tend=tend,
-                      nsweep=1 if mode == "one_sweep" else None)
-    g = BookLeaf(p)
-    if mode == "learned":
-        g.relax_learned(net)
-    info = {}
-    try:
-        g.run(1, stop_at_time_end=True, runner="run_graph")
-        g.synchronize()
-        t0 = time.perf_counter()
-        while g.ts.time < tend:
-            g.run(20, stop_at_time_end=True, runner="run_graph")
-        g.synchronize()
-        info["seconds"] = time.perf_counter() - t0
-    except BookLeafError as e:
-        info["error"] = str(e)
+                      nsweep=1 if mode == "one_sweep" else nsweep)
+    times = []
+    for rep in range(reps):
+        g = BookLeaf(p)
+        if mode == "learned":
+            g.relax_learned(net)
+        info = {}
+        try:
+            g.run(1, stop_at_time_end=True, runner="run_graph")
+            g.synchronize()
+            t0 = time.perf_counter()
+            while g.ts.time < tend:
+                g.run(20, stop_at_time_end=True, runner="run_graph")
+            g.synchronize()
+            times.append(time.perf_counter() - t0)
+        except BookLeafError as e:
+            info["error"] = str(e)
+        if "error" in info or rep == reps - 1:
+            break
+        g.close()
+    if times:
+        info["seconds"] = float(np.median(times))
+        info["seconds_all"] = times
     info.update(steps=int(g.ts.nstep), time=g.ts.time)
     if mode == "learned":
         s_, gd, fb, rs = g.learned_stats(full=True)
@@ -109,4 +120,44 @@ def compare(net, tend=T_END, nx=1024, ny=512, untrained=None):
             info["L1_density"], info["L1_energy"] = l1(st, ref)
             info["speedup_vs_plain"] = plain["seconds"] / info["seconds"]
         out[name] = info
+    return out
+
+
+# the plain settings of the cost/accuracy curve: EQUI at the method's om with
+# these sweep counts, and the accuracy bar for "as good as the method"
+PARETO_SWEEPS = (1, 2, 3, 4, 6, 8)
+PARETO_TOL = 1.0e-3
+
+
+def pareto(net, sweeps=PARETO_SWEEPS, tend=T_END, nx=1024, ny=512, tol=PARETO_TOL, reps=3):
+    """The plain relaxation at each sweep count of sweeps (om 0.25) and the
+    network, each run from t = 0 to tend, with wall time (the median of reps
+    runs) and L1 (density, energy) against the 8-sweep run, the method the
+    network imitates.  Returns {"plain": [rows by sweep count], "learned":
+    row, "cheapest_plain": the sweep count of the fastest plain setting
+    within tol of the 8-sweep run in both L1s, "learned_within_tol",
+    "learned_speedup_vs_cheapest"}."""
+    ref, ref_info = run_to("plain", tend=tend, nx=nx, ny=ny, reps=reps)
+    rows = []
+    for k in sweeps:
+        if k == RELAX[1]:
+            st, info = ref, dict(ref_info)
+        else:
+            st, info = run_to("plain", tend=tend, nx=nx, ny=ny, nsweep=k, reps=reps)
+        info["nsweep"] = k
+        if st is not None and ref is not None:
+            info["L1_density"], info["L1_energy"] = l1(st, ref)
+        rows.append(info)
+    st, lr = run_to("learned", net, tend=tend, nx=nx, ny=ny, reps=reps)
+    if st is not None and ref is not None:
+        lr["L1_density"], lr["L1_energy"] = l1(st, ref)
+    ok = [r for r in rows if "error" not in r and r["L1_density"] <= tol and
+          r["L1_energy"] <= tol]
+    cheapest = min(ok, key=lambda r: r["seconds"]) if ok else None
+    out = {"t_end": tend, "tol": tol, "plain": rows, "learned": lr,
+           "cheapest_plain": cheapest["nsweep"] if cheapest else None,
+           "learned_within_tol": ("error" not in lr and lr.get("L1_density", 1.0) <= tol and
+                                  lr.get("L1_energy", 1.0) <= tol)}
+    if cheapest and "error" not in lr:
+        out["learned_speedup_vs_cheapest"] = cheapest["seconds"] / lr["seconds"]
     return out
