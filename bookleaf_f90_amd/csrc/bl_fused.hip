@@ -58,6 +58,9 @@ __device__ __forceinline__ bool step_skipped(const FusedArgs& A, const DevTimest
   return ts->done || (A.stop_at_time_end && ts->time >= A.time_end);
 }
 
+// the limiter pass's at-rest flags per thread and scan pass (a byte word)
+constexpr int kScanPer = 8;
+
 // a wave's count of elements whose zero viscosity it wrote, to its
 // workgroup's counter slot
 __device__ __forceinline__ void count_zeroed(const FusedArgs& A, uint64_t wrote) {
@@ -325,7 +328,7 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
   // the step before found a negative volume that F1b has not recorded yet:
   // this step does nothing (F1b makes it so)
   if (A.end_in_f1b && (A.dtres->negvol || A.dtres->negvol2)) return;
-  __shared__ int32_t list[kBlock];
+  __shared__ int32_t list[kScanPer * kBlock];
   __shared__ uint32_t wave_n[kBlock / 64];
   double best = DBL_MAX, dv = -1.0;
   int bi = 0x7fffffff, di = 0x7fffffff;
@@ -341,22 +344,51 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
   const int e1 = min(A.nel, (int)(((long long)(lb + 1) * nbe) / gridDim.x) * kBlock);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   uint32_t listed = 0;
-  for (int c = e0; c < e1; c += blockDim.x) {
-    const int e = c + threadIdx.x;
-    bool mv = false;
-    if (e < e1) mv = A.rest8 ? A.rest8[e] == 0 : !((A.rest[e >> 6] >> (e & 63)) & 1ull);
-    const uint64_t mask = __ballot(mv);
-    if (lane == 0) wave_n[wid] = (uint32_t)__popcll(mask);
+  // a pass covers kScanPer elements per thread (from a multiple of 8, so a
+  // thread's flags are one aligned 8-byte word of bytes, or 8 bits of one
+  // 64-bit word) and lists all the pass's moving elements before any
+  // limiter work, so that a range with few of them costs one scan and one
+  // limiter pass
+  const int span = kScanPer * (int)blockDim.x;
+  for (int c = e0; c < e1; c += span) {
+    const int b = c + kScanPer * (int)threadIdx.x;  // this thread's first element
+    uint32_t mv = 0;                                 // bit k: element b + k moves
+    if (b < e1) {
+      if (A.rest8) {
+        if (b + kScanPer <= e1) {
+          const uint64_t w = *reinterpret_cast<const uint64_t*>(A.rest8 + b);
+#pragma unroll
+          for (int k = 0; k < kScanPer; ++k) mv |= (((w >> (8 * k)) & 0xffull) == 0) ? 1u << k : 0u;
+        } else {
+          for (int k = 0; b + k < e1; ++k) mv |= A.rest8[b + k] == 0 ? 1u << k : 0u;
+        }
+      } else {
+        const uint32_t rest = (uint32_t)(A.rest[b >> 6] >> (b & 63)) & 0xffu;
+        mv = ~rest & 0xffu;
+        if (b + kScanPer > e1) mv &= (1u << (e1 - b)) - 1u;
+      }
+    }
+    // the block's exclusive prefix of the counts: within the wave by
+    // shuffles, then over the waves
+    const uint32_t cnt = (uint32_t)__popc(mv);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) wave_n[wid] = incl;
     __syncthreads();
     uint32_t base = 0, total = 0;
     for (int w = 0; w < nw; ++w) {
       base += w < wid ? wave_n[w] : 0u;
       total += wave_n[w];
     }
-    if (mv) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = e;
+    uint32_t at = base + incl - cnt;
+    for (uint32_t m = mv; m; m &= m - 1) list[at++] = b + __ffs(m) - 1;
     __syncthreads();
-    if (threadIdx.x < total) {
-      const int el = list[threadIdx.x];
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+      const int el = list[i];
       better_min(best, bi, f1_moving(A, N, el), el + 1);
     }
     listed += total;
