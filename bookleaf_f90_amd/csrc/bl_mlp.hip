@@ -308,8 +308,9 @@ __global__ void __launch_bounds__(256) k_mlp_forward(MlpPacked W, const float* _
 }
 
 // The learned mesh velocity of a step: one relaxed node per lane (features
-// in FP32 on FP64 offsets, standardised), 256 nodes per workgroup, the
-// network staged in LDS and run on the matrix cores, w = h t / dt with t =
+// in FP32 on FP64 offsets, standardised), the relaxed nodes in nchunk chunks
+// of 256, workgroup b < nbm taking chunks b, b + nbm, ... with the network
+// staged in its LDS once and run on the matrix cores, w = h t / dt with t =
 // f[24..25] + the network's difference, the displacement / h; the one-sweep
 // displacement h f[24..25] into d1 (the check's reference and the guard's
 // motion) and the node's guard mark cleared.  Workgroups from nbm set w and
@@ -317,11 +318,14 @@ __global__ void __launch_bounds__(256) k_mlp_forward(MlpPacked W, const float* _
 // step counters: the previous step's fallback flag[0] and guard flag[4] are
 // added to their counts flag[2] and flag[5] and cleared, flag[1] counts
 // steps.
+constexpr int kMlpBlocks = 1024;  // four per CU, all resident at once
+
 __global__ void __launch_bounds__(256) k_mlp_nodes(MlpPacked W, const float* __restrict__ norm,
                                                    RelaxArgs RA,
                                                    const int32_t* __restrict__ rlist, int nr,
                                                    const int32_t* __restrict__ zlist, int nz,
-                                                   int nbm, const double* __restrict__ x,
+                                                   int nchunk, int nbm,
+                                                   const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    double* __restrict__ wu,
                                                    double* __restrict__ wv, double dt,
@@ -352,24 +356,27 @@ __global__ void __launch_bounds__(256) k_mlp_nodes(MlpPacked W, const float* __r
   __shared__ uint64_t xs[4][64 * (kInPad / 4)];
   __shared__ float2 ys[4][64];
   const int wave = threadIdx.x >> 6;
+  // staged once, for every chunk of 256 relaxed nodes this workgroup takes
   stage_network(W, S);
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  const int node = r < nr ? rlist[r] : -1;
-  float f[kIn];
-  float h = 0.0f;
-  if (node >= 0) {
-    node_features(RA, x, y, node, f, h);
-  } else {
+  const double idt = 1.0 / dt;
+  for (int chunk = blockIdx.x; chunk < nchunk; chunk += nbm) {
+    const int r = chunk * 256 + threadIdx.x;
+    const int node = r < nr ? rlist[r] : -1;
+    float f[kIn];
+    float h = 0.0f;
+    if (node >= 0) {
+      node_features(RA, x, y, node, f, h);
+    } else {
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) f[k] = 0.0f;
-  }
-  const float2 t = mlp_rows64(S, norm, f, xs[wave], ys[wave]);
-  if (node >= 0) {
-    const double idt = 1.0 / dt;
-    mark[node] = 0;
-    d1[node] = make_float2(h * f[24], h * f[25]);
-    wu[node] = (double)h * (double)(f[24] + norm_out(norm, 0, t.x)) * idt;
-    wv[node] = (double)h * (double)(f[25] + norm_out(norm, 1, t.y)) * idt;
+      for (int k = 0; k < kIn; ++k) f[k] = 0.0f;
+    }
+    const float2 t = mlp_rows64(S, norm, f, xs[wave], ys[wave]);
+    if (node >= 0) {
+      mark[node] = 0;
+      d1[node] = make_float2(h * f[24], h * f[25]);
+      wu[node] = (double)h * (double)(f[24] + norm_out(norm, 0, t.x)) * idt;
+      wv[node] = (double)h * (double)(f[25] + norm_out(norm, 1, t.y)) * idt;
+    }
   }
 }
 
@@ -402,14 +409,28 @@ __device__ __forceinline__ void check_element(int e, const int32_t* __restrict__
                                               const float2* __restrict__ d1, double dt,
                                               double amax2, int relative, int stage,
                                               int32_t* __restrict__ mark,
-                                              int32_t* __restrict__ flag) {
+                                              int32_t* __restrict__ flag,
+                                              bool guarded = false) {
   const int4 q = reinterpret_cast<const int4*>(ielnd)[e];
   const int nd[4] = {q.x - 1, q.y - 1, q.z - 1, q.w - 1};
   double px[4], py[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    px[c] = x[nd[c]] + wu[nd[c]] * dt;
-    py[c] = y[nd[c]] + wv[nd[c]] * dt;
+    // with the guard folded in, a marked node moves by d1/dt, the value the
+    // guard writes (a node off the relaxed list has d1 = 0 and w = 0: the
+    // same bits either way); its w is not read, as the guard may be
+    // writing it
+    double u, v;
+    if (guarded && mark[nd[c]]) {
+      const float2 d = d1[nd[c]];
+      u = (double)d.x / dt;
+      v = (double)d.y / dt;
+    } else {
+      u = wu[nd[c]];
+      v = wv[nd[c]];
+    }
+    px[c] = x[nd[c]] + u * dt;
+    py[c] = y[nd[c]] + v * dt;
   }
   double jp, ap, vp;
   quad_quality(px, py, jp, ap, vp);
@@ -448,9 +469,9 @@ __device__ __forceinline__ void check_element(int e, const int32_t* __restrict__
 // x + (d1/dt) dt (what the guard would give it), and an element also fails
 // when the proposal shrinks it below half its one-sweep area (bit 2).  stage 0: a counted element
 // marks its four nodes for the guard and sets flag[4]; stage 1 (only when
-// flag[4] is set, after the guard): a counted element sets the step's
-// fallback flag[0].  flag[3] keeps every reason seen since the network was
-// attached.
+// flag[4] is set, with the guard's velocities: k_learned_recheck): a counted
+// element sets the step's fallback flag[0].  flag[3] keeps every reason seen
+// since the network was attached.
 __global__ void k_learned_check(int nel, const int32_t* __restrict__ ielnd,
                                 const double* __restrict__ x, const double* __restrict__ y,
                                 const double* __restrict__ wu, const double* __restrict__ wv,
@@ -458,20 +479,24 @@ __global__ void k_learned_check(int nel, const int32_t* __restrict__ ielnd,
                                 int relative, int stage, int32_t* __restrict__ mark,
                                 int32_t* __restrict__ flag, const DevTimestep* ts) {
   if (ts && (ts->skip || !ts->ale_on)) return;
-  if (stage == 1 && flag[4] == 0) return;
   if (ts) dt = ts->dt;
   const int stride = gridDim.x * blockDim.x;
   for (int e = xcd_index(); e < nel; e += stride)
     check_element(e, ielnd, x, y, wu, wv, d1, dt, amax2, relative, stage, mark, flag);
 }
 
-// the guard: a relaxed node marked by the check moves by one plain sweep
-// instead of the network's proposal
-__global__ void k_learned_guard(const int32_t* __restrict__ rlist, int nr,
-                                const int32_t* __restrict__ mark,
-                                const float2* __restrict__ d1, double* __restrict__ wu,
-                                double* __restrict__ wv, double dt,
-                                const int32_t* __restrict__ flag, const DevTimestep* ts) {
+// Stage 1 with the guard folded in (only when stage 0 marked nodes,
+// flag[4]): a relaxed node marked by stage 0 moves by one plain sweep
+// instead of the network's proposal (written here), and every element is
+// checked again with the guarded velocities, which it forms itself from the
+// marks, so that the check needs no pass of its own after the guard's
+__global__ void k_learned_recheck(int nel, const int32_t* __restrict__ ielnd,
+                                  const double* __restrict__ x, const double* __restrict__ y,
+                                  double* __restrict__ wu, double* __restrict__ wv,
+                                  const float2* __restrict__ d1, double dt, double amax2,
+                                  int relative, const int32_t* __restrict__ rlist, int nr,
+                                  int32_t* __restrict__ mark, int32_t* __restrict__ flag,
+                                  const DevTimestep* ts) {
   if (ts && (ts->skip || !ts->ale_on)) return;
   if (flag[4] == 0) return;
   if (ts) dt = ts->dt;
@@ -483,6 +508,8 @@ __global__ void k_learned_guard(const int32_t* __restrict__ rlist, int nr,
     wu[n] = (double)d.x / dt;
     wv[n] = (double)d.y / dt;
   }
+  for (int e = xcd_index(); e < nel; e += stride)
+    check_element(e, ielnd, x, y, wu, wv, d1, dt, amax2, relative, 1, mark, flag, true);
 }
 
 // per-column mean and standard deviation of the stored pairs (FP64 sums in
@@ -841,26 +868,25 @@ int learned_velocity(Handle* hh, const double* x, const double* y, double* wu, d
     set_error("learned relaxation: not set up (bl_relax_learned)");
     return BL_FAILURE;
   }
-  // a workgroup per 256 relaxed nodes, then the zero workgroups
-  const int nbm = std::max((nr + 255) / 256, 1);
+  // at most kMlpBlocks workgroups, each taking every kMlpBlocks-th chunk of
+  // 256 relaxed nodes (the network is staged once per workgroup), then the
+  // zero workgroups
+  const int nchunk = std::max((nr + 255) / 256, 1);
+  const int nbm = std::min(nchunk, kMlpBlocks);
   const int nbz = (nz + 255) / 256;
   BL_LAUNCH(h, "mlp_nodes", k_mlp_nodes, (unsigned)(nbm + nbz), 256, packed(m),
             (const float*)m->norm, relax_args(h), (const int32_t*)h->d_rlist, nr,
-            (const int32_t*)h->d_zlist, nz, nbm, x, y, wu, wv, dt, (float2*)h->d_ld1, h->d_lmark,
-            h->d_lflag, ts);
+            (const int32_t*)h->d_zlist, nz, nchunk, nbm, x, y, wu, wv, dt, (float2*)h->d_ld1,
+            h->d_lmark, h->d_lflag, ts);
   const double amax2 = h->learned_max_aspect * h->learned_max_aspect;
-  for (int stage = 0; stage < 2; ++stage) {
-    if (stage == 1)
-      BL_LAUNCH(h, "learned_guard", k_learned_guard,
-                std::min(nblocks(std::max(nr, 1)), kGatedBlocks), kBlock,
-                (const int32_t*)h->d_rlist, nr, (const int32_t*)h->d_lmark,
-                (const float2*)h->d_ld1, wu, wv, dt, (const int32_t*)h->d_lflag, ts);
-    BL_LAUNCH(h, "learned_check", k_learned_check,
-              stage == 0 ? nblocks(nel) : std::min(nblocks(nel), kGatedBlocks), kBlock, nel,
-              (const int32_t*)h->I(BL_IELND), x, y, (const double*)wu, (const double*)wv,
-              (const float2*)h->d_ld1, dt, amax2, h->learned_relative, stage, h->d_lmark,
-              h->d_lflag, ts);
-  }
+  BL_LAUNCH(h, "learned_check", k_learned_check, nblocks(nel), kBlock, nel,
+            (const int32_t*)h->I(BL_IELND), x, y, (const double*)wu, (const double*)wv,
+            (const float2*)h->d_ld1, dt, amax2, h->learned_relative, 0, h->d_lmark,
+            h->d_lflag, ts);
+  BL_LAUNCH(h, "learned_recheck", k_learned_recheck,
+            std::min(nblocks(std::max(nel, nr)), kGatedBlocks), kBlock, nel,
+            (const int32_t*)h->I(BL_IELND), x, y, wu, wv, (const float2*)h->d_ld1, dt, amax2,
+            h->learned_relative, (const int32_t*)h->d_rlist, nr, h->d_lmark, h->d_lflag, ts);
   // the plain relaxation, whose kernels run only when the check failed
   return relax_velocity(h, x, y, wu, wv, dt, ts, h->d_lflag);
 }
