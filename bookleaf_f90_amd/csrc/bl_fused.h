@@ -122,8 +122,8 @@ struct FusedArgs {
   // the tile step (bl_tiles.h): tiles, their element entries (element |
   // flags << 30, four 8-bit local node slots), the boundary list and its
   // corner forces, and the at-rest bytes of the step (rest8: now, rest8_prev:
-  // the step before, rest8_next: written for the next step); worklist
-  // segments and partial slots are the ntile tiles then the boundary blocks
+  // the step before, rest8_next: written for the next step); the F1 partial
+  // slots (nseg) are the ntile tiles' then the boundary blocks
   int ntile, nbnd, nseg;
   const int32_t* tile_eptr;
   const int32_t* tile_nptr;

@@ -1,39 +1,46 @@
 // bl_fused.hip -- the device-resident whole step (bl_run_graph).
 //
-// One Lagrangian step (inf_solver_getdt + inf_solver_lagstep) in four
-// full-grid kernels plus one-workgroup kernels for the step gate and the dt
-// selection, with dt, time and the error state kept on the device, so a step
-// can be captured as a hipGraph and replayed with no host synchronisation:
+// One step (inf_solver_getdt + inf_solver_lagstep [+ inf_solver_alestep])
+// as a few grid kernels and one-workgroup kernels, with dt, time and the
+// error state kept on the device, so that a step is captured as a hipGraph
+// and replayed with no host synchronisation:
 //
-//   F0 begin   : time_end / error gate for the step, per-step counters
-//   F1 getdt   : corner gather, edge viscosity with the Christensen limiter
-//                (neighbour edges recomputed from node data), cnviscx/y,
-//                elvisc, per-workgroup CFL and divergence candidates
-//   F1b select : one workgroup: first-index min/max over the candidates and
-//                time_dr_calc / time_dr_end -> dt, time, nstep on the device
+//   F1 getdt   : every element: corner gather, the DIV term, whether its four
+//                nodes share one velocity (then its zero viscosity and CFL
+//                term here), the at-rest mask, per-workgroup partials
+//   F1L limiter: the elements in motion (from the mask): the limiter and
+//                edge viscosity (neighbour edges from node data), cnviscx/y,
+//                elvisc, CFL term; folds F1's partials into its own
+//   F1b select : one workgroup: the step before's end, first-index min/max
+//                over the partials and time_dr_calc / time_dr_end -> dt,
+//                1/dt, time, nstep on the device
 //   F2 force   : predictor force, half-step geometry, energy and EOS in
 //                registers, then the corrector's corner forces P+Q(+SP)(+HG),
-//                half-step corner weights and nodal-mass terms, stored
-//                corner-major (X[j*nel+e]) so the node gather of F3 reads
-//                each of them once, coalesced (the corrector force of an
-//                element only needs that element's predictor results)
+//                half-step corner weights and nodal-mass terms, corner-major
+//                (X[j*nel+e]) so that F3 reads each of them once, coalesced
 //   F3 accel   : node gather through the (corner, element) CSR map in the
-//                reference's summation order: mass, area, force ->
-//                acceleration, BCs, u^{n+1}, ubar, x^{n+1} into the other
-//                node buffer (ping-pong: no node is read after it is written)
-//   F4 state   : corrector geometry, density, energy with ubar, EOS
-//   F2m, F4m   : mixed cells only (ncp > 0), one thread per mixed cell after
-//                F2 / F4: the component terms of the hydro drivers, and the
-//                cell's corrector corner force replaced by the frvolume
-//                average of its component forces
+//                reference's summation order -> acceleration, BCs, u^{n+1},
+//                ubar, x^{n+1} into the other node set (ping-pong)
+//   F4 state   : corrector geometry, density, energy with ubar, EOS; in a
+//                Lagrangian step of pure cells also the next step's F1 terms
+//                (F1 then runs once per run call)
+//   F5 end     : errors and the node-set flip, when no later kernel of the
+//                step or the next F1b can record them; F2m, F4m: the mixed
+//                cells' component terms, in F2's and F5's extra workgroups
+//   tile step  : in a Lagrangian step of pure cells, F2, F3 and F4 in one
+//                kernel per tile of nodes (k_lag_tile, bl_tiles.hip) and
+//                k_f4_boundary for the elements shared between tiles
+//   R1..R5     : the remap (ale_dr_*), after the Lagrangian step when ALE is
+//                active
 //
 // Every FP64 expression is the same function of the same inputs as in the
 // per-driver path (bl_math.h), so the results are bit-identical to the
 // reference drivers; the intermediate half-step pressure/energy never reach
 // HBM.  Fields that hold only phase scratch in the reference (rscratch*) are
 // not written except ubar (rscratch14/15, which the ALE step reads) and the
-// corrector corner forces (rscratch23/24), which k_materialize writes once at
-// the end of a batch from the last step's corner-major copies.
+// corrector corner forces (rscratch23/24): the tile step's last step of a
+// batch writes them, otherwise k_materialize does at the end of a batch from
+// the last step's corner-major copies.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +67,10 @@ __device__ __forceinline__ bool step_skipped(const FusedArgs& A, const DevTimest
 
 // the limiter pass's at-rest flags per thread and scan pass (a byte word)
 constexpr int kScanPer = 8;
+// F1b's workgroup: two partials of each control per thread at the usual
+// 1024 limiter workgroups (512 measured 1-1.5 us a step faster than 1024
+// threads, which take longer to start)
+constexpr int kF1bThreads = 512;
 
 // a wave's count of elements whose zero viscosity it wrote, to its
 // workgroup's counter slot
@@ -189,11 +200,11 @@ __device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane,
 // velocity difference zero, so geteviscosity zeroes every edge (|du| <=
 // zerocut) whatever the limiter gives, and each edge's elvisc term is
 // w3 * 0 = +0 (finite states): such an element gets its zero viscosity and
-// CFL term here.  Every other element is appended to the worklist for
-// F1L, which computes its viscosity and CFL term; its CFL slot here is
-// DBL_MAX.  The DIV term of every element is computed here.  The CFL and
-// DIV selections are first-index minima/maxima over (value, index) pairs,
-// so the split and the worklist order do not change the result.
+// CFL term here, and its bit in the at-rest mask.  Every other element is
+// left to F1L (which finds it from the mask) for its viscosity and CFL term;
+// its CFL slot here is DBL_MAX.  The DIV term of every element is computed
+// here.  The CFL and DIV selections are first-index minima/maxima over
+// (value, index) pairs, so the split and the order do not change the result.
 
 // F1's terms of element e from the state a step starts from (nodes x, y, u,
 // v; g the geometry at x; cs2e): whether its nodes move, its CFL term when
@@ -258,7 +269,7 @@ __global__ void __launch_bounds__(kBlock) k_work_all(int32_t* work, int n) {
   if (i < n) work[i] = i;
 }
 
-// viscosity, elvisc and CFL term of a worklist element (a NaN CFL term
+// viscosity, elvisc and CFL term of an element in motion (a NaN CFL term
 // becomes DBL_MAX: it never wins the strict '<' scan)
 __device__ __forceinline__ double f1_moving(const FusedArgs& A, const FusedNodes& N, int e) {
   int nd[4];
@@ -1789,7 +1800,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     const unsigned nseg = (unsigned)A.nseg;
     const unsigned nbl = std::min(nseg, (unsigned)h->tune_lim_grid);
     BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nseg);
-    BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nseg, (int)nbl);
+    BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kF1bThreads, A, (int)nseg, (int)nbl);
     BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O);
     if (h->nbnd > 0)
       BL_LAUNCH(h, "f4_boundary", k_f4_boundary, nblocks(h->nbnd), kBlock, A, Oc);
@@ -1809,7 +1820,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     BL_LAUNCH(h, "f1_getdt", k_f1_getdt, nbe, kBlock, A, N);
   }
   BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nbe);
-  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kFinalBlock, A, (int)nbe, (int)nbl);
+  BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kF1bThreads, A, (int)nbe, (int)nbl);
   // F2 and F5 (or R1) take a workgroup per 256 mixed cells for the
   // component work; F2 puts them first, rounded up to whole XCD rounds
   const unsigned nbm = A.mx.ncp > 0 ? nblocks(A.mx.nmx) : 0;
@@ -2292,11 +2303,12 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
 // ensemble) of the current state under each candidate shape -- threads per
 // workgroup 64/128/256 and workgroups 128..2048 (single problem) or
 // elements per workgroup and instance 1024..16384 (ensemble) -- and keeps
-// the fastest.  Each trial runs the viscosity pass (F1/E1) that fills the
-// worklist, then the limiter pass between two events; neither touches the
-// state a step carries over (the viscosity fields they write are saved and
-// restored, the worklist counters cleared).  The choice changes no result:
-// the limiter's CFL selection is a first-index minimum whatever the split.
+// the fastest.  Each trial runs the viscosity pass (F1/E1) that writes the
+// at-rest mask (an ensemble's worklists), then the limiter pass between two
+// events; neither touches the state a step carries over (the viscosity
+// fields they write are saved and restored, the masks and worklist counters
+// reset).  The choice changes no result: the limiter's CFL selection is a
+// first-index minimum whatever the split.
 int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
   if (!h || reps < 1) {
     set_error("bl_autotune: reps >= 1");
@@ -2360,8 +2372,8 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
       const int block = blocks[b], second = ens ? chunks[g] : grids[g];
       if (ens && ens_set_chunk(h, second)) { rc = BL_FAILURE; break; }
       if (ens) E = make_ens(h);
-      // pass 0: the worklist F1 leaves for the current state; pass 1: every
-      // element listed (the worklist as the flow spreads)
+      // pass 0: the elements in motion in the current state; pass 1: every
+      // element in motion (the flow spread everywhere)
       for (int pass = 0; pass < 2 && rc == BL_SUCCESS; ++pass) {
         std::vector<float> t;
         // every element in motion: the single problem's mask all zero, an
@@ -2422,7 +2434,7 @@ int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
     if (ens) ens_set_chunk(h, def_second);
     return rc;
   }
-  // the shape whose worse ratio to the best time of the two worklists is
+  // the shape whose worse ratio to the best time of the two passes is
   // smallest (the first such shape on a tie)
   double bc = DBL_MAX, bf = DBL_MAX;
   for (int k = 0; k < trials; ++k) { bc = std::min(bc, tcur[k]); bf = std::min(bf, tfull[k]); }
