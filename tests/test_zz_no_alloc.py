@@ -68,3 +68,29 @@ def test_noalloc_getdt_repeated_on_one_handle(oracle_lib):
         assert (g.ts.dt, g.ts.time, g.ts.idtel) == (o.ts.dt, o.ts.time, o.ts.idtel)
     assert diff_fields(g.state(), o.state()) == []
 
+
+
+def test_noalloc_getdt_refuses_a_mesh_beyond_its_workspace():
+    """A handle of BL_NOALLOC_MAX_EL + 256 elements (about 15 GB of fields,
+    created through the C-ABI with nothing uploaded): the allocation-free
+    timestep refuses it with the reason before any launch, and the handle
+    stays usable (bl_destroy frees it)."""
+    import ctypes as C
+    from bookleaf_f90_amd import abi
+    from bookleaf_f90_amd.device import load_library
+    lib = load_library()
+    n = abi.NOALLOC_MAX_EL + 256
+    print("case noalloc limit, nel", n, flush=True)
+    sz = abi.Sizes(nel=n, nel1=n, nel2=n, nnd=n, nnd1=n, nnd2=n, nsz=n, nmat=1, nreg=1)
+    h = C.c_void_p()
+    assert lib.bl_create(C.byref(sz), C.byref(h)) == abi.SUCCESS, lib.bl_last_error().decode()
+    try:
+        zd, zi = (C.c_double * 1)(0.0), (C.c_int32 * 1)(0)
+        assert lib.bl_set_regions(h, 1, zd, zd, zi, zi) == abi.SUCCESS
+        cfl, div, err = abi.Dt(), abi.Dt(), abi.Error()
+        rc = lib.bl_hydro_getdt_noalloc(h, C.byref(cfl), C.byref(div), C.byref(err))
+        assert rc == abi.FAILURE
+        msg = lib.bl_last_error().decode()
+        assert "exceed the workspace maximum" in msg and str(abi.NOALLOC_MAX_EL) in msg, msg
+    finally:
+        assert lib.bl_destroy(h) == abi.SUCCESS
