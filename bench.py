@@ -321,8 +321,8 @@ def cpu_baseline_ensemble(params, steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the path is a single mesh partition per "
-              "GPU (see DESIGN.md); --gpus %d refused" % args.gpus)
+        print("bench.py runs on one GPU only: the library steps one mesh partition per "
+              "GPU and has no exchange between partitions; --gpus %d refused" % args.gpus)
         sys.exit(2)
     from bookleaf_f90_amd import problem as P
     from bookleaf_f90_amd.device import BookLeaf, Ensemble
