@@ -28,7 +28,13 @@ Reported next to the throughput:
   driver_boundary_GBps  the throughput expressed in the per-driver
                 algorithmic bytes of the reference's call graph
                 (1664 B/element + 272 B/node per Lagrangian step, +84 B/element
-                with the anti-hourglass force)
+                with the anti-hourglass force): an equivalent rate, not a
+                bandwidth, since the fused step does not move those bytes
+  fused_floor   a Lagrangian step against the state-once floor (144 B/element
+                + 68 B/node): the floor's time at the HBM peak over the step's
+                time (step_frac), the step's algorithmic bytes and its measured
+                counter traffic (profiles/traffic_<workload>.json) over it;
+                null for a step with a remap, whose state the floor omits
 
 --workload tgv runs a Taylor-Green vortex on the same 1024x1024 size with
 every element in motion from the first step (no element skips the limiter),
@@ -80,13 +86,13 @@ KERNEL_BYTES = {
     # writes v0, m0, m 24, face energy flux 32
     "r2_massflux": (160, 0),
     # ielnd, packed ielel/ielfc 32, energy flux 32, m, v0, m0 24, e r/w 16,
-    # cnmass r/w 64, face mass flux 32, ielmat 4; writes corner u, v, m0, dm,
-    # mass flux, m0 (corner-major) 192, rho, p, cs2 24 | u, v, x, y, mesh
-    # velocity 48
-    "r3_energy_update": (428, 48),
-    # packed ielel/ielfc 16, corner m0, dm, u, v 128; writes corner fluxes 64
-    # (a corner with zero mass flux reads only its flux: at rest fewer bytes)
-    "r4_momflux": (208, 0),
+    # cnmass r/w 64, face mass flux 32, ielmat 4; writes corner m0, dm, mass
+    # flux, m0 (corner-major) 128, rho, p, cs2 24 | x, y, mesh velocity 32
+    "r3_energy_update": (356, 32),
+    # packed ielel/ielfc 16, corner m0, dm 64, ielnd 16; writes corner fluxes
+    # 64 | u, v 16 (a corner with zero mass flux reads only its flux: at rest
+    # fewer bytes)
+    "r4_momflux": (160, 16),
     "r5_nodes": (136, 124),
     # node relaxation sweeps: stencil table 32, x, y 16, writes 16 | the last
     # sweep also reads the Lagrangian x, y 16
@@ -98,8 +104,10 @@ KERNEL_BYTES = {
     "mlp_nodes": (0, 80),
     # the check: ielnd 16 | x, y, mesh velocity 32
     "learned_check": (16, 32),
-    # the limiter pass: only the elements on F1's worklist (LIMITER_BYTES)
+    # the limiter pass: only the elements on F1's worklist (LIMITER_BYTES);
+    # the ensemble's pass reads its instances' worklist segments the same way
     "f1_limiter": (0, 0),
+    "e1_limiter": (0, 0),
 }
 # the limiter pass, per listed element: worklist entry 4, ielnd 16, ielel
 # 16, outer-node table 32, own rho, cs2 16; writes cnviscx/y 64, elvisc 8
@@ -164,7 +172,7 @@ def kernel_bytes(k, nel, nnd, moving, zeroed, fold=False, tiles=None, remap=Fals
     entries) of the tile step; remap: the step ends with the remap"""
     if k in TILE_KERNELS:
         return tile_bytes(k, nel, nnd, moving, zeroed, tiles)
-    if k == "f1_limiter":
+    if k in ("f1_limiter", "e1_limiter"):
         return LIMITER_BYTES[0] * moving + LIMITER_BYTES[1] * nnd * min(1.0, moving / nel)
     be, bn = KERNEL_BYTES[k]
     if remap and k == "f4_state":
@@ -184,6 +192,11 @@ DRIVER_BOUNDARY = (1664 + 84, 272)
 # Eulerian: the Lagrangian step without the hourglass force plus the remap's
 # 504 B/element + 108 B/node
 DRIVER_BOUNDARY_EUL = (1664 + 504, 272 + 108)
+# the fused floor: a Lagrangian step's element and node state read once and
+# written once, 144 B/element + 68 B/node, with no intermediate in HBM
+FUSED_FLOOR = (144, 68)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,13 +440,14 @@ def main():
     dom = max((k for k in kern if k in priced), key=lambda k: kern[k]["total_ms"],
               default=None)
     roof = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    traffic_file = tf if os.path.exists(tf) and args.n == 1024 else None
     if dom:
         alg = kernel_bytes(dom, nel, nnd, moving, zeroed, fold, tiles, remap)
         avg_s = kern[dom]["avg_ms"] * 1e-3
         traffic = None
-        tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
-        if os.path.exists(tf) and args.n == 1024:
-            traffic = json.load(open(tf)).get(dom)
+        if traffic_file is not None:
+            traffic = json.load(open(traffic_file)).get(dom)
         roof = {"kernel": dom, "bound": "hbm", "achieved": alg / avg_s / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg / avg_s / HBM_PEAK,
                 "traffic": traffic, "algorithmic_bytes": alg, "avg_ms": kern[dom]["avg_ms"]}
@@ -460,6 +474,14 @@ def main():
     fused_bytes = sum(kernel_bytes(k, nel, nnd, moving, zeroed, fold, tiles, remap) *
                       kern[k]["launches"] / max(args.profile_steps, 1)
                       for k in kern if k in priced)
+    floor_bytes = FUSED_FLOOR[0] * nel + FUSED_FLOOR[1] * nnd
+    # measured HBM bytes of one step (rocprofv3 counters, per launch) when the
+    # traffic file matches this size
+    step_traffic = None
+    if traffic_file is not None:
+        tdict = json.load(open(traffic_file))
+        step_traffic = sum(tdict[k] * kern[k]["launches"] / max(args.profile_steps, 1)
+                           for k in kern if k in tdict)
     out = {
         "metric": "element-steps/s FP64 at 1M cells, %HBM peak; learned vs plain ALE at "
                   "equal L1(rho)",
@@ -487,8 +509,19 @@ def main():
         "cpu_baseline_throughput": cpu_tp,
         "speedup_vs_cpu_throughput": (value / cpu_tp["value"]) if cpu_tp else None,
         "step_hbm_frac": fused_bytes / (sec / args.steps) / HBM_PEAK,
+        # an equivalent rate, not a bandwidth: fusion does not move these bytes
         "driver_boundary_GBps": step_bytes / (sec / args.steps) / 1e9,
-        "driver_boundary_hbm_frac": step_bytes / (sec / args.steps) / HBM_PEAK,
+        # the Lagrangian step against the state-once floor: step_frac is the
+        # floor's time at the HBM peak over the measured step time (a remap
+        # step's own state is not in the floor: null there)
+        "fused_floor": ({"bytes_per_step": floor_bytes,
+                         "ms_at_peak": 1e3 * floor_bytes / HBM_PEAK,
+                         "step_frac": floor_bytes / (sec / args.steps) / HBM_PEAK,
+                         "algorithmic_over_floor": fused_bytes / floor_bytes,
+                         "step_traffic_bytes": step_traffic,
+                         "traffic_over_floor": (step_traffic / floor_bytes
+                                                if step_traffic else None)}
+                        if not remap else None),
         "kernels": kern,
         # elements in motion and zero viscosity writes per step (bl_run_stats),
         # over the profile pass and over every run call of this command

@@ -1306,7 +1306,7 @@ struct AleArgs {
   // temporaries: (4, nel) face / corner arrays and (nel) scalars; the
   // arrays the node pass gathers (cnm0T, cnflux, fu, fv) are corner-major,
   // X[j*nel + e], so one corner slot of a wave's nodes is contiguous
-  double *fcdm, *flux, *cnu, *cnv, *cnm0, *cndm, *cnflux, *fu, *fv;
+  double *fcdm, *flux, *cnm0, *cndm, *cnflux, *fu, *fv;
   double *dv, *v0, *m0, *cnm0T;
   DtResult* dtres;
   const DevTimestep* ts;
@@ -1413,12 +1413,9 @@ __global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
   double en = B.en[e];
   if (m1 > v0 * B.dencut) en = (en * B.m0[e] + s) / m1;
   B.en[e] = en;
-  // corner gathers of the node velocities, corner mass before the remap
-  double cu[4], cw[4], cm[4];
-  load4(B.u, nd, cu);
-  load4(B.v, nd, cw);
-  st4(B.cnu, e, cu);
-  st4(B.cnv, e, cw);
+  // corner mass before the remap (R4 reads the node velocities through
+  // ielnd where the reference gathers them to corners)
+  double cm[4];
   ld4(B.cnmass, e, cm);
   st4(B.cnm0, e, cm);
   // ale_kn_fluxbasisnd, corner part of ale_kn_massbasisnd
@@ -1467,7 +1464,8 @@ __global__ void __launch_bounds__(kBlock) k_r3(AleArgs B) {
 }
 
 // R4: ale_kn_fluxndvl for both velocity components (ale_dr_advect.f90:
-// 241-268; the v flux reads the corner v gathered before the u update)
+// 241-268; the v flux reads the corner v gathered before the u update, and
+// R5 updates the nodes after this kernel, so both read u and v at the nodes)
 // With MIX (R1 does F5's work and the step has mixed cells), R4's first
 // nbm8 workgroups are F4m, one thread per mixed cell, gated as F5 gated
 // it: not skipped, whether or not this step remaps (ts->skip now includes
@@ -1506,14 +1504,14 @@ __global__ void __launch_bounds__(kBlock) k_r4(AleArgs B, FusedArgs A, FusedNode
                     : xcd_index();
   if (e >= B.nel) return;
   const NbFc c = load_nbfc_packed(e, B.nbfc);
-  double out[4];
+  double ou[4], ov[4];
   const int nel = B.nel;
-  fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnu, out);
+  fluxndvl_uv(e, c, B.ielnd, B.cnm0, B.cndm, B.u, B.v, ou, ov);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) B.fu[j * nel + e] = out[j];
-  fluxndvl_el(e, c, B.cnm0, B.cndm, B.cnv, out);
-#pragma unroll
-  for (int j = 0; j < 4; ++j) B.fv[j * nel + e] = out[j];
+  for (int j = 0; j < 4; ++j) {
+    B.fu[j * nel + e] = ou[j];
+    B.fv[j * nel + e] = ov[j];
+  }
 }
 
 // R5: node sums (ale_kn_initbasisnd, ale_kn_calcbasisnd, ale_kn_massbasisnd
@@ -1702,9 +1700,9 @@ AleArgs make_ale(bl_handle_t* h, const FusedNodesOut& O) {
   B.a1 = h->R(BL_A1); B.a2 = h->R(BL_A2); B.a3 = h->R(BL_A3);
   B.b1 = h->R(BL_B1); B.b2 = h->R(BL_B2); B.b3 = h->R(BL_B3);
   double** t = h->ale_tmp;
-  B.fcdm = t[0]; B.flux = t[1]; B.cnu = t[2]; B.cnv = t[3]; B.cnm0 = t[4]; B.cndm = t[5];
-  B.cnflux = t[6]; B.fu = t[7]; B.fv = t[8];
-  B.dv = t[9]; B.v0 = t[10]; B.m0 = t[11]; B.cnm0T = t[12];
+  B.fcdm = t[0]; B.flux = t[1]; B.cnm0 = t[2]; B.cndm = t[3];
+  B.cnflux = t[4]; B.fu = t[5]; B.fv = t[6];
+  B.dv = t[7]; B.v0 = t[8]; B.m0 = t[9]; B.cnm0T = t[10];
   B.dtres = h->d_dtres;
   B.ts = h->d_ts;
   return B;
@@ -1881,8 +1879,8 @@ int ensure_nodesets(bl_handle_t* h) {
     for (int k = 0; k < 4; ++k)
       BL_HIP(hipMalloc(&h->cterm[k], sizeof(double) * 4 * std::max(h->sz.nel, 1)));
   if (h->cfg.zexist && !h->ale_tmp[0])
-    for (int k = 0; k < 13; ++k)
-      BL_HIP(hipMalloc(&h->ale_tmp[k], sizeof(double) * ((k < 9 || k == 12) ? 4 : 1) *
+    for (int k = 0; k < 11; ++k)
+      BL_HIP(hipMalloc(&h->ale_tmp[k], sizeof(double) * ((k < 7 || k == 10) ? 4 : 1) *
                                            std::max(h->sz.nel, 1)));
   if (h->nodeset[1][0]) return BL_SUCCESS;
   const int ids[4] = {BL_NDX, BL_NDY, BL_NDU, BL_NDV};

@@ -138,7 +138,7 @@ struct Handle {
   unsigned long long* d_vstats = nullptr;  // counters of the last run call
   // remap temporaries of the fused step (zexist): 4*nel each for the face
   // and corner arrays, nel each for the element scalars
-  double* ale_tmp[13] = {};
+  double* ale_tmp[11] = {};
   // [p]: one step from node set p; [2 + p]: kGraphSteps steps from node set
   // p (bl_run_graph launches those while at least that many steps remain)
   hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
