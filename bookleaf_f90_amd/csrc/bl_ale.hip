@@ -304,6 +304,7 @@ using namespace bl;
 extern "C" {
 
 int bl_ale_active(bl_handle_t* h, const bl_timestep_t* ts) {
+  bl_touch(h);
   if (!h || !ts || !h->cfg.zexist) return 0;
   return (ts->time >= h->cfg.mintime) && (ts->time <= h->cfg.maxtime);
 }
@@ -342,11 +343,13 @@ static int fluxvolume(bl_handle_t* h, double dt, const DevTimestep* dts) {
 extern "C" {
 
 int bl_ale_getmeshstatus(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   return meshstatus(h, nullptr);
 }
 
 int bl_ale_getfluxvolume(bl_handle_t* h, const bl_timestep_t* ts) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   return fluxvolume(h, ts->dt, nullptr);
 }
@@ -440,6 +443,7 @@ int ale_step_device(Handle* hb, const DevTimestep* d_ts) {
 extern "C" {
 
 int bl_ale_advect(bl_handle_t* h, const bl_timestep_t* ts, bl_error_t* err) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   if (!h->maps_built) { set_error("bl_ale_advect: maps not built"); return BL_FAILURE; }
   switch (h->cfg.adv_type) {
@@ -468,6 +472,7 @@ int bl_ale_advect(bl_handle_t* h, const bl_timestep_t* ts, bl_error_t* err) {
 }
 
 int bl_ale_hydroupdate(bl_handle_t* h, bl_error_t* err) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int rc = bl_geometry_getgeometry(h, err);
   if (bl_density_getdensity(h) || bl_hydro_geteos(h)) return BL_FAILURE;

@@ -691,6 +691,7 @@ int bl_destroy(bl_handle_t* h) {
 
 
 int bl_set_config(bl_handle_t* h, const bl_config_t* cfg) {
+  bl_touch(h);
   if (!h || !cfg) { set_error("bl_set_config: null argument"); return BL_FAILURE; }
   h->cfg = *cfg;
   invalidate_graph(h);
@@ -715,6 +716,7 @@ extern "C" {
 int bl_set_regions(bl_handle_t* h, int32_t nreg, const double* kappareg,
                    const double* pmeritreg, const int32_t* zdtnotreg,
                    const int32_t* zmidlength) {
+  bl_touch(h);
   if (!h || nreg <= 0) { set_error("bl_set_regions: bad argument"); return BL_FAILURE; }
   h->nreg = nreg;
   h->kappareg.assign(kappareg, kappareg + nreg);
@@ -729,6 +731,7 @@ int bl_set_regions(bl_handle_t* h, int32_t nreg, const double* kappareg,
 }
 
 int bl_set_eos(bl_handle_t* h, int32_t nmat, const int32_t* itype, const double* param) {
+  bl_touch(h);
   if (!h || nmat <= 0) { set_error("bl_set_eos: bad argument"); return BL_FAILURE; }
   h->nmat = nmat;
   h->eos_itype.assign(itype, itype + nmat + 1);
@@ -750,10 +753,14 @@ int bl_field_info(bl_handle_t* h, int32_t id, int32_t* kind, int64_t* count) {
 
 void* bl_field_device_ptr(bl_handle_t* h, int32_t id) {
   if (!h || id < 1 || id > BL_NFIELD) return nullptr;
+  // stores through it are not seen here: no run call resumes after this
+  h->ptr_out = true;
+  h->resumable = false;
   return h->f[id];
 }
 
 int bl_upload(bl_handle_t* h, int32_t id, const void* host, int64_t count) {
+  bl_touch(h);
   if (!h || id < 1 || id > BL_NFIELD || !host) { set_error("bl_upload: bad argument"); return BL_FAILURE; }
   if (count < 0 || count > h->count[id]) { set_error("bl_upload: count exceeds field size"); return BL_FAILURE; }
   if (count == 0) return BL_SUCCESS;
@@ -850,6 +857,7 @@ static int build_outer(bl_handle_t* h, const std::vector<int32_t>& ielnd) {
 }
 
 int bl_build_maps(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
   std::vector<int32_t> ielnd(4 * (size_t)nel), s1(nel), s2(nel);
@@ -893,6 +901,7 @@ int bl_build_maps(bl_handle_t* h) {
 }
 
 int bl_init_mesh(bl_handle_t* h) {
+  bl_touch(h);
   // init_kn_connectivity (init_kn.f90:82-91 via utils_kn_connectivity.f90),
   // init_kn_nodetype (init_kn.f90:93-113), ielsort = identity
   // (init_dr_utils.f90:64-69)
@@ -965,6 +974,7 @@ int bl_init_mesh(bl_handle_t* h) {
 
 // --------------------------------------------------------------- utilities
 int bl_geometry_getgeometry(bl_handle_t* h, bl_error_t* err) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
   BL_HIP(hipMemsetAsync(&h->d_dtres->negvol, 0, sizeof(int32_t), h->stream));
@@ -993,6 +1003,7 @@ int bl_geometry_getgeometry(bl_handle_t* h, bl_error_t* err) {
 }
 
 int bl_density_getdensity(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
   BL_LAUNCH(h, "getdensity", k_divide, nblocks(nel), kBlock, nel, h->R(BL_ELMASS),
@@ -1001,6 +1012,7 @@ int bl_density_getdensity(bl_handle_t* h) {
 }
 
 int bl_hydro_geteos(bl_handle_t* h) {
+  bl_touch(h);
   if (!h || !h->d_eos_itype) { set_error("bl_hydro_geteos: EOS not set"); return BL_FAILURE; }
   int nel = h->sz.nel;
   BL_LAUNCH(h, "geteos", k_geteos, nblocks(nel), kBlock, nel, h->I(BL_IELMAT),
@@ -1010,6 +1022,7 @@ int bl_hydro_geteos(bl_handle_t* h) {
 }
 
 int bl_init_state(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
   BL_LAUNCH(h, "elmass", k_elmass, nblocks(nel), kBlock, nel, h->R(BL_ELDENSITY),
@@ -1019,6 +1032,7 @@ int bl_init_state(bl_handle_t* h) {
 }
 
 int bl_hydro_init(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel2 = h->sz.nel2;
   BL_LAUNCH(h, "fill", k_fill_r, nblocks(nel2), kBlock, h->R(BL_ELVISC), (int64_t)nel2, 0.0);
@@ -1076,11 +1090,13 @@ static int getdt_with(bl_handle_t* h, double* part, int32_t* parti, DtResult* dr
 }
 
 int bl_hydro_getdt(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) {
+  bl_touch(h);
   if (!h || !h->d_zdtnotreg) { set_error("bl_hydro_getdt: regions not set"); return BL_FAILURE; }
   return getdt_with(h, h->d_part, h->d_parti, h->d_dtres, cfl, div, err);
 }
 
 int bl_hydro_getdt_noalloc(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_t* err) {
+  bl_touch(h);
   if (!h || !h->d_zdtnotreg) {
     set_error("bl_hydro_getdt_noalloc: regions not set");
     return BL_FAILURE;
@@ -1104,6 +1120,7 @@ int bl_hydro_getdt_noalloc(bl_handle_t* h, bl_dt_t* cfl, bl_dt_t* div, bl_error_
 }
 
 int bl_hydro_setpredictor(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
   BL_LAUNCH(h, "copy", k_copy_r, nblocks(nel), kBlock, h->R(BL_ELENERGY), h->R(BL_RSCRATCH11), nel);
@@ -1111,6 +1128,7 @@ int bl_hydro_setpredictor(bl_handle_t* h) {
 }
 
 int bl_hydro_setcorrector(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nel = h->sz.nel;
   BL_LAUNCH(h, "copy", k_copy_r, nblocks(nel), kBlock, h->R(BL_RSCRATCH11), h->R(BL_ELENERGY), nel);
@@ -1118,6 +1136,7 @@ int bl_hydro_setcorrector(bl_handle_t* h) {
 }
 
 int bl_hydro_getforce(bl_handle_t* h, const bl_timestep_t* ts) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   int nel = h->sz.nel;
   int zhg = (ts->zcorrector && h->cfg.zhg) ? 1 : 0;
@@ -1131,6 +1150,7 @@ int bl_hydro_getforce(bl_handle_t* h, const bl_timestep_t* ts) {
 }
 
 int bl_hydro_getstate(bl_handle_t* h, const bl_timestep_t* ts, bl_error_t* err) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   int nnd = h->sz.nnd;
   BL_LAUNCH(h, "getvertex", k_getvertex, nblocks(nnd), kBlock, nnd, 0.5 * ts->dt, h->R(BL_NDX),
@@ -1140,6 +1160,7 @@ int bl_hydro_getstate(bl_handle_t* h, const bl_timestep_t* ts, bl_error_t* err) 
 }
 
 int bl_hydro_getenergy(bl_handle_t* h, const bl_timestep_t* ts) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   int nel = h->sz.nel;
   BL_LAUNCH(h, "getenergy", k_getenergy, nblocks(nel), kBlock, nel, ts->dts, h->cfg.zerocut,
@@ -1149,6 +1170,7 @@ int bl_hydro_getenergy(bl_handle_t* h, const bl_timestep_t* ts) {
 }
 
 int bl_hydro_initacceleration(bl_handle_t* h) {
+  bl_touch(h);
   // mixed cells: element corner forces from the components
   // (hydro_dr_getacceleration.f90:69-72); nproc=1: no exchange
   if (!h) return BL_FAILURE;
@@ -1156,6 +1178,7 @@ int bl_hydro_initacceleration(bl_handle_t* h) {
 }
 
 int bl_hydro_scatteracceleration(bl_handle_t* h) {
+  bl_touch(h);
   if (!h || !h->maps_built) { set_error("bl_hydro_scatteracceleration: maps not built"); return BL_FAILURE; }
   int nnd = h->sz.nnd1;
   BL_LAUNCH(h, "scatteracc", k_scatteracc, nblocks(nnd), kBlock, nnd, h->d_hyd_ptr, h->d_hyd_idx,
@@ -1166,6 +1189,7 @@ int bl_hydro_scatteracceleration(bl_handle_t* h) {
 }
 
 int bl_hydro_getacceleration(bl_handle_t* h) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   int nnd = h->sz.nnd;
   BL_LAUNCH(h, "getacc", k_getacc, nblocks(nnd), kBlock, nnd, h->cfg.dencut, h->cfg.zerocut,
@@ -1175,6 +1199,7 @@ int bl_hydro_getacceleration(bl_handle_t* h) {
 }
 
 int bl_hydro_applyacceleration(bl_handle_t* h, const bl_timestep_t* ts) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   int nnd = h->sz.nnd, nel = h->sz.nel;
   BL_LAUNCH(h, "applyacc", k_applyacc, nblocks(nnd), kBlock, nnd, ts->dt, h->R(BL_RSCRATCH14),

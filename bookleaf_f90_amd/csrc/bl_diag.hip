@@ -379,6 +379,7 @@ int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* n
 extern "C" {
 
 int bl_shortprint(bl_handle_t* h, bl_flag_totals_t* region, bl_flag_totals_t* material) {
+  bl_touch(h);
   if (!h || !region || !material) return BL_FAILURE;
   const int nel = h->sz.nel, nreg = h->sz.nreg, nmat = h->sz.nmat, ncp = h->sz.ncp;
   const unsigned nb = nblocks(nel), nbc = ncp > 0 ? nblocks(ncp) : 0;
@@ -427,6 +428,7 @@ int bl_shortprint(bl_handle_t* h, bl_flag_totals_t* region, bl_flag_totals_t* ma
 }
 
 int bl_sod_l1(bl_handle_t* h, double time, double x0, double scale, double* l1) {
+  bl_touch(h);
   if (!h || !l1) return BL_FAILURE;
   const int nel = h->sz.nel;
   const unsigned nb = nblocks(nel);
@@ -445,6 +447,7 @@ int bl_sod_l1(bl_handle_t* h, double time, double x0, double scale, double* l1) 
 }
 
 int bl_set_diagnostics(bl_handle_t* h, int32_t on, double x0, double scale, int32_t maxrec) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   BL_HIP(hipStreamSynchronize(h->stream));
   invalidate_graph(h);
@@ -472,6 +475,7 @@ int bl_set_diagnostics(bl_handle_t* h, int32_t on, double x0, double scale, int3
 }
 
 int bl_diagnostics(bl_handle_t* h, int32_t maxrec, double* out, int32_t* nrec) {
+  bl_touch(h);
   if (!h || !nrec) return BL_FAILURE;
   *nrec = 0;
   if (!h->d_diag_count || !h->d_diag_hist) return BL_SUCCESS;

@@ -1922,10 +1922,11 @@ __global__ void k_call_start(DevTimestep init, int set_ts, DevTimestep* __restri
 }
 
 // Forget which elements' viscosity fields hold F1's zeros (and the rest of
-// what k_call_start clears); with init, also write the device timestep
-int clear_rest(bl_handle_t* h, const DevTimestep* init = nullptr) {
-  const int64_t nrest = 8 * (int64_t)nblocks(std::max(h->sz.nel, 1));
-  const int64_t nrest8 = h->d_rest8 ? 2 * (int64_t)std::max(h->sz.nel, 1) : 0;
+// what k_call_start clears); with init, also write the device timestep.
+// keep_masks: a run call that continues the last one keeps the masks
+int clear_rest(bl_handle_t* h, const DevTimestep* init = nullptr, bool keep_masks = false) {
+  const int64_t nrest = keep_masks ? 0 : 8 * (int64_t)nblocks(std::max(h->sz.nel, 1));
+  const int64_t nrest8 = (h->d_rest8 && !keep_masks) ? 2 * (int64_t)std::max(h->sz.nel, 1) : 0;
   const int ncnt = (h->ens_n > 0 && h->d_cnt) ? h->ens_n : 0;
   const int64_t most = std::max(std::max(nrest, nrest8 / 8), (int64_t)std::max(kStatWords, ncnt));
   const unsigned grid = (unsigned)std::min<int64_t>(1024, (most + kBlock - 1) / kBlock);
@@ -2191,6 +2192,7 @@ extern "C" {
 
 int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* idt_hist,
                     int32_t* ctl_hist, int32_t* steps_taken, bl_error_t* err) {
+  bl_touch(h);
   if (!h || h->ens_n <= 0) {
     set_error("bl_ensemble_run: not an ensemble handle");
     return BL_FAILURE;
@@ -2308,6 +2310,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
 // reset).  The choice changes no result: the limiter's CFL selection is a
 // first-index minimum whatever the split.
 int bl_autotune(bl_handle_t* h, int32_t reps, bl_tuning_t* out) {
+  bl_touch(h);
   if (!h || reps < 1) {
     set_error("bl_autotune: reps >= 1");
     return BL_FAILURE;
@@ -2483,6 +2486,7 @@ int bl_run_stats(bl_handle_t* h, int64_t* moving, int64_t* zeroed) {
 // a launch shape chosen before (by bl_autotune on an earlier handle of the
 // same problem, say) applied without timing
 int bl_set_tuning(bl_handle_t* h, const bl_tuning_t* t) {
+  bl_touch(h);
   if (!h || !t || t->limiter_block < 64 || t->limiter_block > kBlock ||
       t->limiter_block % 64 != 0) {
     set_error("bl_set_tuning: limiter_block must be 64, 128, 192 or 256");
@@ -2504,6 +2508,7 @@ int bl_set_tuning(bl_handle_t* h, const bl_tuning_t* t) {
 
 // the tile step on (the default) or off
 int bl_set_tiling(bl_handle_t* h, int32_t on) {
+  bl_touch(h);
   if (!h) {
     set_error("bl_set_tiling: null handle");
     return BL_FAILURE;
@@ -2552,6 +2557,9 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
                  double* dt_hist, int32_t* idt_hist, int32_t* ctl_hist, int32_t* steps_taken,
                  bl_error_t* err) {
   if (!h || !ts) return BL_FAILURE;
+  // continue from the last run call's end (see Handle::resumable)
+  const bool resume = h->resumable && !h->ptr_out && fold_f1(h);
+  h->resumable = false;
   if (h->ens_n > 0) {
     set_error("bl_run_graph: an ensemble handle steps with bl_ensemble_run");
     return BL_FAILURE;
@@ -2587,10 +2595,12 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   d->time = ts->time; d->dt = ts->dt; d->dts = ts->dts;
   d->nstep = ts->nstep; d->idtel = ts->idtel; d->sdt = ts->sdt;
   d->last_step = ts->nstep + nsteps;
-  if (clear_rest(h, d)) return BL_FAILURE;
+  if (clear_rest(h, d, resume)) return BL_FAILURE;
   const bool tiles = use_tiles(h);
   int p = h->cur_set;
-  if (fold_f1(h) && launch_f1_prologue(h, p, stop_at_time_end)) return BL_FAILURE;
+  // F1 of the first step: on a resumed call the last call's last step left
+  // its terms, its at-rest mask and the zeros it stands for
+  if (fold_f1(h) && !resume && launch_f1_prologue(h, p, stop_at_time_end)) return BL_FAILURE;
   for (int k = 0; k < nsteps;) {
     // with kernel timing on, the same step kernels are launched directly so
     // each launch can be bracketed by HIP events on this stream
@@ -2644,6 +2654,8 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   ts->nstep = d->nstep; ts->idtel = d->idtel; ts->sdt = d->sdt;
   ts->zcorrector = 1;
   if (steps_taken) *steps_taken = taken;
+  // every step ran: the next call may continue from here
+  h->resumable = d->err == 0 && taken == nsteps && nsteps > 0 && fold_f1(h) && !h->ptr_out;
   switch (d->err) {
     case 1:
       fill_err(err, d->err_idx, BL_HALT_SINGLE,

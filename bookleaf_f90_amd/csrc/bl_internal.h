@@ -212,6 +212,15 @@ struct Handle {
   double* d_diag_part = nullptr;     // per-workgroup partials
   size_t diag_part_n = 0;
   int graph_stop_at_end = -1;
+  // The next bl_run_graph may continue from the last one's end: it keeps
+  // the at-rest masks and the F1 terms the last step left for the next
+  // step instead of clearing the masks and running F1 again.  Set by a run
+  // call that ran every step of a folded pure-cell Lagrangian batch; cleared
+  // by every entry point that can write device state (bl_touch).  A device
+  // pointer handed out (bl_field_device_ptr) allows writes this library
+  // does not see, so it ends resuming for the handle's lifetime.
+  bool resumable = false;
+  bool ptr_out = false;
   // timing
   bool timing = false;
   std::vector<TimingClass> tclass;
@@ -260,6 +269,12 @@ void invalidate_graph(Handle* h);
 }  // namespace bl
 
 struct bl_handle : public bl::Handle {};
+
+// an entry point that may write the handle's device state: the next run
+// call starts afresh
+inline void bl_touch(bl_handle* h) {
+  if (h) h->resumable = false;
+}
 
 // launch a kernel on the handle's stream, timed by class name when enabled
 #define BL_LAUNCH(h, cls, kern, grid, block, ...)                                  \

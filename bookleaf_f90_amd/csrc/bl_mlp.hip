@@ -1102,6 +1102,7 @@ int bl_mlp_train(bl_mlp_t* m, int32_t iters, int32_t batch, float lr, uint64_t s
 }
 
 int bl_relax_collect(bl_handle_t* h, bl_mlp_t* m, int32_t stride) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   if (m && (!h->d_ring || h->relax_type == BL_RELAX_OFF)) {
     set_error("bl_relax_collect: set a plain relaxation first (bl_set_relaxation)");
@@ -1121,6 +1122,7 @@ int bl_relax_collect(bl_handle_t* h, bl_mlp_t* m, int32_t stride) {
 }
 
 int bl_relax_learned(bl_handle_t* h, bl_mlp_t* m) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   if (m && (!h->d_ring || h->relax_type == BL_RELAX_OFF)) {
     set_error("bl_relax_learned: set the plain relaxation (the fallback) first");
@@ -1139,6 +1141,7 @@ int bl_relax_learned(bl_handle_t* h, bl_mlp_t* m) {
 }
 
 int bl_set_learned_check(bl_handle_t* h, double max_aspect, int32_t relative) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   if (!(max_aspect >= 1.0) || (relative != 0 && relative != 1)) {
     set_error("bl_set_learned_check: max_aspect >= 1 and relative 0 or 1");
@@ -1170,6 +1173,7 @@ int bl_relax_learned_stats(bl_handle_t* h, int32_t* steps, int32_t* guarded, int
 // the features of every relaxed node at the current positions (nnd x 26,
 // zero for nodes that are not relaxed)
 int bl_relax_features(bl_handle_t* h, float* feat) {
+  bl_touch(h);
   if (!h || !feat || !h->d_ring || !h->d_rlist || h->relax_type == BL_RELAX_OFF) {
     set_error("bl_relax_features: set a relaxation first (bl_set_relaxation)");
     return BL_FAILURE;

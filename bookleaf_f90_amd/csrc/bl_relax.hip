@@ -208,6 +208,7 @@ int relax_velocity(Handle* hh, const double* x, const double* y, double* wu, dou
 extern "C" {
 
 int bl_set_relaxation(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   if (!args_ok(type, nsweep, om, tol)) {
     set_error("bl_set_relaxation: type 0..3, nsweep >= 1, om > 0 and tol >= 0");
@@ -225,6 +226,7 @@ int bl_set_relaxation(bl_handle_t* h, int32_t type, int32_t nsweep, double om, d
 }
 
 int bl_relax_mesh(bl_handle_t* h, int32_t type, int32_t nsweep, double om, double tol) {
+  bl_touch(h);
   if (!h) return BL_FAILURE;
   if (!args_ok(type, nsweep, om, tol) || type == BL_RELAX_OFF) {
     set_error("bl_relax_mesh: type 1..3, nsweep >= 1, om > 0 and tol >= 0");

@@ -85,6 +85,7 @@ using namespace bl;
 extern "C" {
 
 int bl_totals(bl_handle_t* h, bl_totals_t* out) {
+  bl_touch(h);
   if (!h || !out) return BL_FAILURE;
   int nel = h->sz.nel;
   unsigned nb = nblocks(nel);
@@ -143,6 +144,7 @@ static int host_step(bl_handle_t* h, bl_timestep_t* ts, bl_error_t* err) {
 int bl_run(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_timestep_t* ts,
            double* dt_hist, int32_t* idt_hist, int32_t* ctl_hist, int32_t* steps_taken,
            bl_error_t* err) {
+  bl_touch(h);
   if (!h || !ts) return BL_FAILURE;
   int taken = 0, rc = BL_SUCCESS;
   fill_err(err, BL_SUCCESS, 0, "");

@@ -143,18 +143,25 @@ def _at_rest(g, nel):
 def test_run_stats_and_the_at_rest_mask(oracle_lib):
     """bl_run_stats against the elements at rest counted on the host; a
     12-step call (whose steps reuse the zeros of elements at rest in the step
-    before) against 12 one-step calls (each writing every element's) and the
-    oracle, bit for bit."""
+    before) against 12 one-step calls and the oracle, bit for bit.  Reads
+    between the one-step calls write nothing, so each call continues from
+    the last one's end (the same writes as the 12-step call); with an upload
+    between them, each call starts afresh and writes every element's zeros."""
     from bookleaf_f90_amd.device import BookLeaf
     from oracle.pyoracle import Oracle
     p = P.sedov(n=24, nsteps=12, kappa=0.01)
     nel = p.nel
-    a = BookLeaf(p)
+    a, c = BookLeaf(p), BookLeaf(p)
     rest = []
-    for _ in range(12):
+    for k in range(12):
         r = _at_rest(a, nel)
         a.run(1, runner="run_graph")
-        assert a.run_stats() == (int(nel - r.sum()), int(r.sum()))
+        new = r if k == 0 else r & ~rest[-1]
+        assert a.run_stats() == (int(nel - r.sum()), int(new.sum()))
+        # the same state uploaded unchanged: the call restarts
+        c.set("elenergy", c.get("elenergy"))
+        c.run(1, runner="run_graph")
+        assert c.run_stats() == (int(nel - r.sum()), int(r.sum()))
         rest.append(r)
     b = BookLeaf(p)
     b.run(12, runner="run_graph")
@@ -164,8 +171,58 @@ def test_run_stats_and_the_at_rest_mask(oracle_lib):
     assert zeroed < 12 * int(rest[-1].sum())  # the mask saved writes
     o = Oracle(p)
     o.run(12)
-    for g in (a, b):
+    for g in (a, b, c):
         assert diff_fields(g.state(), o.state(), compared(p)) == []
+
+
+def _write_through_pointer(g, name, values):
+    """A store the library does not see: hipMemcpy to bl_field_device_ptr."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    a = np.ascontiguousarray(values, np.float64)
+    assert hip.hipMemcpy(g.device_ptr(name), a.ctypes.data, a.nbytes, 1) == 0  # host to device
+
+
+@pytest.mark.parametrize("how", ["upload", "device_ptr"])
+def test_a_write_between_run_calls_restarts_the_call(how, oracle_lib):
+    """Velocities changed between two run calls, so that elements at rest
+    start to move: through bl_upload, or through a device pointer (after
+    which no call of that handle continues from the last one's end).  The
+    next call runs F1 again and the run stays bit-identical to the oracle
+    given the same change."""
+    p = P.sedov(n=24, nsteps=16, kappa=0.01)
+    g, o = _pair(p)
+    g.run(5, runner="run_graph")
+    o.run(5)
+    nel = p.nel
+    u = g.get("ndu")
+    idx = np.arange(0, p.nnd, 7)
+    u[idx] = u[idx] + 1.0e-3
+    if how == "upload":
+        g.set("ndu", u)
+    else:
+        _write_through_pointer(g, "ndu", u)
+    o.set("ndu", u)
+    for n in (4, 3):
+        g.run(n, runner="run_graph")
+        o.run(n)
+        assert diff_fields(g.state(), o.state(), compared(p)) == []
+    # one-step calls with nothing written between them: they continue from
+    # the last call's end (zeros only for elements at rest now but not in
+    # the step before), unless a device pointer was handed out (every
+    # element's zeros at rest, each call)
+    g2, _ = _pair(p)
+    if how == "device_ptr":
+        g2.device_ptr("ndu")
+    g2.run(2, runner="run_graph")
+    r1 = _at_rest(g2, nel)
+    g2.run(1, runner="run_graph")
+    r2 = _at_rest(g2, nel)
+    g2.run(1, runner="run_graph")
+    want = r2 if how == "device_ptr" else r2 & ~r1
+    assert g2.run_stats() == (int(nel - r2.sum()), int(want.sum()))
 
 
 @pytest.mark.parametrize("name", ["sedov_hg", "sod2mat_mixed_eul"])
