@@ -1998,43 +1998,55 @@ int build_nbfc(bl_handle_t* h) {
   return BL_SUCCESS;
 }
 
-// Steps per multi-step graph: consecutive graph launches leave a gap of
-// several microseconds on the device between them, kernels inside one graph
-// none, so a run launches kGraphSteps steps at a time (an even count: the
-// node set parity is the same after them).
-constexpr int kGraphSteps = 8;
+// Step graphs: consecutive graph launches leave a gap of several
+// microseconds on the device between them, kernels inside one graph none,
+// so a run call launches the fewest graphs its steps need (8, 4, 2 or 1
+// steps each, largest first; 8 steps took the Sedov step from 0.1524 to
+// 0.1486 ms), the last one with the batch's end in it.
+constexpr int kGraphLogs = 4;
+inline int graph_index(int fin, int si, int p) { return 8 * fin + 2 * si + p; }
 
-int capture_graphs(bl_handle_t* h, int stop_at_time_end) {
-  invalidate_graph(h);
-  if (h->cfg.zexist && (h->nbfc_stale || !h->d_nbfc) && build_nbfc(h)) return BL_FAILURE;
-  bool timing = h->timing;
+int launch_batch_end(bl_handle_t* h, const DevTimestep* d_ts, int cur);
+
+// capture graph_exec[graph_index(fin, si, p)]; before any launch of a call
+int capture_graph(bl_handle_t* h, int fin, int si, int p, int stop_at_time_end) {
+  const bool timing = h->timing;
   h->timing = false;  // no event records inside a capture
-  for (int q = 0; q < 4; ++q) {
-    // q < 2: one step from node set q; q >= 2: kGraphSteps from set q - 2
-    const int p = q & 1, steps = q < 2 ? 1 : kGraphSteps;
-    hipGraph_t g = nullptr;
-    BL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = BL_SUCCESS;
-    for (int k = 0; k < steps && rc == BL_SUCCESS; ++k)
-      rc = launch_step(h, p ^ (k & 1), stop_at_time_end);
-    hipError_t e = hipStreamEndCapture(h->stream, &g);
-    if (rc != BL_SUCCESS || e != hipSuccess) {
-      if (g) hipGraphDestroy(g);
-      h->timing = timing;
-      set_error("step graph capture failed");
-      return BL_FAILURE;
-    }
-    e = hipGraphInstantiate(&h->graph_exec[q], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      h->timing = timing;
-      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-      return BL_FAILURE;
-    }
-  }
+  hipGraph_t g = nullptr;
+  BL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = BL_SUCCESS;
+  for (int k = 0; k < (1 << si) && rc == BL_SUCCESS; ++k)
+    rc = launch_step(h, p ^ (k & 1), stop_at_time_end);
+  if (rc == BL_SUCCESS && fin) rc = launch_batch_end(h, h->d_ts, 0);
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
   h->timing = timing;
-  h->graph_stop_at_end = stop_at_time_end;
+  if (rc != BL_SUCCESS || e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    set_error("step graph capture failed");
+    return BL_FAILURE;
+  }
+  e = hipGraphInstantiate(&h->graph_exec[graph_index(fin, si, p)], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return BL_FAILURE;
+  }
   return BL_SUCCESS;
+}
+
+// the graphs of a run call of nsteps from node set p, in launch order:
+// (fin, si, parity) of each
+std::vector<int> graph_plan(int nsteps, int p) {
+  std::vector<int> plan;
+  for (int rem = nsteps; rem > 0;) {
+    int si = kGraphLogs - 1;
+    while ((1 << si) > rem) --si;
+    const int fin = rem == (1 << si) ? 1 : 0;
+    plan.push_back(graph_index(fin, si, p));
+    p ^= (1 << si) & 1;
+    rem -= 1 << si;
+  }
+  return plan;
 }
 
 // The end of a batch whose last step left its end open (no F1b after it),
@@ -2048,9 +2060,9 @@ __global__ void k_batch_end(DevTimestep* ts, const DtResult* res) {
 // registers, so they are rebuilt once from the final coordinates; the last
 // step's corrector corner forces go back to the element-major rscratch23/24
 // from the corner-major copy fxT, fyT (fxT null: the tile step's last step
-// wrote them there itself).  The final node set is (cur + ts->flips) & 1 of
-// (x0, y0), (x1, y1): read on the device, after the batch; with ts null it
-// is set cur.  (Writing the geometry in the batch's last tile step instead
+// wrote them there itself).  The final node set is (ts->set0 + ts->flips) & 1
+// of (x0, y0), (x1, y1): read on the device, after the batch; with ts null
+// it is set cur.  (Writing the geometry in the batch's last tile step instead
 // slowed every step's tile kernels by more than this pass costs once.)
 __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
                               const double* __restrict__ x0, const double* __restrict__ y0,
@@ -2065,7 +2077,7 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
                               double* __restrict__ cnfx, double* __restrict__ cnfy) {
   int e = xcd_index();
   if (e >= nel) return;
-  const bool other = ((ts ? cur + ts->flips : cur) & 1) != 0;
+  const bool other = ((ts ? ts->set0 + ts->flips : cur) & 1) != 0;
   const double* __restrict__ ndx = other ? x1 : x0;
   const double* __restrict__ ndy = other ? y1 : y0;
   if (fxT) {
@@ -2091,6 +2103,24 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
   b1[e] = g.b1; b2[e] = g.b2; b3[e] = g.b3;
   st4(cnwt, e, g.wt);
   vol[e] = g.vol;
+}
+
+// The batch's end: the open end of its last step, then the registry's
+// geometry at the final node set (from ts after the steps, or set cur with
+// ts null); the tile step's last step wrote rscratch23/24 itself
+int launch_batch_end(bl_handle_t* h, const DevTimestep* d_ts, int cur) {
+  if (end_in_f1b(h))
+    BL_LAUNCH(h, "batch_end", k_batch_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
+  const int nel = h->sz.nel;
+  const bool tiles = use_tiles(h);
+  BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
+            h->nodeset[0][0], h->nodeset[0][1], h->nodeset[1][0], h->nodeset[1][1], d_ts, cur,
+            h->R(BL_CNX), h->R(BL_CNY), h->R(BL_A1), h->R(BL_A2), h->R(BL_A3), h->R(BL_B1),
+            h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT), h->R(BL_ELVOLUME),
+            tiles ? nullptr : (const double*)h->cterm[0],
+            tiles ? nullptr : (const double*)h->cterm[1], h->R(BL_RSCRATCH23),
+            h->R(BL_RSCRATCH24));
+  return BL_SUCCESS;
 }
 
 void fill_err(bl_error_t* err, int ierr, int iout, const char* msg) {
@@ -2587,54 +2617,47 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
     h->hist_cap = cap;
     invalidate_graph(h);
   }
-  // the graphs bake in the node set pointers, which are fixed; capture once
-  if (!h->graph_exec[0] || h->graph_stop_at_end != stop_at_time_end)
-    if (capture_graphs(h, stop_at_time_end)) return BL_FAILURE;
+  // the graphs bake in the node set pointers, which are fixed; each is
+  // captured once, before the call's first launch
+  if (h->graph_stop_at_end != stop_at_time_end) {
+    invalidate_graph(h);
+    h->graph_stop_at_end = stop_at_time_end;
+  }
+  if (h->cfg.zexist && (h->nbfc_stale || !h->d_nbfc) && build_nbfc(h)) return BL_FAILURE;
+  // (all of them at once, so that a later call of another length finds
+  // its graphs: a capture costs about 0.1 ms)
+  const std::vector<int> plan = graph_plan(h->timing ? 0 : nsteps, h->cur_set);
+  bool missing = false;
+  for (int q : plan) missing = missing || !h->graph_exec[q];
+  for (int q = 0; missing && q < 16; ++q)
+    if (!h->graph_exec[q] && capture_graph(h, q >> 3, (q >> 1) & 3, q & 1, stop_at_time_end))
+      return BL_FAILURE;
   DevTimestep* d = h->h_ts;
   std::memset(d, 0, sizeof(DevTimestep));
   d->time = ts->time; d->dt = ts->dt; d->dts = ts->dts;
   d->nstep = ts->nstep; d->idtel = ts->idtel; d->sdt = ts->sdt;
   d->last_step = ts->nstep + nsteps;
+  d->set0 = h->cur_set;
   if (clear_rest(h, d, resume)) return BL_FAILURE;
   const bool tiles = use_tiles(h);
   int p = h->cur_set;
   // F1 of the first step: on a resumed call the last call's last step left
   // its terms, its at-rest mask and the zeros it stands for
   if (fold_f1(h) && !resume && launch_f1_prologue(h, p, stop_at_time_end)) return BL_FAILURE;
-  for (int k = 0; k < nsteps;) {
-    // with kernel timing on, the same step kernels are launched directly so
-    // each launch can be bracketed by HIP events on this stream
-    if (h->timing) {
+  // with kernel timing on, the same step kernels are launched directly so
+  // each launch can be bracketed by HIP events on this stream
+  if (h->timing)
+    for (int k = 0; k < nsteps; ++k, p ^= 1)
       if (launch_step(h, p, stop_at_time_end)) return BL_FAILURE;
-    } else if (nsteps - k >= kGraphSteps) {
-      BL_HIP(hipGraphLaunch(h->graph_exec[2 + p], h->stream));
-      k += kGraphSteps;  // even: the parity after them is p again
-      continue;
-    } else {
-      BL_HIP(hipGraphLaunch(h->graph_exec[p], h->stream));
-    }
-    p ^= 1;
-    ++k;
-  }
+  for (int q : plan) BL_HIP(hipGraphLaunch(h->graph_exec[q], h->stream));
   // everything after the batch is queued behind it with no host round trip:
-  // the open end of the last step, the geometry at the final node set (every
-  // step whose F3 ran wrote the other set; the device knows how many did),
-  // then one read of the timestep, the dt history (at most one entry per
-  // step) and the error state
-  if (end_in_f1b(h))
-    BL_LAUNCH(h, "batch_end", k_batch_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
-  {
-    // (the tile step's last step wrote rscratch23/24 itself)
-    int nel = h->sz.nel;
-    BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
-              h->nodeset[0][0], h->nodeset[0][1], h->nodeset[1][0], h->nodeset[1][1],
-              (const DevTimestep*)h->d_ts, h->cur_set, h->R(BL_CNX), h->R(BL_CNY),
-              h->R(BL_A1), h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3),
-              h->R(BL_CNWT), h->R(BL_ELVOLUME), tiles ? nullptr : (const double*)h->cterm[0],
-              tiles ? nullptr : (const double*)h->cterm[1], h->R(BL_RSCRATCH23),
-              h->R(BL_RSCRATCH24));
-    if (!h->cfg.zexist && mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
-  }
+  // the batch's end (in the last graph, or launched here: the open end of
+  // the last step, the geometry at the final node set, which every step
+  // whose F3 ran flipped; the device knows how many did), then one read of
+  // the timestep, the dt history (at most one entry per step) and the error
+  // state
+  if (plan.empty() && launch_batch_end(h, h->d_ts, 0)) return BL_FAILURE;
+  if (!h->cfg.zexist && mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
   std::vector<double> hist(3 * (size_t)std::max(nsteps, 1));
   BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
   if (nsteps > 0)

@@ -68,6 +68,8 @@ struct DevTimestep {
   // outcome and the node-set flip) the next step's F1b, or the host after
   // the batch, still has to record
   int32_t open;
+  // the node set the batch started from (the final one is set0 + flips)
+  int32_t set0;
 };
 constexpr int32_t kNoPending = 0x7fffffff;
 
@@ -139,9 +141,11 @@ struct Handle {
   // remap temporaries of the fused step (zexist): 4*nel each for the face
   // and corner arrays, nel each for the element scalars
   double* ale_tmp[11] = {};
-  // [p]: one step from node set p; [2 + p]: kGraphSteps steps from node set
-  // p (bl_run_graph launches those while at least that many steps remain)
-  hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  // step graphs, each captured when a run call first needs it: [8 * fin +
+  // 2 * si + p] runs 2^si steps (si = 0..3) from node set p, then with fin
+  // the batch's end (k_batch_end, k_materialize).  Ensembles use [p], one
+  // step from node set p.
+  hipGraphExec_t graph_exec[16] = {};
   // ensembles (bl_ensemble_create): instances stepped together
   int32_t ens_n = 0;
   std::vector<int32_t> ens_eoff, ens_noff;  // host copies, ens_n + 1 each

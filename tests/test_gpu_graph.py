@@ -185,6 +185,41 @@ def _write_through_pointer(g, name, values):
     assert hip.hipMemcpy(g.device_ptr(name), a.ctypes.data, a.nbytes, 1) == 0  # host to device
 
 
+def _read_through_pointer(g, name):
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    out = np.empty(g._info(name)[1], np.float64)
+    assert hip.hipMemcpy(out.ctypes.data, g.device_ptr(name), out.nbytes, 2) == 0  # to host
+    return out
+
+
+def test_geometry_after_run_calls(oracle_lib):
+    """The registry's geometry after run calls that continue one another
+    (the second keeps the first's masks and F1 terms): a download and a
+    device pointer each see the geometry of the final node set, as the
+    oracle has it, and so do calls after the pointer was handed out."""
+    p = P.sedov(n=24, nsteps=12, kappa=0.01)
+    g, o = _pair(p)
+    g.run(3, runner="run_graph")
+    g.run(4, runner="run_graph")
+    o.run(7)
+    geo = ["a1", "a2", "a3", "b1", "b2", "b3", "cnwt", "cnx", "cny", "elvolume",
+           "rscratch23", "rscratch24"]
+    for f in geo:
+        assert np.array_equal(g.get(f), o.get(f)), f
+    g.run(2, runner="run_graph")
+    o.run(2)
+    for f in ("a1", "cnwt", "elvolume"):
+        assert np.array_equal(_read_through_pointer(g, f), o.get(f)), f
+    # a device pointer is out: the next calls start afresh
+    g.run(2, runner="run_graph")
+    o.run(2)
+    assert np.array_equal(_read_through_pointer(g, "cnwt"), o.get("cnwt"))
+    assert diff_fields(g.state(), o.state(), compared(p)) == []
+
+
 @pytest.mark.parametrize("how", ["upload", "device_ptr"])
 def test_a_write_between_run_calls_restarts_the_call(how, oracle_lib):
     """Velocities changed between two run calls, so that elements at rest
