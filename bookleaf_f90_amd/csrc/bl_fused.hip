@@ -1100,12 +1100,17 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   const int tid = threadIdx.x;
   const int eb = A.tile_eptr[t], ne = A.tile_eptr[t + 1] - eb;
   const int nb = A.tile_nptr[t], nn = A.tile_nptr[t + 1] - nb;
-  // the owned node's start-of-step values, loaded now for F3 below
+  // the tile's node's start-of-step values, loaded now for F3 below; the
+  // high bit of its entry: this tile writes it (other tiles holding it
+  // compute the same values for their own elements)
   const bool own = tid < nn;
   int n = 0, ntype = 0;
+  bool writes = false;
   double x0 = 0.0, y0 = 0.0, u0 = 0.0, v0 = 0.0;
   if (own) {
-    n = A.tile_nodes[nb + tid];
+    const uint32_t nv = (uint32_t)A.tile_nodes[nb + tid];
+    n = (int)(nv & 0x7fffffffu);
+    writes = (nv >> 31) != 0;
     x0 = N.x[n]; y0 = N.y[n]; u0 = N.u[n]; v0 = N.v[n];
     ntype = A.indtype[n];
 #pragma unroll
@@ -1173,12 +1178,14 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
       }
     double out[6];
     f3_node(A, dt, m, a, fu, fv, x0, y0, u0, v0, ntype, out);
-    O.x[n] = out[0];
-    O.y[n] = out[1];
-    O.u[n] = out[2];
-    O.v[n] = out[3];
-    A.ubar[n] = out[4];
-    A.vbar[n] = out[5];
+    if (writes) {
+      O.x[n] = out[0];
+      O.y[n] = out[1];
+      O.u[n] = out[2];
+      O.v[n] = out[3];
+      A.ubar[n] = out[4];
+      A.vbar[n] = out[5];
+    }
 #pragma unroll
     for (int k = 0; k < 6; ++k) nout[k][tid] = out[k];
   }

@@ -6,9 +6,9 @@ namespace bl {
 
 struct Handle;
 
-constexpr int kTileNodes = 255;   // owned nodes per tile (an 8-bit slot, kNoSlot free)
+constexpr int kTileNodes = 255;   // nodes per tile (an 8-bit slot, kNoSlot free)
 constexpr int kTileEls = 256;     // elements around a tile's nodes: one per thread
-constexpr int kTileTarget = 200;  // the node count the bisection aims at
+constexpr int kTileTarget = 180;  // the element count the bisection aims at
 constexpr uint32_t kNoSlot = 0xFF;
 constexpr uint32_t kTileWriter = 1;    // this tile writes the element's side outputs
 constexpr uint32_t kTileInterior = 2;  // all four nodes in this tile

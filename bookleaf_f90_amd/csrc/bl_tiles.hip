@@ -36,42 +36,58 @@ namespace bl {
 
 namespace {
 
+// Recursive coordinate bisection of the elements by their centroids: each
+// cut along the longer side of the element set's bounding box, at the share
+// that keeps the tile counts of the two sides proportional, down to about
+// kTileTarget elements; a leaf whose nodes or the elements around them do
+// not fit one workgroup is cut again.
 struct Rcb {
-  const std::vector<double>& x;
-  const std::vector<double>& y;
+  const std::vector<double>& cx;
+  const std::vector<double>& cy;
+  const std::vector<int32_t>& ielnd;     // 4 per element, 1-based
   const std::vector<int32_t>& adj_ptr;  // node -> incident elements
   const std::vector<int32_t>& adj;
-  std::vector<int32_t> idx;              // node ids, cut in place
-  std::vector<int32_t> stamp;            // per element: last tile counted
+  std::vector<int32_t> idx;              // element ids, cut in place
+  std::vector<int32_t> nstamp;           // per node: last count that saw it
+  std::vector<int32_t> estamp;           // per element: the same
   std::vector<std::pair<int, int>> leaves;
   int counter = 0;
 
-  int incident(int lo, int hi) {
+  // the nodes of idx[lo, hi) fit the tile's node slots and the elements
+  // around them one workgroup
+  bool fits(int lo, int hi) {
     ++counter;
-    int n = 0;
-    for (int k = lo; k < hi; ++k) {
-      const int v = idx[k];
-      for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q)
-        if (stamp[adj[q]] != counter) { stamp[adj[q]] = counter; ++n; }
-    }
-    return n;
+    int nn = 0, ne = 0;
+    for (int k = lo; k < hi; ++k)
+      for (int j = 0; j < 4; ++j) {
+        const int v = ielnd[4 * (size_t)idx[k] + j] - 1;
+        if (nstamp[v] == counter) continue;
+        nstamp[v] = counter;
+        if (++nn > kTileNodes) return false;
+        for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q)
+          if (estamp[adj[q]] != counter) {
+            estamp[adj[q]] = counter;
+            if (++ne > kTileEls) return false;
+          }
+      }
+    return true;
   }
 
   void cut(int lo, int hi, int k) {
     if (k <= 1) {
-      if ((hi - lo <= kTileNodes && incident(lo, hi) <= kTileEls) || hi - lo < 2) {
+      if (hi - lo < 2 || fits(lo, hi)) {
         leaves.emplace_back(lo, hi);
         return;
       }
       k = 2;
     }
-    double x0 = x[idx[lo]], x1 = x0, y0 = y[idx[lo]], y1 = y0;
+    double x0 = cx[idx[lo]], x1 = x0, y0 = cy[idx[lo]], y1 = y0;
     for (int q = lo + 1; q < hi; ++q) {
-      const int v = idx[q];
-      x0 = std::min(x0, x[v]); x1 = std::max(x1, x[v]);
-      y0 = std::min(y0, y[v]); y1 = std::max(y1, y[v]);
+      const int e = idx[q];
+      x0 = std::min(x0, cx[e]); x1 = std::max(x1, cx[e]);
+      y0 = std::min(y0, cy[e]); y1 = std::max(y1, cy[e]);
     }
-    const std::vector<double>& c = (x1 - x0 >= y1 - y0) ? x : y;
+    const std::vector<double>& c = (x1 - x0 >= y1 - y0) ? cx : cy;
     const int k1 = k / 2;
     int m = lo + (int)((int64_t)(hi - lo) * k1 / k);
     m = std::max(lo + 1, std::min(hi - 1, m));
@@ -135,66 +151,77 @@ int build_tiles(Handle* h) {
   std::vector<int32_t> adj(adj_ptr[nnd]), fill(adj_ptr.begin(), adj_ptr.end() - 1);
   for (int e = 0; e < nel; ++e)
     for (int j = 0; j < 4; ++j) adj[fill[ielnd[4 * (size_t)e + j] - 1]++] = e;
-  if (nel >= (1 << 30)) { h->tile_state = -1; return BL_SUCCESS; }
-  Rcb r{x, y, adj_ptr, adj, {}, std::vector<int32_t>(nel, 0), {}, 0};
-  r.idx.resize(nnd);
-  for (int v = 0; v < nnd; ++v) r.idx[v] = v;
-  r.cut(0, nnd, std::max(1, (nnd + kTileTarget - 1) / kTileTarget));
+  if (nel >= (1 << 30) || nnd >= (1 << 31) - 1) { h->tile_state = -1; return BL_SUCCESS; }
+  std::vector<double> cx(nel), cy(nel);
+  for (int e = 0; e < nel; ++e) {
+    const int32_t* q = &ielnd[4 * (size_t)e];
+    cx[e] = 0.25 * (x[q[0] - 1] + x[q[1] - 1] + x[q[2] - 1] + x[q[3] - 1]);
+    cy[e] = 0.25 * (y[q[0] - 1] + y[q[1] - 1] + y[q[2] - 1] + y[q[3] - 1]);
+  }
+  Rcb r{cx, cy, ielnd, adj_ptr, adj, {}, std::vector<int32_t>(nnd, 0),
+        std::vector<int32_t>(nel, 0), {}, 0};
+  r.idx.resize(nel);
+  for (int e = 0; e < nel; ++e) r.idx[e] = e;
+  r.cut(0, nel, std::max(1, (nel + kTileTarget - 1) / kTileTarget));
 
   const int ntile = (int)r.leaves.size();
-  std::vector<int32_t> tile_of(nnd), slot_of(nnd), nptr(ntile + 1, 0), nodes(nnd);
-  int pos = 0;
-  for (int t = 0; t < ntile; ++t) {
-    const auto lr = r.leaves[t];
-    std::sort(r.idx.begin() + lr.first, r.idx.begin() + lr.second);
-    if (lr.second - lr.first > kTileNodes) { h->tile_state = -1; return BL_SUCCESS; }
-    nptr[t] = pos;
-    for (int k = lr.first; k < lr.second; ++k) {
-      const int v = r.idx[k];
-      tile_of[v] = t;
-      slot_of[v] = k - lr.first;
-      nodes[pos++] = v;
-    }
-  }
-  nptr[ntile] = pos;
-  // each tile's incident elements, ascending; the boundary list in tile order
-  std::vector<int32_t> eptr(ntile + 1, 0), bnd, bslot(nel, -1);
+  std::vector<int32_t> tile_of(nel);
+  for (int t = 0; t < ntile; ++t)
+    for (int k = r.leaves[t].first; k < r.leaves[t].second; ++k) tile_of[r.idx[k]] = t;
+  // per tile: its nodes (ascending; the high bit marks the tile that writes
+  // the node, the one holding the node's first incident element), then the
+  // entries: its own elements ascending, then the ring around them
+  std::vector<int32_t> nptr(ntile + 1, 0), nodes, eptr(ntile + 1, 0), bnd, bslot(nel, -1);
   std::vector<int2> ents;
   std::vector<int4> ent_nd;
-  ents.reserve((size_t)nel + nel / 4);
-  ent_nd.reserve((size_t)nel + nel / 4);
-  std::vector<int32_t> els;
-  std::vector<int32_t> stamp(nel, -1);
+  nodes.reserve((size_t)nnd + nnd / 4);
+  ents.reserve((size_t)nel + nel / 2);
+  ent_nd.reserve((size_t)nel + nel / 2);
+  std::vector<int32_t> nstamp(nnd, -1), slot_of(nnd, 0), estamp(nel, -1), tn, own, ring;
   for (int t = 0; t < ntile; ++t) {
-    els.clear();
-    for (int k = nptr[t]; k < nptr[t + 1]; ++k) {
-      const int v = nodes[k];
-      for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q)
-        if (stamp[adj[q]] != t) { stamp[adj[q]] = t; els.push_back(adj[q]); }
-    }
-    if ((int)els.size() > kTileEls) { h->tile_state = -1; return BL_SUCCESS; }
-    std::sort(els.begin(), els.end());
-    eptr[t] = (int)ents.size();
-    for (int e : els) {
-      uint32_t slots = 0;
-      bool interior = true;
+    own.assign(r.idx.begin() + r.leaves[t].first, r.idx.begin() + r.leaves[t].second);
+    std::sort(own.begin(), own.end());
+    tn.clear();
+    for (int e : own) {
+      estamp[e] = t;
       for (int j = 0; j < 4; ++j) {
         const int v = ielnd[4 * (size_t)e + j] - 1;
-        const bool own = tile_of[v] == t;
-        interior = interior && own;
-        slots |= (uint32_t)(own ? slot_of[v] : kNoSlot) << (8 * j);
+        if (nstamp[v] != t) { nstamp[v] = t; tn.push_back(v); }
       }
-      const bool writer = tile_of[ielnd[4 * (size_t)e] - 1] == t;
-      if (writer && !interior) {
-        bslot[e] = (int32_t)bnd.size();
-        bnd.push_back(e);
-      }
-      const uint32_t flags = (writer ? kTileWriter : 0u) | (interior ? kTileInterior : 0u);
-      ents.push_back(make_int2((int)((uint32_t)e | (flags << 30)), (int)slots));
-      const int32_t* q = &ielnd[4 * (size_t)e];
-      ent_nd.push_back(make_int4(q[0] - 1, q[1] - 1, q[2] - 1, q[3] - 1));
     }
+    std::sort(tn.begin(), tn.end());
+    ring.clear();
+    for (int v : tn)
+      for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q)
+        if (estamp[adj[q]] != t) { estamp[adj[q]] = t; ring.push_back(adj[q]); }
+    std::sort(ring.begin(), ring.end());
+    if ((int)tn.size() > kTileNodes || own.size() + ring.size() > (size_t)kTileEls) {
+      h->tile_state = -1;
+      return BL_SUCCESS;
+    }
+    nptr[t] = (int)nodes.size();
+    for (size_t k = 0; k < tn.size(); ++k) {
+      const int v = tn[k];
+      slot_of[v] = (int)k;
+      const bool writes = tile_of[adj[adj_ptr[v]]] == t;
+      nodes.push_back((int32_t)((uint32_t)v | (writes ? 0x80000000u : 0u)));
+    }
+    eptr[t] = (int)ents.size();
+    for (int pass = 0; pass < 2; ++pass)
+      for (int e : pass == 0 ? own : ring) {
+        uint32_t slots = 0;
+        for (int j = 0; j < 4; ++j) {
+          const int v = ielnd[4 * (size_t)e + j] - 1;
+          slots |= (uint32_t)(nstamp[v] == t ? slot_of[v] : kNoSlot) << (8 * j);
+        }
+        // an own element is this tile's to write and has every node here
+        const uint32_t flags = pass == 0 ? (kTileWriter | kTileInterior) : 0u;
+        ents.push_back(make_int2((int)((uint32_t)e | (flags << 30)), (int)slots));
+        const int32_t* q = &ielnd[4 * (size_t)e];
+        ent_nd.push_back(make_int4(q[0] - 1, q[1] - 1, q[2] - 1, q[3] - 1));
+      }
   }
+  nptr[ntile] = (int)nodes.size();
   eptr[ntile] = (int)ents.size();
   const int nbnd = (int)bnd.size();
   if (upload_vec(&h->d_tile_eptr, eptr) || upload_vec(&h->d_tile_nptr, nptr) ||
