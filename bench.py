@@ -129,36 +129,28 @@ FOLD_BYTES = {"f4_state": (0.125, 16)}
 
 
 # The tile step (a pure-cell Lagrangian step: F2, F3 and F4 per tile of
-# nodes, bl_tiles.hip), priced from bl_get_tiling's counts: nent entries
-# (an element around a tile's nodes, once per tile that holds one of its
-# nodes), nbnd boundary elements (nodes in two or more tiles).
+# elements and the ring of elements around their nodes, bl_tiles.hip),
+# priced from bl_get_tiling's counts: nent entries (each element once in its
+# own tile and once in the ring of every other tile holding one of its
+# nodes).
 #  lag_tile: per entry its element id and node slots 8, node ids 16; per
 #    element once its at-rest byte 1, p, m, e 24, ielmat, ielreg 8, cnmass
-#    32, and the next at-rest byte written 1; per interior element rho, e, p,
-#    cs2 written 32; per boundary element its corner forces written 64 (for
-#    f4_boundary) | per node the tile's node list 4, x, y, u, v 32, indtype
-#    4; x, y, u, v, ubar, vbar written 48; per element in motion cnviscx/y
-#    64; per element whose zero viscosity it writes 72.  Halo entries read
-#    their elements' values a second time through the cache: not counted.
-#  f4_boundary: per boundary element its id 4, ielnd 16, corner forces 64,
-#    m, e 16, ielmat 4, x, y, u, v, ubar, vbar of its nodes 96 (a strip of
-#    elements, each node shared by two), rho, e, p, cs2 and the at-rest byte
-#    written 33.
-#  f1_seg (F1 of a run call's first step, once per call): per entry 8 and per
-#    element ielnd 16, cs2 8, the at-rest byte written 1 | x, y, u, v 32.
-TILE_KERNELS = ("lag_tile", "f4_boundary", "f1_seg")
+#    32, the next at-rest byte written 1, and rho, e, p, cs2 written 32 |
+#    per node once the tile's node list 4, x, y, u, v 32, indtype 4; x, y,
+#    u, v, ubar, vbar written 48; per element in motion cnviscx/y 64; per
+#    element whose zero viscosity it writes 72.  Ring entries and the nodes
+#    a tile shares with others read their values a second time through the
+#    cache: not counted.
+#  f1_seg (F1 of a fresh run call's first step): per entry 8 and per element
+#    ielnd 16, cs2 8, the at-rest byte written 1 | x, y, u, v 32.
+TILE_KERNELS = ("lag_tile", "f1_seg")
 
 
 def tile_bytes(k, nel, nnd, moving, zeroed, tiles):
-    _, nbnd, nent = tiles
+    _, _, nent = tiles
     if k == "lag_tile":
-        return (24 * nent + 66 * nel + 32 * (nel - nbnd) + 64 * nbnd + 88 * nnd +
-                64 * moving + 72 * zeroed)
-    if k == "f4_boundary":
-        return 233 * nbnd
-    # two launches a call (the tiles' interior elements, then the boundary
-    # list): half of the bytes each on average, as the timing averages both
-    return (8 * nent + 25 * nel + 32 * nnd) / 2
+        return 24 * nent + 98 * nel + 88 * nnd + 64 * moving + 72 * zeroed
+    return 8 * nent + 25 * nel + 32 * nnd
 
 
 # before a remap F4 also writes the corner weights and the volume at x^{n+1}
@@ -499,8 +491,7 @@ def main():
         "config": {"workload": wname, "nel": nel, "nnd": nnd, "runner": args.runner,
                    "diagnostics": bool(args.diagnostics),
                    "tile_step": (None if tiles is None else
-                                 {"tiles": tiles[0], "boundary_elements": tiles[1],
-                                  "entries": tiles[2]}),
+                                 {"tiles": tiles[0], "entries": tiles[2]}),
                    "launch": tune if tune is not None else g.get_tuning()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -119,20 +119,17 @@ struct FusedArgs {
   double* hist;
   int hist_cap;
   MixArgs mx;  // mixed cells (mx.ncp == 0: none)
-  // the tile step (bl_tiles.h): tiles, their element entries (element |
-  // flags << 30, four 8-bit local node slots), the boundary list and its
-  // corner forces, and the at-rest bytes of the step (rest8: now, rest8_prev:
-  // the step before, rest8_next: written for the next step); the F1 partial
-  // slots (nseg) are the ntile tiles' then the boundary blocks
-  int ntile, nbnd, nseg;
+  // the tile step (bl_tiles.h): tiles, their nodes (0-based | the tile
+  // writes it << 31), their element entries (element | flags << 30, four
+  // 8-bit local node slots) and the at-rest bytes of the step (rest8: now,
+  // rest8_prev: the step before, rest8_next: written for the next step);
+  // one F1 partial slot per tile (nseg = ntile)
+  int ntile, nseg;
   const int32_t* tile_eptr;
   const int32_t* tile_nptr;
   const int32_t* tile_nodes;
   const int2* tile_el;
   const int4* tile_nd;  // the entry's four nodes (0-based), beside tile_el
-  const int32_t* bnd_el;
-  const int32_t* el_bslot;
-  double* bnd_f;
   const uint8_t* rest8;
   const uint8_t* rest8_prev;
   uint8_t* rest8_next;

@@ -28,8 +28,8 @@
 //                step or the next F1b can record them; F2m, F4m: the mixed
 //                cells' component terms, in F2's and F5's extra workgroups
 //   tile step  : in a Lagrangian step of pure cells, F2, F3 and F4 in one
-//                kernel per tile of nodes (k_lag_tile, bl_tiles.hip) and
-//                k_f4_boundary for the elements shared between tiles
+//                kernel per tile of elements and the ring around them
+//                (k_lag_tile, bl_tiles.hip)
 //   R1..R5     : the remap (ale_dr_*), after the Lagrangian step when ALE is
 //                active
 //
@@ -1063,9 +1063,12 @@ k_f4_state_ens(FusedArgs A, FusedNodes O, EnsArgs E) {
 
 // ------------------------------------------------------------ the tile step
 // F2, F3 and F4 of a pure-cell Lagrangian step in one kernel per tile of
-// nodes (bl_tiles.hip): the corner terms of F2 stay in LDS for the tile's
-// node sums, the node results stay in LDS for the corrector state of the
-// elements the tile holds whole.  The arithmetic is F2's, F3's and F4's own
+// elements (bl_tiles.hip): F2 of the tile's own elements and of the ring of
+// elements around their nodes, whose corner terms stay in LDS for the sums
+// of every node of the own elements; the node results stay in LDS for the
+// corrector state of the own elements, which the tile completes.  A node
+// shared by several tiles is computed by each (the same inputs, the same
+// bits) and written by one.  The arithmetic is F2's, F3's and F4's own
 // (f2_element, f3_node, f4_state_of), so the results are the same bits.  A
 // node's corner terms are summed over jj = 1..4 in that order: on a mesh
 // where no node is corner jj of two elements (build_tiles checks it) this is
@@ -1131,7 +1134,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
     e = ent.x & 0x3fffffff;
     flags = (uint32_t)ent.x >> 30;
     slots = (uint32_t)ent.y;
-    const bool writer = (flags & kTileWriter) != 0;
+    const bool writer = (flags & kTileOwn) != 0;
     const bool at_rest = A.rest8[e] != 0;
     double wt[4], mt[4];
     f2_element(A, N, e, nd, dt, ts->rdt, at_rest, writer, fx, fy, wt, mt);
@@ -1147,13 +1150,8 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
       }
     }
     if (writer) {
-      // a boundary element's corner forces for k_f4_boundary; the batch's
-      // last step leaves them in rscratch23/24 (lagstep cnfx/cnfy)
-      if (!(flags & kTileInterior)) {
-        double* bf = A.bnd_f + 8 * (size_t)A.el_bslot[e];
-        st4(bf, 0, fx);
-        st4(bf, 1, fy);
-      }
+      // the batch's last step leaves the corner forces in rscratch23/24
+      // (lagstep cnfx/cnfy)
       if (ts->last) {
         st4(A.cnfx, e, fx);
         st4(A.cnfy, e, fy);
@@ -1191,7 +1189,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   }
   __syncthreads();
   // --- F4 and the next step's F1 terms: the elements held whole
-  const bool interior = active && (flags & kTileInterior);
+  const bool interior = active && (flags & kTileOwn);
   double cflv = DBL_MAX, divv = -1.0;
   if (interior) {
     double x[4], y[4], u[4], v[4], ub[4], vb[4];
@@ -1214,56 +1212,18 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   seg_collect(A, t, A.nseg, interior, e, cflv, divv);
 }
 
-// F4 and the next step's F1 terms of the boundary list's elements
-__global__ void __launch_bounds__(kBlock)
-k_f4_boundary(FusedArgs A, FusedNodes O) {
-  const DevTimestep* ts = A.ts;
-  if (ts->skip || A.dtres->negvol) return;
-  const int b = xcd_block();
-  const int k = b * kBlock + threadIdx.x;
-  const bool active = k < A.nbnd;
-  int e = 0;
-  double cflv = DBL_MAX, divv = -1.0;
-  if (active) {
-    bool moving;
-    e = A.bnd_el[k];
-    int nd[4];
-    load_nd(A.ielnd, e, nd);
-    double x[4], y[4], u[4], v[4], ub[4], vb[4], fx[4], fy[4];
-    load4(O.x, nd, x); load4(O.y, nd, y); load4(A.ubar, nd, ub); load4(A.vbar, nd, vb);
-    ld4(A.bnd_f, 2 * k, fx);
-    ld4(A.bnd_f, 2 * k + 1, fy);
-    Iso g;
-    const double c2 = f4_state_of(A, e, ts->dt, x, y, ub, vb, fx, fy, g);
-    load4(O.u, nd, u); load4(O.v, nd, v);
-    f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
-    A.rest8_next[e] = moving ? 0 : 1;
-  }
-  seg_collect(A, A.ntile + b, A.nseg, active, e, cflv, divv);
-}
-
 // F1 of a tile-step run call's first step, in the tile step's segments:
-// one workgroup per tile over its interior elements, then (boundary) one
-// per boundary block; the at-rest bytes of that step go to rest8_next
+// one workgroup per tile over its own elements; the at-rest bytes of that
+// step go to rest8_next
 __global__ void __launch_bounds__(kBlock)
-k_f1_seg(FusedArgs A, FusedNodes N, int boundary) {
+k_f1_seg(FusedArgs A, FusedNodes N) {
   if (step_skipped(A, A.ts)) return;
   const int tid = threadIdx.x;
   const int b = xcd_block();
-  int e = 0, seg;
-  bool active;
-  if (!boundary) {
-    const int eb = A.tile_eptr[b], ne = A.tile_eptr[b + 1] - eb;
-    const int2 ent = tid < ne ? A.tile_el[eb + tid] : make_int2(0, 0);
-    active = tid < ne && (((uint32_t)ent.x >> 30) & kTileInterior);
-    e = ent.x & 0x3fffffff;
-    seg = b;
-  } else {
-    const int k = b * kBlock + tid;
-    active = k < A.nbnd;
-    if (active) e = A.bnd_el[k];
-    seg = A.ntile + b;
-  }
+  const int eb = A.tile_eptr[b], ne = A.tile_eptr[b + 1] - eb;
+  const int2 ent = tid < ne ? A.tile_el[eb + tid] : make_int2(0, 0);
+  const bool active = tid < ne && (((uint32_t)ent.x >> 30) & kTileOwn);
+  const int e = ent.x & 0x3fffffff, seg = b;
   double cflv = DBL_MAX, divv = -1.0;
   if (active) {
     bool moving;
@@ -1740,16 +1700,12 @@ bool use_tiles(const bl_handle_t* h) {
 void tile_args(bl_handle_t* h, FusedArgs& A, int p) {
   const int nel = h->sz.nel;
   A.ntile = h->ntile;
-  A.nbnd = h->nbnd;
-  A.nseg = h->ntile + (int)nblocks(h->nbnd);
+  A.nseg = h->ntile;
   A.tile_eptr = h->d_tile_eptr;
   A.tile_nptr = h->d_tile_nptr;
   A.tile_nodes = h->d_tile_nodes;
   A.tile_el = h->d_tile_el;
   A.tile_nd = h->d_tile_nd;
-  A.bnd_el = h->d_bnd_el;
-  A.el_bslot = h->d_el_bslot;
-  A.bnd_f = h->d_bnd_f;
   A.rho = h->elset[p][0]; A.en = h->elset[p][1]; A.pre = h->elset[p][2]; A.cs2 = h->elset[p][3];
   A.rho_o = h->elset[1 - p][0]; A.en_o = h->elset[1 - p][1];
   A.pre_o = h->elset[1 - p][2]; A.cs2_o = h->elset[1 - p][3];
@@ -1776,8 +1732,7 @@ int launch_f1_prologue(bl_handle_t* h, int p, int stop_at_time_end) {
     tile_args(h, A, p);
     A.rest8_next = h->d_rest8 + (size_t)p * h->sz.nel;  // the first step's own bytes
     FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
-    BL_LAUNCH(h, "f1_seg", k_f1_seg, (unsigned)h->ntile, kBlock, A, N, 0);
-    if (h->nbnd > 0) BL_LAUNCH(h, "f1_seg", k_f1_seg, nblocks(h->nbnd), kBlock, A, N, 1);
+    BL_LAUNCH(h, "f1_seg", k_f1_seg, (unsigned)h->ntile, kBlock, A, N);
     return BL_SUCCESS;
   }
   uint64_t* m[2];
@@ -1807,8 +1762,6 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nseg);
     BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kF1bThreads, A, (int)nseg, (int)nbl);
     BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O);
-    if (h->nbnd > 0)
-      BL_LAUNCH(h, "f4_boundary", k_f4_boundary, nblocks(h->nbnd), kBlock, A, Oc);
     return BL_SUCCESS;
   }
   const unsigned nbl = std::min(nbe, (unsigned)h->tune_lim_grid);
@@ -2566,7 +2519,7 @@ int bl_get_tiling(bl_handle_t* h, int32_t* on, int32_t* ntile, int32_t* nboundar
   const bool built = h->tile_state == 1;
   if (on) *on = (h->tiles_on && built && end_in_f1b(h)) ? 1 : 0;
   if (ntile) *ntile = built ? h->ntile : 0;
-  if (nboundary) *nboundary = built ? h->nbnd : 0;
+  if (nboundary) *nboundary = 0;  // every element completes in its own tile
   if (nentries) *nentries = built ? h->nentries : 0;
   return BL_SUCCESS;
 }

@@ -187,20 +187,17 @@ struct Handle {
   int32_t ens_lblocks_cap = 0;    // limiter partial slots allocated
   bool nbfc_stale = true;         // rebuild it before the next capture
   // the tile step of a pure-cell Lagrangian problem (bl_tiles.hip): the
-  // nodes cut into tiles of at most kTileNodes, each with the elements
-  // around its nodes (at most kTileEls); elements with every node in one
-  // tile are that tile's interior, the rest the boundary list
+  // elements cut into tiles, each with the nodes of its own elements (at
+  // most kTileNodes) and the elements around those nodes (its own, then the
+  // ring; at most kTileEls)
   int32_t tile_state = 0;         // 0 not built, 1 built, -1 the mesh has no tiling
   bool tiles_on = true;           // bl_set_tiling
-  int32_t ntile = 0, nbnd = 0, nentries = 0;
+  int32_t ntile = 0, nentries = 0;
   int32_t* d_tile_eptr = nullptr;  // ntile + 1
   int32_t* d_tile_nptr = nullptr;  // ntile + 1
-  int32_t* d_tile_nodes = nullptr; // owned nodes, 0-based
+  int32_t* d_tile_nodes = nullptr; // the tile's nodes, 0-based | (it writes the node) << 31
   int2* d_tile_el = nullptr;       // (element | flags << 30, four local node slots)
   int4* d_tile_nd = nullptr;       // the entries' four nodes, 0-based
-  int32_t* d_bnd_el = nullptr;     // boundary elements, 0-based
-  int32_t* d_el_bslot = nullptr;   // nel: an element's place in the boundary list, or -1
-  double* d_bnd_f = nullptr;       // 8 per boundary element: its corrector corner forces
   uint8_t* d_rest8 = nullptr;      // 2 x nel at-rest bytes of the tile step
   // the element state of the tile step in two sets (eldensity, elenergy,
   // elpressure, elcs2); el_cur names the one the registry points at

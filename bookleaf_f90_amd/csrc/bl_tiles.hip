@@ -1,13 +1,16 @@
-// bl_tiles.hip -- the node tiling of the tile step (host side).
+// bl_tiles.hip -- the element tiling of the tile step (host side).
 //
 // The tile step fuses the fused step's F2, F3 and F4 for a pure-cell
-// Lagrangian problem: one workgroup owns a tile of nodes, computes the
-// predictor and the corrector corner terms of every element around them
-// (F2's work, kept in LDS), sums them per owned node in the reference's
-// scatter order (F3's work), and then the corrector state of the elements
-// whose four nodes it owns (F4's work).  The elements with nodes in two or
-// more tiles (the boundary list) get their corrector state from a second
-// kernel.  The F2->F3->F4 corner terms then never reach HBM.
+// Lagrangian problem: one workgroup owns a tile of elements.  It computes
+// the predictor and the corrector corner terms (F2's work, kept in LDS) of
+// its own elements and of the ring of elements around their nodes, sums
+// them for every node of its own elements in the reference's scatter order
+// (F3's work), and then the corrector state of its own elements (F4's
+// work), whose nodes it now holds.  Every element is completed by the one
+// tile that owns it, so the F2->F3->F4 corner terms never reach HBM and no
+// second pass is needed.  A node that belongs to several tiles is computed
+// by each of them from the same inputs (the same bits) and written by one:
+// the tile owning the node's first incident element.
 //
 // The reference sums a node's corner terms in (corner jj, element in
 // ielsort1 order) order (hydro_kn_getacceleration.f90:53-72).  When no node
@@ -16,13 +19,10 @@
 // (node, jj).  A mesh where some node is corner jj of two elements keeps the
 // three-kernel step.
 //
-// Tiles come from recursive coordinate bisection of the node positions
+// Tiles come from recursive coordinate bisection of the element centroids
 // when the tiling is built (setup_rcb.f90 cuts the reference's partitions
-// the same way): each cut is along the longer side of the node set's
-// bounding box, at the share of the nodes that makes the tile counts of the
-// two sides proportional, down to about kTileTarget nodes; a tile whose
-// nodes or incident elements do not fit one workgroup is cut once more.
-// The positions only shape the tiles: any tiling gives the same results.
+// the same way; see Rcb below).  The positions only shape the tiles: any
+// tiling gives the same results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -110,16 +110,14 @@ int upload_vec(T** d, const std::vector<T>& v) {
 
 void free_tiles(Handle* h) {
   void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_nodes, h->d_tile_el, h->d_tile_nd,
-                  h->d_bnd_el, h->d_el_bslot, h->d_bnd_f, h->d_rest8};
+                  h->d_rest8};
   for (void* p : bufs)
     if (p) hipFree(p);
   h->d_tile_eptr = h->d_tile_nptr = h->d_tile_nodes = nullptr;
   h->d_tile_el = nullptr;
   h->d_tile_nd = nullptr;
-  h->d_bnd_el = h->d_el_bslot = nullptr;
-  h->d_bnd_f = nullptr;
   h->d_rest8 = nullptr;
-  h->ntile = h->nbnd = h->nentries = 0;
+  h->ntile = h->nentries = 0;
   h->tile_state = 0;
 }
 
@@ -171,7 +169,7 @@ int build_tiles(Handle* h) {
   // per tile: its nodes (ascending; the high bit marks the tile that writes
   // the node, the one holding the node's first incident element), then the
   // entries: its own elements ascending, then the ring around them
-  std::vector<int32_t> nptr(ntile + 1, 0), nodes, eptr(ntile + 1, 0), bnd, bslot(nel, -1);
+  std::vector<int32_t> nptr(ntile + 1, 0), nodes, eptr(ntile + 1, 0);
   std::vector<int2> ents;
   std::vector<int4> ent_nd;
   nodes.reserve((size_t)nnd + nnd / 4);
@@ -214,8 +212,8 @@ int build_tiles(Handle* h) {
           const int v = ielnd[4 * (size_t)e + j] - 1;
           slots |= (uint32_t)(nstamp[v] == t ? slot_of[v] : kNoSlot) << (8 * j);
         }
-        // an own element is this tile's to write and has every node here
-        const uint32_t flags = pass == 0 ? (kTileWriter | kTileInterior) : 0u;
+        // an own element: this tile completes and writes it
+        const uint32_t flags = pass == 0 ? kTileOwn : 0u;
         ents.push_back(make_int2((int)((uint32_t)e | (flags << 30)), (int)slots));
         const int32_t* q = &ielnd[4 * (size_t)e];
         ent_nd.push_back(make_int4(q[0] - 1, q[1] - 1, q[2] - 1, q[3] - 1));
@@ -223,27 +221,21 @@ int build_tiles(Handle* h) {
   }
   nptr[ntile] = (int)nodes.size();
   eptr[ntile] = (int)ents.size();
-  const int nbnd = (int)bnd.size();
   if (upload_vec(&h->d_tile_eptr, eptr) || upload_vec(&h->d_tile_nptr, nptr) ||
       upload_vec(&h->d_tile_nodes, nodes) || upload_vec(&h->d_tile_el, ents) ||
-      upload_vec(&h->d_tile_nd, ent_nd) ||
-      upload_vec(&h->d_bnd_el, bnd) || upload_vec(&h->d_el_bslot, bslot))
+      upload_vec(&h->d_tile_nd, ent_nd))
     return BL_FAILURE;
-  BL_HIP(hipMalloc(&h->d_bnd_f, sizeof(double) * 8 * std::max(nbnd, 1)));
   BL_HIP(hipMalloc(&h->d_rest8, 2 * (size_t)std::max(nel, 1)));
   BL_HIP(hipMemset(h->d_rest8, 0, 2 * (size_t)std::max(nel, 1)));
   h->ntile = ntile;
-  h->nbnd = nbnd;
   h->nentries = (int32_t)ents.size();
-  // the partial slots of the tile step: one per tile, then one per boundary
-  // block
-  const int nseg = ntile + (int)nblocks(nbnd);
-  if (nseg + 16 > h->npart) {
+  // the partial slots of the tile step: one per tile
+  if (ntile + 16 > h->npart) {
     hipFree(h->d_part);
     hipFree(h->d_parti);
     h->d_part = nullptr;
     h->d_parti = nullptr;
-    h->npart = nseg + 16;
+    h->npart = ntile + 16;
     BL_HIP(hipMalloc(&h->d_part, sizeof(double) * 4 * h->npart));
     BL_HIP(hipMalloc(&h->d_parti, sizeof(int32_t) * 4 * h->npart));
   }

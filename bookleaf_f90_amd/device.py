@@ -468,8 +468,8 @@ class BookLeaf(DriverState):
         self._check(self.lib.bl_set_tiling(self.h, int(bool(on))), "set_tiling")
 
     def get_tiling(self):
-        """(tile step in use, tiles, boundary elements, element entries)
-        (bl_get_tiling)."""
+        """(tile step in use, tiles, 0 (no boundary elements: every element
+        completes in its own tile), element entries) (bl_get_tiling)."""
         v = [C.c_int32(0) for _ in range(4)]
         self._check(self.lib.bl_get_tiling(self.h, *(C.byref(x) for x in v)), "get_tiling")
         return bool(v[0].value), v[1].value, v[2].value, v[3].value

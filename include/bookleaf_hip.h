@@ -491,20 +491,22 @@ int bl_get_tuning(bl_handle_t* h, bl_tuning_t* t);
  * fuse).  A Lagrangian step of a problem with pure cells only and no
  * per-step diagnostics can run its predictor, node update and corrector
  * (the work of hydro_dr_getforce, getstate, getenergy, geteos and the
- * acceleration drivers) as one kernel per tile of about 200 nodes, keeping
- * the corner forces and node results in LDS: tiles come from recursive
- * coordinate bisection of the node positions the first time bl_run_graph
- * needs them after the connectivity was built (bl_build_maps).  The mesh
+ * acceleration drivers) as one kernel per tile of about 180 elements and
+ * the ring of elements around their nodes, keeping the corner forces and
+ * node results in LDS: tiles come from recursive coordinate bisection of
+ * the element centroids the first time bl_run_graph needs them after the
+ * connectivity was built (bl_build_maps).  The mesh
  * must have no node that is the same corner (1..4) of two elements; a mesh
  * that does runs the three-kernel step.  on = 0 keeps the three-kernel step
  * for every problem.  No result depends on the choice: the sums over a
  * node's corners stay in the reference's order. */
 int bl_set_tiling(bl_handle_t* h, int32_t on);
 /* whether the next run uses the tile step (as far as known: the tiling is
- * built by the first run that needs it), the tiles, the boundary elements
- * (those with nodes in two or more tiles) and the tiles' element entries
- * (each element once per tile that owns one of its nodes); any output may
- * be NULL */
+ * built by the first run that needs it), the tiles, nboundary (always 0:
+ * every element is completed by the tile that owns it; kept for callers of
+ * the node tiling this replaced) and the tiles' element entries (each
+ * element once in its own tile and once in the ring of every other tile
+ * that holds one of its nodes); any output may be NULL */
 int bl_get_tiling(bl_handle_t* h, int32_t* on, int32_t* ntile, int32_t* nboundary,
                   int32_t* nentries);
 

@@ -51,8 +51,8 @@ def test_tiles_match_three_kernel_step_and_oracle(name, oracle_lib):
         state[tiles] = g.state()
         on, ntile, nbnd, nent = g.get_tiling()
         if tiles:
-            assert on and ntile >= 1 and nent >= p.nel and 0 <= nbnd < p.nel
-            print("tiles", ntile, "boundary", nbnd, "entries", nent, flush=True)
+            assert on and ntile >= 1 and nent >= p.nel and nbnd == 0
+            print("tiles", ntile, "entries", nent, flush=True)
         else:
             assert not on
     o = Oracle(p)
