@@ -166,6 +166,9 @@ struct EnsArgs {
   int32_t* lparti;
   double* hist;
   int hist_cap;
+  // with the tile step: instance k's tiles are toff[k] .. toff[k+1]-1, and
+  // their F1 partials the tile step's slots (null: the element blocks')
+  const int32_t* toff;
 };
 
 }  // namespace bl

@@ -580,20 +580,31 @@ __global__ void __launch_bounds__(kBlock) k_e1b_select(FusedArgs A, EnsArgs E, i
       ts->skip = skip ? 1 : 0;
       A.dtres->negvol = 0;
       A.dtres->negvol2 = 0;
-      if (!skip) ts->nstep = ts->nstep + 1;
+      if (!skip) {
+        ts->nstep = ts->nstep + 1;
+        ts->last = ts->nstep == ts->last_step ? 1 : 0;  // the tile step's last
+      }
     }
     if (!skip)
       atomicAdd(A.vstats, (unsigned long long)min(E.cnt[k], (uint32_t)(E.eoff[k + 1] - E.eoff[k])));
     E.cnt[k] = 0;  // E1L has read it
   }
   if (skip) return;
-  const int lb0 = E.eoff[k] / kBlock, lb1 = (E.eoff[k + 1] - 1) / kBlock;
   double va = -1.0, vc = DBL_MAX;
   int xa = 0x7fffffff, xc = 0x7fffffff;
-  for (int lb = lb0 + t; lb <= lb1; lb += kBlock) {
-    const int slot = (int)A.einst[lb * kBlock] == k ? 0 : 1;
-    better_max(va, xa, E.part[slot * nb + lb], E.parti[slot * nb + lb]);
-    better_min(vc, xc, E.part[(2 + slot) * nb + lb], E.parti[(2 + slot) * nb + lb]);
+  if (E.toff) {
+    // the tile step's slots: one per tile (nb tiles), the instance's own
+    for (int s = E.toff[k] + t; s < E.toff[k + 1]; s += kBlock) {
+      better_max(va, xa, A.part[s], A.parti[s]);
+      better_min(vc, xc, A.part[2 * nb + s], A.parti[2 * nb + s]);
+    }
+  } else {
+    const int lb0 = E.eoff[k] / kBlock, lb1 = (E.eoff[k + 1] - 1) / kBlock;
+    for (int lb = lb0 + t; lb <= lb1; lb += kBlock) {
+      const int slot = (int)A.einst[lb * kBlock] == k ? 0 : 1;
+      better_max(va, xa, E.part[slot * nb + lb], E.parti[slot * nb + lb]);
+      better_min(vc, xc, E.part[(2 + slot) * nb + lb], E.parti[(2 + slot) * nb + lb]);
+    }
   }
   for (int b = E.lboff[k] + t; b < E.lboff[k + 1]; b += kBlock)
     better_min(vc, xc, E.lpart[b], E.lparti[b]);
@@ -1077,6 +1088,21 @@ k_f4_state_ens(FusedArgs A, FusedNodes O, EnsArgs E) {
 // stops on a negative volume leaves the state it started from, as the
 // three-kernel step does (F4 does not run after F2's negative volume there).
 
+// An ensemble tile's elements in motion to its instance k's worklist
+// segment, one atomic per wave (as e1_collect lists them); every lane of
+// the wave takes part
+__device__ __forceinline__ void ens_list(const FusedArgs& A, const EnsArgs& E, int k, int lane,
+                                         bool moving, int e) {
+  const uint64_t mask = __ballot(moving);
+  if (!mask) return;
+  const int lead = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == lead) base = atomicAdd(E.cnt + k, (uint32_t)__popcll(mask));
+  base = __shfl(base, lead, 64);
+  const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (moving && slot < (uint32_t)(E.eoff[k + 1] - E.eoff[k])) A.work[E.eoff[k] + slot] = e;
+}
+
 // The partial slots of one workgroup of F1's terms: its DIV maximum and
 // CFL minimum to slots seg and 2 * nseg + seg.  Inactive lanes carry the
 // neutral values and an index that never wins a tie.
@@ -1091,9 +1117,9 @@ __device__ __forceinline__ void seg_collect(const FusedArgs& A, int seg, int nse
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
+k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const DevTimestep* ts = A.ts;
-  if (ts->skip) return;
+  if (ts->skip || ens_pending(A, ts)) return;
   // slot-fastest, so that a wave's lanes of one (quantity, corner) are
   // consecutive doubles: (quantity m, area, fx, fy; corner jj; node slot)
   __shared__ double acc[4][4][kTileNodes + 1];
@@ -1120,7 +1146,14 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
     for (int j = 0; j < 4; ++j) pres[j][tid] = 0;
   }
   __syncthreads();
-  const double dt = ts->dt;
+  // an ensemble tile lies in one instance: its dt
+  int inst = 0;
+  double dt = ts->dt, rdt = ts->rdt;
+  if (A.einst) {
+    inst = A.einst[A.tile_el[eb].x & 0x3fffffff];
+    dt = A.inst_dt[inst];
+    rdt = A.inst_rdt[inst];
+  }
   // --- F2: every element around the tile's nodes, one per thread
   const bool active = tid < ne;
   int e = 0;
@@ -1137,7 +1170,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
     const bool writer = (flags & kTileOwn) != 0;
     const bool at_rest = A.rest8[e] != 0;
     double wt[4], mt[4];
-    f2_element(A, N, e, nd, dt, ts->rdt, at_rest, writer, fx, fy, wt, mt);
+    f2_element(A, N, e, nd, dt, rdt, at_rest, writer, fx, fy, wt, mt);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t sl = (slots >> (8 * j)) & 0xFFu;
@@ -1191,6 +1224,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
   // --- F4 and the next step's F1 terms: the elements held whole
   const bool interior = active && (flags & kTileOwn);
   double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false;
   if (interior) {
     double x[4], y[4], u[4], v[4], ub[4], vb[4];
 #pragma unroll
@@ -1204,11 +1238,12 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
       vb[j] = nout[5][sl];
     }
     Iso g;
-    bool moving;
     const double c2 = f4_state_of(A, e, dt, x, y, ub, vb, fx, fy, g);
     f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
     A.rest8_next[e] = moving ? 0 : 1;
   }
+  // an ensemble's next-step limiter reads its instance's worklist
+  if (A.einst) ens_list(A, E, inst, tid & 63, interior && moving, e);
   seg_collect(A, t, A.nseg, interior, e, cflv, divv);
 }
 
@@ -1216,7 +1251,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O) {
 // one workgroup per tile over its own elements; the at-rest bytes of that
 // step go to rest8_next
 __global__ void __launch_bounds__(kBlock)
-k_f1_seg(FusedArgs A, FusedNodes N) {
+k_f1_seg(FusedArgs A, FusedNodes N, EnsArgs E) {
   if (step_skipped(A, A.ts)) return;
   const int tid = threadIdx.x;
   const int b = xcd_block();
@@ -1225,8 +1260,8 @@ k_f1_seg(FusedArgs A, FusedNodes N) {
   const bool active = tid < ne && (((uint32_t)ent.x >> 30) & kTileOwn);
   const int e = ent.x & 0x3fffffff, seg = b;
   double cflv = DBL_MAX, divv = -1.0;
+  bool moving = false;
   if (active) {
-    bool moving;
     int nd[4];
     load_nd(A.ielnd, e, nd);
     double x[4], y[4], u[4], v[4];
@@ -1236,6 +1271,9 @@ k_f1_seg(FusedArgs A, FusedNodes N) {
     f1_terms(A, e, x, y, u, v, g, A.cs2[e], moving, cflv, divv);
     A.rest8_next[e] = moving ? 0 : 1;
   }
+  // an ensemble tile: its instance's worklist for the limiter pass
+  if (A.einst)
+    ens_list(A, E, A.einst[A.tile_el[eb].x & 0x3fffffff], tid & 63, active && moving, e);
   seg_collect(A, seg, A.nseg, active, e, cflv, divv);
 }
 
@@ -1695,6 +1733,12 @@ bool use_tiles(const bl_handle_t* h) {
   return end_in_f1b(h) && h->tiles_on && h->tile_state == 1;
 }
 
+// an ensemble's step on the tiling (tiles within instances): F5 still ends
+// each step, and E1's terms go to the tiles' slots and the worklists
+bool ens_tiles(const bl_handle_t* h) {
+  return h->ens_n > 0 && h->tiles_on && h->tile_state == 1;
+}
+
 // the tile step's arguments for a step from node and element set p: it
 // reads element set p and writes set 1 - p
 void tile_args(bl_handle_t* h, FusedArgs& A, int p) {
@@ -1732,7 +1776,7 @@ int launch_f1_prologue(bl_handle_t* h, int p, int stop_at_time_end) {
     tile_args(h, A, p);
     A.rest8_next = h->d_rest8 + (size_t)p * h->sz.nel;  // the first step's own bytes
     FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
-    BL_LAUNCH(h, "f1_seg", k_f1_seg, (unsigned)h->ntile, kBlock, A, N);
+    BL_LAUNCH(h, "f1_seg", k_f1_seg, (unsigned)h->ntile, kBlock, A, N, EnsArgs{});
     return BL_SUCCESS;
   }
   uint64_t* m[2];
@@ -1761,7 +1805,7 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
     const unsigned nbl = std::min(nseg, (unsigned)h->tune_lim_grid);
     BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nseg);
     BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kF1bThreads, A, (int)nseg, (int)nbl);
-    BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O);
+    BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O, EnsArgs{});
     return BL_SUCCESS;
   }
   const unsigned nbl = std::min(nbe, (unsigned)h->tune_lim_grid);
@@ -1919,7 +1963,7 @@ void point_elements_at(bl_handle_t* h, int s) {
 // (a batch of another step kind moved the node sets but kept the elements
 // in place).  Outside any capture.
 int prepare_tiles(bl_handle_t* h) {
-  if (!h->tiles_on || !end_in_f1b(h)) return BL_SUCCESS;
+  if (!h->tiles_on || !(end_in_f1b(h) || h->ens_n > 0)) return BL_SUCCESS;
   if (h->tile_state == 0) {
     if (build_tiles(h)) return BL_FAILURE;
     invalidate_graph(h);
@@ -2112,6 +2156,7 @@ EnsArgs make_ens(bl_handle_t* h) {
   E.lparti = h->d_eparti + 4 * nb;
   E.hist = h->d_ehist;
   E.hist_cap = (int)h->ehist_cap;
+  E.toff = ens_tiles(h) ? h->d_ens_toff : nullptr;
   return E;
 }
 
@@ -2131,6 +2176,18 @@ int launch_ensemble_step(bl_handle_t* h, int p) {
   FusedNodesOut O{h->nodeset[1 - p][0], h->nodeset[1 - p][1], h->nodeset[1 - p][2],
                   h->nodeset[1 - p][3]};
   FusedNodes Oc{O.x, O.y, O.u, O.v};
+  if (ens_tiles(h)) {
+    // E1 ran in the previous step's tile step (or in the run call's
+    // prologue); the tile step reads element set p and writes 1 - p
+    tile_args(h, A, p);
+    A.end_in_f1b = 0;  // F5 ends each step
+    BL_LAUNCH(h, "e1_limiter", k_e1_limiter, (unsigned)h->ens_lblocks,
+              (unsigned)h->tune_lim_block, A, N, E);
+    BL_LAUNCH(h, "e1b_select", k_e1b_select, (unsigned)h->ens_n, kBlock, A, E, (int)h->ntile);
+    BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O, E);
+    BL_LAUNCH(h, "f5_end", k_f5_end, 1, kBlock, A, Oc);
+    return BL_SUCCESS;
+  }
   // E1 ran in the previous step's F4 (or in the run call's prologue)
   uint64_t* m[2];
   rest_masks(h, m);
@@ -2189,7 +2246,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   }
   fill_err(err, BL_SUCCESS, 0, "");
   if (nsteps <= 0) { if (steps_taken) *steps_taken = 0; return BL_SUCCESS; }
-  if (ensure_nodesets(h) || clear_rest(h)) return BL_FAILURE;
+  if (ensure_nodesets(h) || prepare_tiles(h) || clear_rest(h)) return BL_FAILURE;
   if (h->ehist_cap < nsteps) {
     // as bl_run_graph: room for runs of up to 1024 steps without re-capture
     const int cap = std::max(nsteps, 1024);
@@ -2208,6 +2265,7 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
   d->done = 0; d->skip = 0; d->err = 0; d->err_step = 0; d->err_idx = 0; d->flips = 0;
   d->hist_base = start;
   d->epend = kNoPending;
+  d->last_step = start + nsteps;
   BL_HIP(hipMemcpyAsync(h->d_ts, d, sizeof(DevTimestep), hipMemcpyHostToDevice, h->stream));
   int p = h->cur_set;
   {
@@ -2225,7 +2283,13 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
     A.rest = m[p];
     const EnsArgs E = make_ens(h);
     FusedNodes N{h->nodeset[p][0], h->nodeset[p][1], h->nodeset[p][2], h->nodeset[p][3]};
-    BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nblocks(h->sz.nel), kBlock, A, N, E);
+    if (ens_tiles(h)) {
+      tile_args(h, A, p);
+      A.rest8_next = h->d_rest8 + (size_t)p * h->sz.nel;  // the first step's own bytes
+      BL_LAUNCH(h, "f1_seg", k_f1_seg, (unsigned)h->ntile, kBlock, A, N, E);
+    } else {
+      BL_LAUNCH(h, "e1_getdt", k_e1_getdt, nblocks(h->sz.nel), kBlock, A, N, E);
+    }
   }
   for (int k = 0; k < nsteps; ++k) {
     if (h->timing) {
@@ -2244,6 +2308,9 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
     d->err_idx = d->epend >> 2;
   }
   point_fields_at(h, (h->cur_set + d->flips) & 1);
+  // (on the tiling the last step wrote rscratch23/24 itself)
+  const bool tiles = ens_tiles(h);
+  if (tiles) point_elements_at(h, h->cur_set);
   {
     int nel = h->sz.nel;
     BL_LAUNCH(h, "materialize", k_materialize, nblocks(nel), kBlock, nel, h->I(BL_IELND),
@@ -2251,8 +2318,9 @@ int bl_ensemble_run(bl_handle_t* h, int32_t nsteps, double* dt_hist, int32_t* id
               (const DevTimestep*)nullptr, h->cur_set, h->R(BL_CNX), h->R(BL_CNY),
               h->R(BL_A1),
               h->R(BL_A2), h->R(BL_A3), h->R(BL_B1), h->R(BL_B2), h->R(BL_B3), h->R(BL_CNWT),
-              h->R(BL_ELVOLUME), (const double*)h->cterm[0], (const double*)h->cterm[1],
-              h->R(BL_RSCRATCH23), h->R(BL_RSCRATCH24));
+              h->R(BL_ELVOLUME), tiles ? nullptr : (const double*)h->cterm[0],
+              tiles ? nullptr : (const double*)h->cterm[1], h->R(BL_RSCRATCH23),
+              h->R(BL_RSCRATCH24));
   }
   if (taken > 0 && (dt_hist || idt_hist || ctl_hist)) {
     const size_t n = (size_t)taken * h->ens_n;
@@ -2517,7 +2585,7 @@ int bl_get_tiling(bl_handle_t* h, int32_t* on, int32_t* ntile, int32_t* nboundar
     return BL_FAILURE;
   }
   const bool built = h->tile_state == 1;
-  if (on) *on = (h->tiles_on && built && end_in_f1b(h)) ? 1 : 0;
+  if (on) *on = (h->tiles_on && built && (end_in_f1b(h) || h->ens_n > 0)) ? 1 : 0;
   if (ntile) *ntile = built ? h->ntile : 0;
   if (nboundary) *nboundary = 0;  // every element completes in its own tile
   if (nentries) *nentries = built ? h->nentries : 0;

@@ -198,6 +198,7 @@ struct Handle {
   int32_t* d_tile_nodes = nullptr; // the tile's nodes, 0-based | (it writes the node) << 31
   int2* d_tile_el = nullptr;       // (element | flags << 30, four local node slots)
   int4* d_tile_nd = nullptr;       // the entries' four nodes, 0-based
+  int32_t* d_ens_toff = nullptr;   // ensembles: ens_n + 1 tile offsets per instance
   uint8_t* d_rest8 = nullptr;      // 2 x nel at-rest bytes of the tile step
   // the element state of the tile step in two sets (eldensity, elenergy,
   // elpressure, elcs2); el_cur names the one the registry points at

@@ -110,12 +110,13 @@ int upload_vec(T** d, const std::vector<T>& v) {
 
 void free_tiles(Handle* h) {
   void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_nodes, h->d_tile_el, h->d_tile_nd,
-                  h->d_rest8};
+                  h->d_rest8, h->d_ens_toff};
   for (void* p : bufs)
     if (p) hipFree(p);
   h->d_tile_eptr = h->d_tile_nptr = h->d_tile_nodes = nullptr;
   h->d_tile_el = nullptr;
   h->d_tile_nd = nullptr;
+  h->d_ens_toff = nullptr;
   h->d_rest8 = nullptr;
   h->ntile = h->nentries = 0;
   h->tile_state = 0;
@@ -160,7 +161,20 @@ int build_tiles(Handle* h) {
         std::vector<int32_t>(nel, 0), {}, 0};
   r.idx.resize(nel);
   for (int e = 0; e < nel; ++e) r.idx[e] = e;
-  r.cut(0, nel, std::max(1, (nel + kTileTarget - 1) / kTileTarget));
+  // an ensemble's instances own contiguous element ranges: each is cut on
+  // its own, so that a tile lies in one instance (one dt)
+  std::vector<int32_t> toff;
+  if (h->ens_n > 0) {
+    toff.assign(h->ens_n + 1, 0);
+    for (int k = 0; k < h->ens_n; ++k) {
+      const int lo = h->ens_eoff[k], hi = h->ens_eoff[k + 1];
+      toff[k] = (int)r.leaves.size();
+      r.cut(lo, hi, std::max(1, (hi - lo + kTileTarget - 1) / kTileTarget));
+    }
+    toff[h->ens_n] = (int)r.leaves.size();
+  } else {
+    r.cut(0, nel, std::max(1, (nel + kTileTarget - 1) / kTileTarget));
+  }
 
   const int ntile = (int)r.leaves.size();
   std::vector<int32_t> tile_of(nel);
@@ -223,7 +237,8 @@ int build_tiles(Handle* h) {
   eptr[ntile] = (int)ents.size();
   if (upload_vec(&h->d_tile_eptr, eptr) || upload_vec(&h->d_tile_nptr, nptr) ||
       upload_vec(&h->d_tile_nodes, nodes) || upload_vec(&h->d_tile_el, ents) ||
-      upload_vec(&h->d_tile_nd, ent_nd))
+      upload_vec(&h->d_tile_nd, ent_nd) ||
+      (!toff.empty() && upload_vec(&h->d_ens_toff, toff)))
     return BL_FAILURE;
   BL_HIP(hipMalloc(&h->d_rest8, 2 * (size_t)std::max(nel, 1)));
   BL_HIP(hipMemset(h->d_rest8, 0, 2 * (size_t)std::max(nel, 1)));

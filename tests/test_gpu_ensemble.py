@@ -19,7 +19,7 @@ INIT_FIELDS = ("ielnd", "ielel", "ielfc", "indtype", "ielreg", "ielmat", "ielsor
                "b2", "b3")
 STATE_FIELDS = ("eldensity", "elenergy", "elpressure", "elcs2", "elvolume", "elmass", "elvisc",
                 "cnwt", "cnmass", "cnviscx", "cnviscy", "cnx", "cny", "a1", "a3", "b1", "b3",
-                "ndx", "ndy", "ndu", "ndv")
+                "ndx", "ndy", "ndu", "ndv", "rscratch23", "rscratch24")
 
 
 # a larger first dt than the ensemble default, so that the small instances'
@@ -45,15 +45,20 @@ def test_generated_ensemble_matches_host_setup():
             assert np.array_equal(mine, ref), (k, f)
 
 
-@pytest.mark.parametrize("seed", [11, 12])
-def test_ensemble_run_matches_each_instance_alone(seed, oracle_lib):
+@pytest.mark.parametrize("seed,tiles", [(11, True), (12, True), (11, False)])
+def test_ensemble_run_matches_each_instance_alone(seed, tiles, oracle_lib):
+    """The tile step (tiles within instances, each tile's dt its
+    instance's) and the three-kernel step (bl_set_tiling off)."""
     from bookleaf_f90_amd.device import Ensemble
     from oracle.pyoracle import Oracle
     params = _small(seed=seed, n=6)
-    print("ensemble run", [(q["kind"], q["nx"]) for q in params], flush=True)
+    print("ensemble run", tiles, [(q["kind"], q["nx"]) for q in params], flush=True)
     ens = Ensemble(params, config=CFG)
+    ens.set_tiling(tiles)
     h1 = ens.run(15)
     h2 = ens.run(25)
+    on, ntile, _, nent = ens.get_tiling()
+    assert on == tiles and (ntile >= len(params) if tiles else True)
     dt, idt, ctl = (np.concatenate([a, b]) for a, b in zip(h1, h2))
     times, last_dt, nstep = ens.times()
     state = {f: ens.get(f) for f in STATE_FIELDS}
@@ -115,7 +120,9 @@ def test_ensemble_dt_min_failure_is_deterministic(fail_step, oracle_lib):
     that step's (two instances at step 3, one at step 8).  Two runs give the
     same bits, and each instance holds the oracle's state after
     fail_step - 1 steps of it alone (the viscosity fields aside: the failing
-    step's limiter pass has written those of its moving elements)."""
+    step's limiter pass has written those of its moving elements; and the
+    corrector corner-force scratch rscratch23/24, which on the tile step only
+    a batch's planned last step writes)."""
     from bookleaf_f90_amd.device import BookLeafError, Ensemble
     from oracle.pyoracle import Oracle
     params = _small(seed=11, n=6)
@@ -125,7 +132,8 @@ def test_ensemble_dt_min_failure_is_deterministic(fail_step, oracle_lib):
     failing = [k for k in range(len(params)) if D[k, s] < dt_min]
     assert failing, "no instance falls below dt_min at this step"
     cfg = dict(CFG, dt_min=dt_min)
-    fields = [f for f in STATE_FIELDS if f not in ("cnviscx", "cnviscy", "elvisc")]
+    fields = [f for f in STATE_FIELDS
+              if f not in ("cnviscx", "cnviscy", "elvisc", "rscratch23", "rscratch24")]
     runs = []
     for rep in range(2):
         print("ensemble dt_min failure at step", fail_step, "rep", rep, flush=True)
