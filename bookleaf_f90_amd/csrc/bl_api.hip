@@ -623,7 +623,8 @@ int bl_create(const bl_sizes_t* sizes, bl_handle_t** out) {
       hipMalloc(&h->d_dtres, sizeof(DtResult)) != hipSuccess ||
       hipHostMalloc(&h->h_dtres, sizeof(DtResult)) != hipSuccess ||
       hipMalloc(&h->d_ts, sizeof(DevTimestep)) != hipSuccess ||
-      hipHostMalloc(&h->h_ts, sizeof(DevTimestep)) != hipSuccess) {
+      hipHostMalloc(&h->h_ts, sizeof(DevTimestep)) != hipSuccess ||
+      hipHostMalloc(&h->h_pub_ts, sizeof(DevTimestep), hipHostMallocCoherent) != hipSuccess) {
     set_error("bl_create: out of device memory"); bl_destroy(h); return BL_FAILURE;
   }
   // the allocation-free timestep's workspace address on this device (a
@@ -683,6 +684,8 @@ int bl_destroy(bl_handle_t* h) {
   for (double* p : h->relax_tmp) if (p) hipFree(p);
   if (h->h_dtres) hipHostFree(h->h_dtres);
   if (h->h_ts) hipHostFree(h->h_ts);
+  if (h->h_pub_ts) hipHostFree(h->h_pub_ts);
+  if (h->h_pub_hist) hipHostFree(h->h_pub_hist);
   for (hipEvent_t ev : h->ev_pool) hipEventDestroy(ev);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;

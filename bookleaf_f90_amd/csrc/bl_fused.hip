@@ -46,6 +46,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "bl_fused.h"
@@ -2006,9 +2007,13 @@ int build_nbfc(bl_handle_t* h) {
 // microseconds on the device between them, kernels inside one graph none,
 // so a run call launches the fewest graphs its steps need (8, 4, 2 or 1
 // steps each, largest first; 8 steps took the Sedov step from 0.1524 to
-// 0.1486 ms), the last one with the batch's end in it.
+// 0.1486 ms), the last one with the batch's end in it.  Graphs of 16 and
+// 32 steps made a 20-step call slower (0.127 against 0.126 ms a step).
 constexpr int kGraphLogs = 4;
-inline int graph_index(int fin, int si, int p) { return 8 * fin + 2 * si + p; }
+constexpr int kGraphs = 4 * kGraphLogs;
+static_assert(std::extent<decltype(bl::Handle::graph_exec)>::value >= (size_t)kGraphs,
+              "the handle's graph table holds every step graph");
+inline int graph_index(int fin, int si, int p) { return 2 * kGraphLogs * fin + 2 * si + p; }
 
 int launch_batch_end(bl_handle_t* h, const DevTimestep* d_ts, int cur);
 
@@ -2109,9 +2114,21 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
   vol[e] = g.vol;
 }
 
+// The call's results for the host, in pinned coherent host memory, so that
+// no copy follows the batch: the timestep and the dt history (3 values per
+// step that selected a dt)
+__global__ void k_publish(const DevTimestep* __restrict__ ts, const double* __restrict__ hist,
+                          DevTimestep* __restrict__ out_ts, double* __restrict__ out_hist,
+                          int cap) {
+  const int n = min(ts->hist_n, cap);
+  for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) out_hist[i] = hist[i];
+  if (threadIdx.x == 0) *out_ts = *ts;
+}
+
 // The batch's end: the open end of its last step, then the registry's
 // geometry at the final node set (from ts after the steps, or set cur with
-// ts null); the tile step's last step wrote rscratch23/24 itself
+// ts null; the tile step's last step wrote rscratch23/24 itself), then the
+// results for the host
 int launch_batch_end(bl_handle_t* h, const DevTimestep* d_ts, int cur) {
   if (end_in_f1b(h))
     BL_LAUNCH(h, "batch_end", k_batch_end, 1, 64, h->d_ts, (const DtResult*)h->d_dtres);
@@ -2124,6 +2141,8 @@ int launch_batch_end(bl_handle_t* h, const DevTimestep* d_ts, int cur) {
             tiles ? nullptr : (const double*)h->cterm[0],
             tiles ? nullptr : (const double*)h->cterm[1], h->R(BL_RSCRATCH23),
             h->R(BL_RSCRATCH24));
+  BL_LAUNCH(h, "publish", k_publish, 1, kBlock, (const DevTimestep*)h->d_ts,
+            (const double*)h->d_hist, h->h_pub_ts, h->h_pub_hist, (int)h->hist_cap);
   return BL_SUCCESS;
 }
 
@@ -2635,13 +2654,16 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
                   err);
   fill_err(err, BL_SUCCESS, 0, "");
   if (ensure_nodesets(h) || prepare_tiles(h)) return BL_FAILURE;
-  if (h->hist_cap < nsteps) {
+  if (h->hist_cap < nsteps || !h->h_pub_hist) {
     // the graphs bake in the history's capacity: size it generously so that
     // runs of different lengths do not re-capture
-    const int cap = std::max(nsteps, kMinHistory);
+    const int cap = std::max(std::max(nsteps, (int)h->hist_cap), kMinHistory);
     if (h->d_hist) hipFree(h->d_hist);
+    if (h->h_pub_hist) hipHostFree(h->h_pub_hist);
     h->d_hist = nullptr;
+    h->h_pub_hist = nullptr;
     BL_HIP(hipMalloc(&h->d_hist, sizeof(double) * 3 * cap));
+    BL_HIP(hipHostMalloc(&h->h_pub_hist, sizeof(double) * 3 * cap, hipHostMallocCoherent));
     h->hist_cap = cap;
     invalidate_graph(h);
   }
@@ -2657,8 +2679,9 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   const std::vector<int> plan = graph_plan(h->timing ? 0 : nsteps, h->cur_set);
   bool missing = false;
   for (int q : plan) missing = missing || !h->graph_exec[q];
-  for (int q = 0; missing && q < 16; ++q)
-    if (!h->graph_exec[q] && capture_graph(h, q >> 3, (q >> 1) & 3, q & 1, stop_at_time_end))
+  for (int q = 0; missing && q < kGraphs; ++q)
+    if (!h->graph_exec[q] && capture_graph(h, q / (2 * kGraphLogs), (q / 2) % kGraphLogs, q & 1,
+                                           stop_at_time_end))
       return BL_FAILURE;
   DevTimestep* d = h->h_ts;
   std::memset(d, 0, sizeof(DevTimestep));
@@ -2686,12 +2709,10 @@ int bl_run_graph(bl_handle_t* h, int32_t nsteps, int32_t stop_at_time_end, bl_ti
   // state
   if (plan.empty() && launch_batch_end(h, h->d_ts, 0)) return BL_FAILURE;
   if (!h->cfg.zexist && mix_cpu_overlay(h, nullptr)) return BL_FAILURE;
-  std::vector<double> hist(3 * (size_t)std::max(nsteps, 1));
-  BL_HIP(hipMemcpyAsync(d, h->d_ts, sizeof(DevTimestep), hipMemcpyDeviceToHost, h->stream));
-  if (nsteps > 0)
-    BL_HIP(hipMemcpyAsync(hist.data(), h->d_hist, sizeof(double) * 3 * (size_t)nsteps,
-                          hipMemcpyDeviceToHost, h->stream));
+  // the batch's end published the timestep and the history (k_publish)
   BL_HIP(hipStreamSynchronize(h->stream));
+  *d = *h->h_pub_ts;
+  const double* hist = h->h_pub_hist;
   // steps that selected a dt (a failing step included)
   const int taken = d->hist_n;
   for (int k = 0; k < taken; ++k) {

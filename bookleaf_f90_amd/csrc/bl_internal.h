@@ -124,6 +124,10 @@ struct Handle {
   DevTimestep* d_ts = nullptr;
   DevTimestep* h_ts = nullptr;  // pinned
   double* d_hist = nullptr;     // dt history (3 values per step)
+  // what a run call's end publishes for the host (pinned, coherent host
+  // memory written by k_publish): the timestep and hist_cap history records
+  DevTimestep* h_pub_ts = nullptr;
+  double* h_pub_hist = nullptr;
   int64_t hist_cap = 0;
   double* d_tmp[8] = {};        // fused-step temporaries (nel or nnd sized)
   // node ping-pong of the fused step: set[0] is the registry's own ndx..ndv,
@@ -143,8 +147,8 @@ struct Handle {
   double* ale_tmp[11] = {};
   // step graphs, each captured when a run call first needs it: [8 * fin +
   // 2 * si + p] runs 2^si steps (si = 0..3) from node set p, then with fin
-  // the batch's end (k_batch_end, k_materialize).  Ensembles use [p], one
-  // step from node set p.
+  // the batch's end (k_batch_end, k_materialize, k_publish).  Ensembles use
+  // [p], one step from node set p.
   hipGraphExec_t graph_exec[16] = {};
   // ensembles (bl_ensemble_create): instances stepped together
   int32_t ens_n = 0;
