@@ -144,7 +144,14 @@ int build_tiles(Handle* h) {
       seen[4 * (size_t)v + j] = 1;
       ++adj_ptr[v + 1];
     }
-  for (int v = 0; v < nnd; ++v) adj_ptr[v + 1] += adj_ptr[v];
+  for (int v = 0; v < nnd; ++v) {
+    // a node no element uses would be written by no tile
+    if (adj_ptr[v + 1] == 0) {
+      h->tile_state = -1;
+      return BL_SUCCESS;
+    }
+    adj_ptr[v + 1] += adj_ptr[v];
+  }
   // (an element with a repeated node is listed once per corner: the counts
   // below take each element once)
   std::vector<int32_t> adj(adj_ptr[nnd]), fill(adj_ptr.begin(), adj_ptr.end() - 1);
