@@ -350,21 +350,21 @@ namespace bl {
 
 // the per-step record of the fused step from the node set the step wrote
 int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* ndu,
-                const double* ndv) {
+                const double* ndv, const double* rho, const double* en, const double* pre) {
   if (!h->diag_on) return BL_SUCCESS;
+  if (!rho) rho = h->R(BL_ELDENSITY);
+  if (!en) en = h->R(BL_ELENERGY);
+  if (!pre) pre = h->R(BL_ELPRESSURE);
   const int nel = h->sz.nel, nreg = h->nreg;
   const unsigned nb = nblocks(nel);
   double* part = h->d_diag_part;  // sized by bl_set_diagnostics
   double* part2 = part + (size_t)nb * nreg * kNq;
   BL_LAUNCH(h, "diag_shortprint", k_sp_elements<true>, nb, kBlock, nel, nreg, h->cfg.dencut,
-            (const int32_t*)h->I(BL_IELREG), (const int32_t*)h->I(BL_IELND),
-            (const double*)h->R(BL_ELDENSITY), (const double*)h->R(BL_ELENERGY),
-            (const double*)h->R(BL_ELPRESSURE), (const double*)nullptr,
-            (const double*)h->R(BL_ELMASS), (const double*)nullptr, ndx, ndy, ndu, ndv, part,
-            (const DevTimestep*)h->d_ts);
+            (const int32_t*)h->I(BL_IELREG), (const int32_t*)h->I(BL_IELND), rho, en, pre,
+            (const double*)nullptr, (const double*)h->R(BL_ELMASS), (const double*)nullptr, ndx,
+            ndy, ndu, ndv, part, (const DevTimestep*)h->d_ts);
   BL_LAUNCH(h, "diag_l1", k_l1<true>, nb, kBlock, nel, (const int32_t*)h->I(BL_IELND), ndx, ndy,
-            (const double*)nullptr, (const double*)h->R(BL_ELDENSITY),
-            (const double*)h->R(BL_ELENERGY), h->diag_x0, h->diag_scale,
+            (const double*)nullptr, rho, en, h->diag_x0, h->diag_scale,
             (const double*)&h->d_ts->time, 0.0, part2, (const DevTimestep*)h->d_ts);
   double* col = part2 + (size_t)nb * kNq;
   BL_LAUNCH(h, "diag_columns", k_diag_columns, nreg * kNq + 3, kBlock, (int)nb, nreg,

@@ -1728,10 +1728,11 @@ bool end_in_f1b(const bl_handle_t* h) {
   return fold_f1(h) && !h->diag_on;
 }
 
-// the tile step runs a folded Lagrangian step with no F5 when the mesh has
-// a tiling (bl_tiles.hip) and tiling is on
+// the tile step runs a folded Lagrangian step when the mesh has a tiling
+// (bl_tiles.hip) and tiling is on; with per-step diagnostics F5 ends each
+// step and the records follow it
 bool use_tiles(const bl_handle_t* h) {
-  return end_in_f1b(h) && h->tiles_on && h->tile_state == 1;
+  return fold_f1(h) && h->tiles_on && h->tile_state == 1;
 }
 
 // an ensemble's step on the tiling (tiles within instances): F5 still ends
@@ -1802,12 +1803,17 @@ int launch_step(bl_handle_t* h, int p, int stop_at_time_end) {
   FusedNodes Oc{O.x, O.y, O.u, O.v};
   if (use_tiles(h)) {
     tile_args(h, A, p);
+    A.end_in_f1b = end_in_f1b(h) ? 1 : 0;  // with per-step diagnostics F5 ends the step
     const unsigned nseg = (unsigned)A.nseg;
     const unsigned nbl = std::min(nseg, (unsigned)h->tune_lim_grid);
     BL_LAUNCH(h, "f1_limiter", k_f1_limiter, nbl, (unsigned)h->tune_lim_block, A, N, (int)nseg);
     BL_LAUNCH(h, "f1b_select", k_f1b_select, 1, kF1bThreads, A, (int)nseg, (int)nbl);
     BL_LAUNCH(h, "lag_tile", k_lag_tile, (unsigned)h->ntile, kBlock, A, N, O, EnsArgs{});
-    return BL_SUCCESS;
+    if (end_in_f1b(h)) return BL_SUCCESS;
+    // per-step diagnostics: F5 records the step's end, then the record of
+    // the node and element sets the step wrote
+    BL_LAUNCH(h, "f5_end", k_f5_end, 1, kBlock, A, Oc);
+    return launch_diag(h, O.x, O.y, O.u, O.v, A.rho_o, A.en_o, A.pre_o);
   }
   const unsigned nbl = std::min(nbe, (unsigned)h->tune_lim_grid);
   A.end_in_f1b = end_in_f1b(h) ? 1 : 0;
@@ -1964,7 +1970,7 @@ void point_elements_at(bl_handle_t* h, int s) {
 // (a batch of another step kind moved the node sets but kept the elements
 // in place).  Outside any capture.
 int prepare_tiles(bl_handle_t* h) {
-  if (!h->tiles_on || !(end_in_f1b(h) || h->ens_n > 0)) return BL_SUCCESS;
+  if (!h->tiles_on || !(fold_f1(h) || h->ens_n > 0)) return BL_SUCCESS;
   if (h->tile_state == 0) {
     if (build_tiles(h)) return BL_FAILURE;
     invalidate_graph(h);
@@ -2604,7 +2610,7 @@ int bl_get_tiling(bl_handle_t* h, int32_t* on, int32_t* ntile, int32_t* nboundar
     return BL_FAILURE;
   }
   const bool built = h->tile_state == 1;
-  if (on) *on = (h->tiles_on && built && (end_in_f1b(h) || h->ens_n > 0)) ? 1 : 0;
+  if (on) *on = (h->tiles_on && built && (fold_f1(h) || h->ens_n > 0)) ? 1 : 0;
   if (ntile) *ntile = built ? h->ntile : 0;
   if (nboundary) *nboundary = 0;  // every element completes in its own tile
   if (nentries) *nentries = built ? h->nentries : 0;

@@ -263,9 +263,11 @@ int collect_pairs(Handle* h, const double* x, const double* y, const double* wu,
 int mesh_motion(Handle* h, const double* x, const double* y, double* wu, double* wv, double dt,
                 const DevTimestep* ts);
 int relax_rebuild(Handle* h);
-// the per-step diagnostics record from the node set a step wrote
+// the per-step diagnostics record from the node set a step wrote (and the
+// element state it wrote: the registry's unless given)
 int launch_diag(Handle* h, const double* ndx, const double* ndy, const double* ndu,
-                const double* ndv);
+                const double* ndv, const double* rho = nullptr, const double* en = nullptr,
+                const double* pre = nullptr);
 // ensembles: the limiter pass's workgroups per instance for a chunk of
 // elements each (bl_ensemble.hip)
 int ens_set_chunk(Handle* h, int chunk);
