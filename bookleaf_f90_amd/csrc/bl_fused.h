@@ -128,6 +128,7 @@ struct FusedArgs {
   const int32_t* tile_eptr;
   const int32_t* tile_nptr;
   const int32_t* tile_nodes;
+  const uint8_t* tile_ncm;  // per tile node: bit jj, the node is corner jj of an element
   const int2* tile_el;
   const int4* tile_nd;  // the entry's four nodes (0-based), beside tile_el
   const uint8_t* rest8;

@@ -1125,28 +1125,27 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   // consecutive doubles: (quantity m, area, fx, fy; corner jj; node slot)
   __shared__ double acc[4][4][kTileNodes + 1];
   __shared__ double nout[6][kTileNodes + 1];  // x, y, u, v at n+1, ubar, vbar
-  __shared__ uint8_t pres[4][kTileNodes + 1];
   const int t = xcd_block();
   const int tid = threadIdx.x;
   const int eb = A.tile_eptr[t], ne = A.tile_eptr[t + 1] - eb;
   const int nb = A.tile_nptr[t], nn = A.tile_nptr[t + 1] - nb;
-  // the tile's node's start-of-step values, loaded now for F3 below; the
-  // high bit of its entry: this tile writes it (other tiles holding it
-  // compute the same values for their own elements)
-  const bool own = tid < nn;
-  int n = 0, ntype = 0;
-  bool writes = false;
-  double x0 = 0.0, y0 = 0.0, u0 = 0.0, v0 = 0.0;
-  if (own) {
-    const uint32_t nv = (uint32_t)A.tile_nodes[nb + tid];
-    n = (int)(nv & 0x7fffffffu);
-    writes = (nv >> 31) != 0;
-    x0 = N.x[n]; y0 = N.y[n]; u0 = N.u[n]; v0 = N.v[n];
-    ntype = A.indtype[n];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) pres[j][tid] = 0;
-  }
-  __syncthreads();
+  // every thread loads a node entry and an element entry (a tile has at
+  // least one of each; threads past the end reload the last), so that these
+  // loads, and the node values after them, are in flight together; no
+  // barrier comes before phase A
+  const bool own = tid < nn, active = tid < ne;
+  const uint32_t nv = (uint32_t)A.tile_nodes[nb + min(tid, nn - 1)];
+  // the node's corner mask: the slots of F2 that phase B sums
+  const uint32_t cm = A.tile_ncm[nb + min(tid, nn - 1)];
+  const int2 ent = A.tile_el[eb + min(tid, ne - 1)];
+  const int4 q = A.tile_nd[eb + min(tid, ne - 1)];
+  // the tile's node's start-of-step values, for F3 below; the high bit of
+  // its entry: this tile writes it (other tiles holding it compute the
+  // same values for their own elements)
+  const int n = (int)(nv & 0x7fffffffu);
+  const bool writes = own && (nv >> 31) != 0;
+  const double x0 = N.x[n], y0 = N.y[n], u0 = N.u[n], v0 = N.v[n];
+  const int ntype = A.indtype[n];
   // an ensemble tile lies in one instance: its dt
   int inst = 0;
   double dt = ts->dt, rdt = ts->rdt;
@@ -1156,14 +1155,11 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
     rdt = A.inst_rdt[inst];
   }
   // --- F2: every element around the tile's nodes, one per thread
-  const bool active = tid < ne;
   int e = 0;
   uint32_t flags = 0, slots = 0xFFFFFFFFu;
   double fx[4], fy[4];
   bool zeroed = false;
   if (active) {
-    const int2 ent = A.tile_el[eb + tid];
-    const int4 q = A.tile_nd[eb + tid];
     const int nd[4] = {q.x, q.y, q.z, q.w};
     e = ent.x & 0x3fffffff;
     flags = (uint32_t)ent.x >> 30;
@@ -1180,7 +1176,6 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
         acc[1][j][sl] = wt[j];
         acc[2][j][sl] = fx[j];
         acc[3][j][sl] = fy[j];
-        pres[j][sl] = 1;
       }
     }
     if (writer) {
@@ -1202,7 +1197,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
     double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      if (pres[j][tid]) {
+      if ((cm >> j) & 1u) {
         m = m + acc[0][j][tid];
         a = a + acc[1][j][tid];
         fu = fu + acc[2][j][tid];
@@ -1750,6 +1745,7 @@ void tile_args(bl_handle_t* h, FusedArgs& A, int p) {
   A.tile_eptr = h->d_tile_eptr;
   A.tile_nptr = h->d_tile_nptr;
   A.tile_nodes = h->d_tile_nodes;
+  A.tile_ncm = h->d_tile_ncm;
   A.tile_el = h->d_tile_el;
   A.tile_nd = h->d_tile_nd;
   A.rho = h->elset[p][0]; A.en = h->elset[p][1]; A.pre = h->elset[p][2]; A.cs2 = h->elset[p][3];

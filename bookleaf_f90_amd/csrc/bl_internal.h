@@ -200,6 +200,7 @@ struct Handle {
   int32_t* d_tile_eptr = nullptr;  // ntile + 1
   int32_t* d_tile_nptr = nullptr;  // ntile + 1
   int32_t* d_tile_nodes = nullptr; // the tile's nodes, 0-based | (it writes the node) << 31
+  uint8_t* d_tile_ncm = nullptr;   // beside d_tile_nodes: bit jj, the node is some element's corner jj
   int2* d_tile_el = nullptr;       // (element | flags << 30, four local node slots)
   int4* d_tile_nd = nullptr;       // the entries' four nodes, 0-based
   int32_t* d_ens_toff = nullptr;   // ensembles: ens_n + 1 tile offsets per instance

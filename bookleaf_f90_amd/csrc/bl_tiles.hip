@@ -109,11 +109,12 @@ int upload_vec(T** d, const std::vector<T>& v) {
 }  // namespace
 
 void free_tiles(Handle* h) {
-  void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_nodes, h->d_tile_el, h->d_tile_nd,
-                  h->d_rest8, h->d_ens_toff};
+  void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_nodes, h->d_tile_ncm, h->d_tile_el,
+                  h->d_tile_nd, h->d_rest8, h->d_ens_toff};
   for (void* p : bufs)
     if (p) hipFree(p);
   h->d_tile_eptr = h->d_tile_nptr = h->d_tile_nodes = nullptr;
+  h->d_tile_ncm = nullptr;
   h->d_tile_el = nullptr;
   h->d_tile_nd = nullptr;
   h->d_ens_toff = nullptr;
@@ -191,9 +192,11 @@ int build_tiles(Handle* h) {
   // the node, the one holding the node's first incident element), then the
   // entries: its own elements ascending, then the ring around them
   std::vector<int32_t> nptr(ntile + 1, 0), nodes, eptr(ntile + 1, 0);
+  std::vector<uint8_t> ncm;
   std::vector<int2> ents;
   std::vector<int4> ent_nd;
   nodes.reserve((size_t)nnd + nnd / 4);
+  ncm.reserve((size_t)nnd + nnd / 4);
   ents.reserve((size_t)nel + nel / 2);
   ent_nd.reserve((size_t)nel + nel / 2);
   std::vector<int32_t> nstamp(nnd, -1), slot_of(nnd, 0), estamp(nel, -1), tn, own, ring;
@@ -224,6 +227,11 @@ int build_tiles(Handle* h) {
       slot_of[v] = (int)k;
       const bool writes = tile_of[adj[adj_ptr[v]]] == t;
       nodes.push_back((int32_t)((uint32_t)v | (writes ? 0x80000000u : 0u)));
+      // the corners jj the node is (every element around it is an entry
+      // of this tile, own or ring)
+      uint8_t cm = 0;
+      for (int j = 0; j < 4; ++j) cm |= seen[4 * (size_t)v + j] ? (uint8_t)(1u << j) : 0u;
+      ncm.push_back(cm);
     }
     eptr[t] = (int)ents.size();
     for (int pass = 0; pass < 2; ++pass)
@@ -243,7 +251,8 @@ int build_tiles(Handle* h) {
   nptr[ntile] = (int)nodes.size();
   eptr[ntile] = (int)ents.size();
   if (upload_vec(&h->d_tile_eptr, eptr) || upload_vec(&h->d_tile_nptr, nptr) ||
-      upload_vec(&h->d_tile_nodes, nodes) || upload_vec(&h->d_tile_el, ents) ||
+      upload_vec(&h->d_tile_nodes, nodes) || upload_vec(&h->d_tile_ncm, ncm) ||
+      upload_vec(&h->d_tile_el, ents) ||
       upload_vec(&h->d_tile_nd, ent_nd) ||
       (!toff.empty() && upload_vec(&h->d_ens_toff, toff)))
     return BL_FAILURE;
