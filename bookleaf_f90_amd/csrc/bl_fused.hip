@@ -167,22 +167,38 @@ __device__ __forceinline__ void f1_viscosity(const FusedArgs& A, const FusedNode
   geteviscosity(x, y, u, v, ed, ed, A.zerocut, qx, qy);
 }
 
+// the region switches of e's CFL term (zdtnotreg, zmidlength)
+struct CflIn {
+  int dtnot, mid;
+};
+// (of region ireg, 0-based)
+__device__ __forceinline__ CflIn cfl_of_region(const FusedArgs& A, int ireg) {
+  CflIn r{0, 0};
+  if (A.reg_tables) {
+    r.dtnot = A.zdtnotreg[ireg];
+    r.mid = A.zmidlength[ireg];
+  }
+  return r;
+}
+__device__ __forceinline__ CflIn cfl_in(const FusedArgs& A, int e) {
+  return cfl_of_region(A, A.reg_tables ? A.ielreg[e] - 1 : 0);
+}
+
 // the CFL term of e (hydro_kn_getdtcfl with cs2 + elvisc, hydro_dr_getcs2;
 // a mixed cell takes the frmass average of its components instead)
 __device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double x[4],
-                                         const double y[4], double cs2e, double ev) {
+                                         const double y[4], double cs2e, double ev,
+                                         CflIn r) {
   double c2 = cs2e + ev;
   if (A.mx.ncp > 0) {
     const int im = A.ielmat[e];
     if (im < 0) c2 = mix_cs2(A.mx, -im - 1);
   }
-  int dtnot = 0, mid = 0;
-  if (A.reg_tables) {
-    int ireg = A.ielreg[e] - 1;
-    dtnot = A.zdtnotreg[ireg];
-    mid = A.zmidlength[ireg];
-  }
-  return dtnot ? DBL_MAX : minlength(x, y, A.zcut, mid != 0) / c2;
+  return r.dtnot ? DBL_MAX : minlength(x, y, A.zcut, r.mid != 0) / c2;
+}
+__device__ __forceinline__ double f1_cfl(const FusedArgs& A, int e, const double x[4],
+                                         const double y[4], double cs2e, double ev) {
+  return f1_cfl(A, e, x, y, cs2e, ev, cfl_in(A, e));
 }
 
 // the wave's word of the at-rest mask, and the count of elements whose
@@ -213,12 +229,18 @@ __device__ __forceinline__ void rest_update(const FusedArgs& A, int e, int lane,
 // (hydro_kn_getdtdiv)
 __device__ __forceinline__ void f1_terms(const FusedArgs& A, int e, const double x[4],
                                          const double y[4], const double u[4], const double v[4],
-                                         const Iso& g, double cs2e, bool& moving, double& cflv,
-                                         double& divv) {
+                                         const Iso& g, double cs2e, CflIn r, bool& moving,
+                                         double& cflv, double& divv) {
   moving = !(u[0] == u[1] && u[0] == u[2] && u[0] == u[3] && v[0] == v[1] && v[0] == v[2] &&
              v[0] == v[3]);
-  if (!moving) cflv = f1_cfl(A, e, x, y, cs2e, 0.0);
+  if (!moving) cflv = f1_cfl(A, e, x, y, cs2e, 0.0, r);
   divv = divterm(u, v, g.a1, g.a3, g.b1, g.b3, g.vol);
+}
+__device__ __forceinline__ void f1_terms(const FusedArgs& A, int e, const double x[4],
+                                         const double y[4], const double u[4], const double v[4],
+                                         const Iso& g, double cs2e, bool& moving, double& cflv,
+                                         double& divv) {
+  f1_terms(A, e, x, y, u, v, g, cs2e, cfl_in(A, e), moving, cflv, divv);
 }
 
 // the workgroup's DIV and CFL partials, at part[blockIdx.x] and
@@ -968,18 +990,31 @@ k_f3_accel(FusedArgs A, FusedNodes N, FusedNodesOut O) {
 // energy with ubar, EOS, written to the output element fields (en_o ...,
 // the same arrays as en ... unless the step writes the other element set);
 // returns the new cs2
-__device__ __forceinline__ double f4_state_of(const FusedArgs& A, int e, double dt,
-                                              const double x[4], const double y[4],
-                                              const double ub[4], const double vb[4],
-                                              const double fx[4], const double fy[4], Iso& g) {
-  getiso(x, y, g);
-  const double m = A.mass[e];
-  const double rho = m / g.vol;
-  const double en = energy_update(A.en[e], m, fx, fy, ub, vb, dt, A.zerocut);
-  int mat = A.ielmat[e];
+// the element inputs of F4: mass, energy at the step's start, EOS type and
+// parameters of its material (eos_param(:, max(mat, 0)))
+struct ElIn {
+  double m, en;
+  int et;
+  double p[BL_NEOSPARAM];
+};
+// the EOS of material mat (ielmat; 0 for a value <= 0)
+__device__ __forceinline__ void el_eos(const FusedArgs& A, int mat, ElIn& r) {
   mat = mat > 0 ? mat : 0;
-  const int et = A.eos_itype[mat];
-  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
+  r.et = A.eos_itype[mat];
+#pragma unroll
+  for (int k = 0; k < BL_NEOSPARAM; ++k) r.p[k] = A.eos_param[BL_NEOSPARAM * mat + k];
+}
+
+// F4 with its element inputs m, en0 (energy at the step's start), the EOS
+// type et and parameters ep
+__device__ __forceinline__ double f4_state_from(const FusedArgs& A, int e, double dt, double m,
+                                                double en0, int et, const double* ep,
+                                                const double x[4], const double y[4],
+                                                const double ub[4], const double vb[4],
+                                                const double fx[4], const double fy[4], Iso& g) {
+  getiso(x, y, g);
+  const double rho = m / g.vol;
+  const double en = energy_update(en0, m, fx, fy, ub, vb, dt, A.zerocut);
   A.en_o[e] = en;
   A.rho_o[e] = rho;
   A.pre_o[e] = eos_pressure(et, ep, rho, en, A.pcut);
@@ -994,6 +1029,15 @@ __device__ __forceinline__ double f4_state_of(const FusedArgs& A, int e, double 
   // last: no load of the element passes this atomic
   if (g.vol < 0.0) atomicMax(&A.dtres->negvol2, e + 1);
   return c2;
+}
+__device__ __forceinline__ double f4_state_of(const FusedArgs& A, int e, double dt,
+                                              const double x[4], const double y[4],
+                                              const double ub[4], const double vb[4],
+                                              const double fx[4], const double fy[4], Iso& g) {
+  int mat = A.ielmat[e];
+  mat = mat > 0 ? mat : 0;
+  return f4_state_from(A, e, dt, A.mass[e], A.en[e], A.eos_itype[mat],
+                       A.eos_param + BL_NEOSPARAM * mat, x, y, ub, vb, fx, fy, g);
 }
 
 // F4's work for element e with its inputs from F3's node set and F2's
@@ -1155,15 +1199,14 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
     rdt = A.inst_rdt[inst];
   }
   // --- F2: every element around the tile's nodes, one per thread
-  int e = 0;
-  uint32_t flags = 0, slots = 0xFFFFFFFFu;
+  // (every thread's entry names an element of the tile)
+  const int e = ent.x & 0x3fffffff;
+  const uint32_t flags = active ? (uint32_t)ent.x >> 30 : 0u;
+  const uint32_t slots = (uint32_t)ent.y;
   double fx[4], fy[4];
   bool zeroed = false;
   if (active) {
     const int nd[4] = {q.x, q.y, q.z, q.w};
-    e = ent.x & 0x3fffffff;
-    flags = (uint32_t)ent.x >> 30;
-    slots = (uint32_t)ent.y;
     const bool writer = (flags & kTileOwn) != 0;
     const bool at_rest = A.rest8[e] != 0;
     double wt[4], mt[4];
@@ -1192,6 +1235,17 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const uint64_t wrote = __ballot(zeroed);
   if ((tid & 63) == 0 && wrote) count_zeroed(A, wrote);
   __syncthreads();
+  // phase C's element inputs, loaded before phase B so that their latency
+  // overlaps it (the tables they index after it); every thread loads them
+  // (a ring element's are L2 hits), as loads under a branch make the
+  // compiler wait for them where the branches join
+  const bool interior = active && (flags & kTileOwn);
+  ElIn ein;
+  ein.m = A.mass[e];
+  ein.en = A.en[e];
+  int emat = A.ielmat[e];
+  // (used with reg_tables only)
+  int ereg = (A.reg_tables ? A.ielreg : A.ielmat)[e];
   // --- F3: the tile's nodes, their corners in jj order
   if (own) {
     double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
@@ -1216,9 +1270,13 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) nout[k][tid] = out[k];
   }
+  // (an empty asm: the indices are used from here, so the compiler does not
+  // wait for their loads before phase B)
+  asm volatile("" : "+v"(emat), "+v"(ereg));
+  el_eos(A, emat, ein);
+  const CflIn cin = cfl_of_region(A, ereg - 1);
   __syncthreads();
   // --- F4 and the next step's F1 terms: the elements held whole
-  const bool interior = active && (flags & kTileOwn);
   double cflv = DBL_MAX, divv = -1.0;
   bool moving = false;
   if (interior) {
@@ -1234,8 +1292,9 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
       vb[j] = nout[5][sl];
     }
     Iso g;
-    const double c2 = f4_state_of(A, e, dt, x, y, ub, vb, fx, fy, g);
-    f1_terms(A, e, x, y, u, v, g, c2, moving, cflv, divv);
+    const double c2 =
+        f4_state_from(A, e, dt, ein.m, ein.en, ein.et, ein.p, x, y, ub, vb, fx, fy, g);
+    f1_terms(A, e, x, y, u, v, g, c2, cin, moving, cflv, divv);
     A.rest8_next[e] = moving ? 0 : 1;
   }
   // an ensemble's next-step limiter reads its instance's worklist
