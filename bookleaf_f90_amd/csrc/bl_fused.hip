@@ -804,8 +804,14 @@ __device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes&
                                            bool at_rest, bool check_volume, double fx[4],
                                            double fy[4], double wt[4], double mt[4]) {
   const double h = 0.5 * dt;  // geometry_dr_getvertex: 0.5*dt
+  // the element's inputs first, in one group of loads before any branch
+  // (the compiler waits for every load in flight where branches join); the
+  // region index is a load either way, used with reg_tables_force only
   double x[4], y[4], u[4], v[4];
   load4(N.x, nd, x); load4(N.y, nd, y); load4(N.u, nd, u); load4(N.v, nd, v);
+  const double pre = A.pre[e], m = A.mass[e], en0 = A.en[e];
+  int mat = A.ielmat[e];
+  const int ireg_raw = (A.reg_tables_force ? A.ielreg : A.ielmat)[e];
   double qx[4], qy[4];
   if (at_rest) {
 #pragma unroll
@@ -814,12 +820,18 @@ __device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes&
     ld4(A.cnviscx, e, qx);
     ld4(A.cnviscy, e, qy);
   }
-  int ireg = 0;
-  if (A.reg_tables_force) ireg = A.ielreg[e] - 1;
+  mat = mat > 0 ? mat : 0;
+  const int et = A.eos_itype[mat];
+  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
+  const int ireg = A.reg_tables_force ? ireg_raw - 1 : 0;
+  // the hourglass coefficient and the EOS's first parameter, loaded ahead
+  // of their use (each a load either way)
+  const double kap = (A.zhg ? A.kappareg : A.eos_param)[A.zhg ? ireg : 0];
+  const double ep0 = ep[0];
   // --- predictor: hydro_dr_getforce at x^n with p^n
   Iso g;
   getiso(x, y, g);
-  forceP(A.pre[e], g.a1, g.a3, g.b1, g.b3, fx, fy);
+  forceP(pre, g.a1, g.a3, g.b1, g.b3, fx, fy);
 #pragma unroll
   for (int c = 0; c < 4; ++c) { fx[c] = fx[c] + qx[c]; fy[c] = fy[c] + qy[c]; }
   if (A.zsp) {
@@ -835,14 +847,12 @@ __device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes&
   for (int c = 0; c < 4; ++c) { xh[c] = x[c] + h * u[c]; yh[c] = y[c] + h * v[c]; }
   Iso gh;
   getiso(xh, yh, gh);
-  const double m = A.mass[e];
   const double rhoh = m / gh.vol;
-  const double eh = energy_update(A.en[e], m, fx, fy, u, v, h, A.zerocut);
-  int mat = A.ielmat[e];
-  mat = mat > 0 ? mat : 0;
-  const int et = A.eos_itype[mat];
-  const double* ep = A.eos_param + BL_NEOSPARAM * mat;
-  const double ph = eos_pressure(et, ep, rhoh, eh, A.pcut);
+  const double eh = energy_update(en0, m, fx, fy, u, v, h, A.zerocut);
+  const double ph = eos_pressure(et, ep0, ep, rhoh, eh, A.pcut);
+  // the nodal masses, loaded here so that the corrector hides their latency
+  double cm[4];
+  ld4(A.cnmass, e, cm);
   // --- corrector force at the half step (zcorrector: HG on)
   forceP(ph, gh.a1, gh.a3, gh.b1, gh.b3, fx, fy);
 #pragma unroll
@@ -853,11 +863,9 @@ __device__ __forceinline__ void f2_element(const FusedArgs& A, const FusedNodes&
     const double cs2h = eos_cs2(et, ep, rhoh, eh, A.pcut, A.ccut);
     forceSP(A.pmeritreg[ireg], rhoh, cs2h, xh, yh, spm, fx, fy);
   }
-  if (A.zhg) forceHG(A.kappareg[ireg], rhoh, gh.vol, rdt, u, v, fx, fy);
+  if (A.zhg) forceHG(kap, rhoh, gh.vol, rdt, u, v, fx, fy);
   // nodal-mass term of each corner: cnmass, else the previous corner's
   // cnmass, else rho*cnwt at the half step
-  double cm[4];
-  ld4(A.cnmass, e, cm);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (cm[j] > A.zerocut) mt[j] = cm[j];
@@ -1183,13 +1191,10 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const uint32_t cm = A.tile_ncm[nb + min(tid, nn - 1)];
   const int2 ent = A.tile_el[eb + min(tid, ne - 1)];
   const int4 q = A.tile_nd[eb + min(tid, ne - 1)];
-  // the tile's node's start-of-step values, for F3 below; the high bit of
-  // its entry: this tile writes it (other tiles holding it compute the
-  // same values for their own elements)
+  // the high bit of the node's entry: this tile writes it (other tiles
+  // holding it compute the same values for their own elements)
   const int n = (int)(nv & 0x7fffffffu);
   const bool writes = own && (nv >> 31) != 0;
-  const double x0 = N.x[n], y0 = N.y[n], u0 = N.u[n], v0 = N.v[n];
-  const int ntype = A.indtype[n];
   // an ensemble tile lies in one instance: its dt
   int inst = 0;
   double dt = ts->dt, rdt = ts->rdt;
@@ -1203,12 +1208,15 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const int e = ent.x & 0x3fffffff;
   const uint32_t flags = active ? (uint32_t)ent.x >> 30 : 0u;
   const uint32_t slots = (uint32_t)ent.y;
+  // its at-rest bytes now and in the step before, and whether this is the
+  // batch's last step, read before any branch
+  const bool at_rest = A.rest8[e] != 0, rest_before = A.rest8_prev[e] != 0;
+  const bool last = ts->last != 0;
   double fx[4], fy[4];
   bool zeroed = false;
   if (active) {
     const int nd[4] = {q.x, q.y, q.z, q.w};
     const bool writer = (flags & kTileOwn) != 0;
-    const bool at_rest = A.rest8[e] != 0;
     double wt[4], mt[4];
     f2_element(A, N, e, nd, dt, rdt, at_rest, writer, fx, fy, wt, mt);
 #pragma unroll
@@ -1224,11 +1232,11 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
     if (writer) {
       // the batch's last step leaves the corner forces in rscratch23/24
       // (lagstep cnfx/cnfy)
-      if (ts->last) {
+      if (last) {
         st4(A.cnfx, e, fx);
         st4(A.cnfy, e, fy);
       }
-      zeroed = at_rest && !A.rest8_prev[e];
+      zeroed = at_rest && !rest_before;
       if (zeroed) write_rest_zeros(A, e);
     }
   }
@@ -1240,6 +1248,10 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   // (a ring element's are L2 hits), as loads under a branch make the
   // compiler wait for them where the branches join
   const bool interior = active && (flags & kTileOwn);
+  // the tile's node's start-of-step values for F3 (L2 hits: phase A's
+  // elements read them), loaded here to spare phase A's registers
+  const double x0 = N.x[n], y0 = N.y[n], u0 = N.u[n], v0 = N.v[n];
+  const int ntype = A.indtype[n];
   ElIn ein;
   ein.m = A.mass[e];
   ein.en = A.en[e];

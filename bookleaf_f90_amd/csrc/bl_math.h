@@ -101,12 +101,13 @@ __device__ __forceinline__ double minlength(const double x[4], const double y[4]
 
 // eos_kn_getpressure / eos_kn_getcs2 (eos_kn_forms.f90:28-106); param points
 // at the 6 parameters of material imat = max(mat,0)
-__device__ __forceinline__ double eos_pressure(int type, const double* p, double density,
-                                               double energy, double pcut) {
+// (p0: p[0], which a caller may have loaded ahead)
+__device__ __forceinline__ double eos_pressure(int type, double p0, const double* p,
+                                               double density, double energy, double pcut) {
   double pressure, t1, t2, t3, t4, t5;
   switch (type) {
-    case 0: pressure = p[0]; break;
-    case 1: pressure = energy * density * (p[0] - 1.0); break;
+    case 0: pressure = p0; break;
+    case 1: pressure = energy * density * (p0 - 1.0); break;
     case 2:
       t1 = density / p[2];
       pressure = p[0] * (pow(t1, p[1]) - 1.0);
@@ -124,6 +125,10 @@ __device__ __forceinline__ double eos_pressure(int type, const double* p, double
   }
   if (fabs(pressure) < pcut) pressure = 0.0;
   return pressure;
+}
+__device__ __forceinline__ double eos_pressure(int type, const double* p, double density,
+                                               double energy, double pcut) {
+  return eos_pressure(type, p[0], p, density, energy, pcut);
 }
 __device__ __forceinline__ double eos_cs2(int type, const double* p, double density,
                                           double energy, double pcut, double ccut) {
