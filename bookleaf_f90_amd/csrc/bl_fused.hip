@@ -172,13 +172,13 @@ struct CflIn {
   int dtnot, mid;
 };
 // (of region ireg, 0-based)
+// (loads either way, from the element's material when there are no region
+// tables, so that no branch joins behind them)
 __device__ __forceinline__ CflIn cfl_of_region(const FusedArgs& A, int ireg) {
-  CflIn r{0, 0};
-  if (A.reg_tables) {
-    r.dtnot = A.zdtnotreg[ireg];
-    r.mid = A.zmidlength[ireg];
-  }
-  return r;
+  const int k = A.reg_tables ? ireg : 0;
+  const int dtnot = (A.reg_tables ? A.zdtnotreg : A.ielmat)[k];
+  const int mid = (A.reg_tables ? A.zmidlength : A.ielmat)[k];
+  return CflIn{A.reg_tables ? dtnot : 0, A.reg_tables ? mid : 0};
 }
 __device__ __forceinline__ CflIn cfl_in(const FusedArgs& A, int e) {
   return cfl_of_region(A, A.reg_tables ? A.ielreg[e] - 1 : 0);
@@ -886,7 +886,8 @@ __device__ __forceinline__ void write_rest_zeros(const FusedArgs& A, int e) {
   A.elvisc[e] = 0.0;
 }
 
-__global__ void __launch_bounds__(kBlock)
+// (three waves per SIMD: at most 168 VGPRs)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_f2_force(FusedArgs A, FusedNodes N, int nbm) {
   const DevTimestep* ts = A.ts;
   if (ts->skip || ens_pending(A, ts)) return;
@@ -1169,7 +1170,8 @@ __device__ __forceinline__ void seg_collect(const FusedArgs& A, int seg, int nse
   block_reduce_at<true>(cflv, idx, A.part + 2 * nseg + seg, A.parti + 2 * nseg + seg);
 }
 
-__global__ void __launch_bounds__(kBlock)
+// (three waves per SIMD: at most 168 VGPRs; LDS allows three workgroups)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const DevTimestep* ts = A.ts;
   if (ts->skip || ens_pending(A, ts)) return;
@@ -1252,12 +1254,18 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   // elements read them), loaded here to spare phase A's registers
   const double x0 = N.x[n], y0 = N.y[n], u0 = N.u[n], v0 = N.v[n];
   const int ntype = A.indtype[n];
+  // (F3 waits for these alone: the loads below issue after them)
+  __builtin_amdgcn_sched_barrier(0);
   ElIn ein;
   ein.m = A.mass[e];
   ein.en = A.en[e];
-  int emat = A.ielmat[e];
+  const int emat = A.ielmat[e];
   // (used with reg_tables only)
-  int ereg = (A.reg_tables ? A.ielreg : A.ielmat)[e];
+  const int ereg = (A.reg_tables ? A.ielreg : A.ielmat)[e];
+  // the EOS and region tables of those (their loads overlap F3; the index
+  // loads arrive with the node values F3 waits for)
+  el_eos(A, emat, ein);
+  const CflIn cin = cfl_of_region(A, ereg - 1);
   // --- F3: the tile's nodes, their corners in jj order
   if (own) {
     double m = 0.0, a = 0.0, fu = 0.0, fv = 0.0;
@@ -1282,11 +1290,6 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) nout[k][tid] = out[k];
   }
-  // (an empty asm: the indices are used from here, so the compiler does not
-  // wait for their loads before phase B)
-  asm volatile("" : "+v"(emat), "+v"(ereg));
-  el_eos(A, emat, ein);
-  const CflIn cin = cfl_of_region(A, ereg - 1);
   __syncthreads();
   // --- F4 and the next step's F1 terms: the elements held whole
   double cflv = DBL_MAX, divv = -1.0;
