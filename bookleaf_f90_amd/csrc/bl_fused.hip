@@ -1165,9 +1165,8 @@ __device__ __forceinline__ void seg_collect(const FusedArgs& A, int seg, int nse
   const int idx = active ? e + 1 : 0x7fffffff;
   if (!active || !(divv == divv)) divv = -1.0;      // a NaN never wins the strict '>' scan
   if (!active || !(cflv == cflv)) cflv = DBL_MAX;   // nor the strict '<' scan
-  block_reduce_at<false>(divv, idx, A.part + seg, A.parti + seg);
-  __syncthreads();
-  block_reduce_at<true>(cflv, idx, A.part + 2 * nseg + seg, A.parti + 2 * nseg + seg);
+  block_reduce_div_cfl(divv, idx, cflv, idx, A.part + seg, A.parti + seg, A.part + 2 * nseg + seg,
+                       A.parti + 2 * nseg + seg);
 }
 
 // (three waves per SIMD: at most 168 VGPRs; LDS allows three workgroups)
@@ -1244,18 +1243,17 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   }
   const uint64_t wrote = __ballot(zeroed);
   if ((tid & 63) == 0 && wrote) count_zeroed(A, wrote);
+  // the tile's node's start-of-step values for F3 (L2 hits: phase A's
+  // elements read them), loaded after phase A's work to spare its registers
+  // and before the barrier, whose wait hides their latency
+  const double x0 = N.x[n], y0 = N.y[n], u0 = N.u[n], v0 = N.v[n];
+  const int ntype = A.indtype[n];
   __syncthreads();
   // phase C's element inputs, loaded before phase B so that their latency
   // overlaps it (the tables they index after it); every thread loads them
   // (a ring element's are L2 hits), as loads under a branch make the
   // compiler wait for them where the branches join
   const bool interior = active && (flags & kTileOwn);
-  // the tile's node's start-of-step values for F3 (L2 hits: phase A's
-  // elements read them), loaded here to spare phase A's registers
-  const double x0 = N.x[n], y0 = N.y[n], u0 = N.u[n], v0 = N.v[n];
-  const int ntype = A.indtype[n];
-  // (F3 waits for these alone: the loads below issue after them)
-  __builtin_amdgcn_sched_barrier(0);
   ElIn ein;
   ein.m = A.mass[e];
   ein.en = A.en[e];

@@ -54,6 +54,36 @@ __device__ __forceinline__ void block_reduce_at(double v, int i, double* outv, i
   }
 }
 
+// the DIV maximum (dv, di) and the CFL minimum (cv, ci) over the workgroup
+// together: one pass of shuffles for both, one barrier; thread 0 writes them
+__device__ __forceinline__ void block_reduce_div_cfl(double dv, int di, double cv, int ci,
+                                                     double* out_dv, int32_t* out_di,
+                                                     double* out_cv, int32_t* out_ci) {
+  __shared__ double sv[2][kBlock / 64];
+  __shared__ int si[2][kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double dv2 = __shfl_xor(dv, off, 64), cv2 = __shfl_xor(cv, off, 64);
+    const int di2 = __shfl_xor(di, off, 64), ci2 = __shfl_xor(ci, off, 64);
+    better_max(dv, di, dv2, di2);
+    better_min(cv, ci, cv2, ci2);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[0][wid] = dv; si[0][wid] = di;
+    sv[1][wid] = cv; si[1][wid] = ci;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      better_max(dv, di, sv[0][w], si[0][w]);
+      better_min(cv, ci, sv[1][w], si[1][w]);
+    }
+    *out_dv = dv; *out_di = di;
+    *out_cv = cv; *out_ci = ci;
+  }
+}
+
 // the partial of this workgroup at part[blockIdx.x]
 template <bool IS_MIN>
 __device__ __forceinline__ void block_reduce_store(double v, int i, double* part, int32_t* parti) {
