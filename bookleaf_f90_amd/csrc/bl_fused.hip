@@ -91,6 +91,16 @@ __device__ __forceinline__ void load_nd(const int32_t* __restrict__ ielnd, int e
 __device__ __forceinline__ void st4(double* __restrict__ a, int e, const double v[4]) {
   reinterpret_cast<double4*>(a)[e] = make_double4(v[0], v[1], v[2], v[3]);
 }
+// a streaming store, for results the step's kernels do not read again (the
+// batch's end writes them once): the cache keeps the step's working set
+typedef double nt_double4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st4_nt(double* __restrict__ a, int e, const double v[4]) {
+  const nt_double4 q = {v[0], v[1], v[2], v[3]};
+  __builtin_nontemporal_store(q, reinterpret_cast<nt_double4*>(a) + e);
+}
+__device__ __forceinline__ void st_nt(double* __restrict__ a, int e, double v) {
+  __builtin_nontemporal_store(v, a + e);
+}
 __device__ __forceinline__ void ld4(const double* __restrict__ a, int e, double v[4]) {
   double4 q = reinterpret_cast<const double4*>(a)[e];
   v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
@@ -1234,8 +1244,8 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
       // the batch's last step leaves the corner forces in rscratch23/24
       // (lagstep cnfx/cnfy)
       if (last) {
-        st4(A.cnfx, e, fx);
-        st4(A.cnfy, e, fy);
+        st4_nt(A.cnfx, e, fx);
+        st4_nt(A.cnfy, e, fy);
       }
       zeroed = at_rest && !rest_before;
       if (zeroed) write_rest_zeros(A, e);
@@ -2170,8 +2180,8 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
       fx[j] = fxT[j * nel + e];
       fy[j] = fyT[j * nel + e];
     }
-    st4(cnfx, e, fx);
-    st4(cnfy, e, fy);
+    st4_nt(cnfx, e, fx);
+    st4_nt(cnfy, e, fy);
   }
   int nd[4];
   load_nd(ielnd, e, nd);
@@ -2180,12 +2190,13 @@ __global__ void k_materialize(int nel, const int32_t* __restrict__ ielnd,
   load4(ndy, nd, y);
   Iso g;
   getiso(x, y, g);
-  st4(cnx, e, x);
-  st4(cny, e, y);
-  a1[e] = g.a1; a2[e] = g.a2; a3[e] = g.a3;
-  b1[e] = g.b1; b2[e] = g.b2; b3[e] = g.b3;
-  st4(cnwt, e, g.wt);
-  vol[e] = g.vol;
+  // (streaming stores: the next call's first step finds its state cached)
+  st4_nt(cnx, e, x);
+  st4_nt(cny, e, y);
+  st_nt(a1, e, g.a1); st_nt(a2, e, g.a2); st_nt(a3, e, g.a3);
+  st_nt(b1, e, g.b1); st_nt(b2, e, g.b2); st_nt(b3, e, g.b3);
+  st4_nt(cnwt, e, g.wt);
+  st_nt(vol, e, g.vol);
 }
 
 // The call's results for the host, in pinned coherent host memory, so that
