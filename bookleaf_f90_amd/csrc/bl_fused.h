@@ -127,6 +127,7 @@ struct FusedArgs {
   int ntile, nseg;
   const int32_t* tile_eptr;
   const int32_t* tile_nptr;
+  const int4* tile_bounds;  // (tile_eptr[t], tile_eptr[t+1], tile_nptr[t], tile_nptr[t+1])
   const int32_t* tile_nodes;
   const uint8_t* tile_ncm;  // per tile node: bit jj, the node is corner jj of an element
   const int2* tile_el;

@@ -450,6 +450,11 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
 // last workgroup needs a device-scope fence in every workgroup, which costs
 // far more than the launch with an L2 per XCD.)
 __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb, int nbl) {
+  // F1L's partials first: their loads do not wait for the gate below
+  double va, vc;
+  int xa, xc;
+  final_dt_gather(nbl, A.part + nb, A.parti + nb, nbl, A.part + 3 * nb, A.parti + 3 * nb, va, xa,
+                  vc, xc);
   DevTimestep* ts = A.ts;
   if (A.end_in_f1b) {
     if (threadIdx.x == 0) step_end(ts, A.dtres);
@@ -465,8 +470,7 @@ __global__ void __launch_bounds__(kFinalBlock) k_f1b_select(FusedArgs A, int nb,
     if (A.end_in_f1b && !skip) ts->open = 1;
   }
   if (skip) return;
-  final_dt_reduce(nbl, A.part + nb, A.parti + nb, nbl, A.part + 3 * nb, A.parti + 3 * nb,
-                  A.dtres);
+  final_dt_combine(va, xa, vc, xc, A.dtres);
   if (threadIdx.x != 0) return;
   const DtResult& r = *A.dtres;
   if (r.cfl_min < 0.0) {  // hydro_kn_getdt.f90:75-78
@@ -1182,16 +1186,16 @@ __device__ __forceinline__ void seg_collect(const FusedArgs& A, int seg, int nse
 // (three waves per SIMD: at most 168 VGPRs; LDS allows three workgroups)
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
 k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
-  const DevTimestep* ts = A.ts;
-  if (ts->skip || ens_pending(A, ts)) return;
   // slot-fastest, so that a wave's lanes of one (quantity, corner) are
   // consecutive doubles: (quantity m, area, fx, fy; corner jj; node slot)
   __shared__ double acc[4][4][kTileNodes + 1];
   __shared__ double nout[6][kTileNodes + 1];  // x, y, u, v at n+1, ubar, vbar
   const int t = xcd_block();
   const int tid = threadIdx.x;
-  const int eb = A.tile_eptr[t], ne = A.tile_eptr[t + 1] - eb;
-  const int nb = A.tile_nptr[t], nn = A.tile_nptr[t + 1] - nb;
+  // the tile's bounds: one scalar load
+  const int4 tb = A.tile_bounds[t];
+  const int eb = tb.x, ne = tb.y - tb.x;
+  const int nb = tb.z, nn = tb.w - tb.z;
   // every thread loads a node entry and an element entry (a tile has at
   // least one of each; threads past the end reload the last), so that these
   // loads, and the node values after them, are in flight together; no
@@ -1202,6 +1206,8 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const uint32_t cm = A.tile_ncm[nb + min(tid, nn - 1)];
   const int2 ent = A.tile_el[eb + min(tid, ne - 1)];
   const int4 q = A.tile_nd[eb + min(tid, ne - 1)];
+  const DevTimestep* ts = A.ts;
+  if (ts->skip || ens_pending(A, ts)) return;
   // the high bit of the node's entry: this tile writes it (other tiles
   // holding it compute the same values for their own elements)
   const int n = (int)(nv & 0x7fffffffu);
@@ -1826,6 +1832,7 @@ void tile_args(bl_handle_t* h, FusedArgs& A, int p) {
   A.nseg = h->ntile;
   A.tile_eptr = h->d_tile_eptr;
   A.tile_nptr = h->d_tile_nptr;
+  A.tile_bounds = h->d_tile_bounds;
   A.tile_nodes = h->d_tile_nodes;
   A.tile_ncm = h->d_tile_ncm;
   A.tile_el = h->d_tile_el;

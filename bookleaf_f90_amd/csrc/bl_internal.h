@@ -199,6 +199,7 @@ struct Handle {
   int32_t ntile = 0, nentries = 0;
   int32_t* d_tile_eptr = nullptr;  // ntile + 1
   int32_t* d_tile_nptr = nullptr;  // ntile + 1
+  int4* d_tile_bounds = nullptr;   // per tile (eptr[t], eptr[t+1], nptr[t], nptr[t+1])
   int32_t* d_tile_nodes = nullptr; // the tile's nodes, 0-based | (it writes the node) << 31
   uint8_t* d_tile_ncm = nullptr;   // beside d_tile_nodes: bit jj, the node is some element's corner jj
   int2* d_tile_el = nullptr;       // (element | flags << 30, four local node slots)

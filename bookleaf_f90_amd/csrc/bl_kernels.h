@@ -103,20 +103,48 @@ __device__ __forceinline__ void block_argmax_first(double v, int i, double* part
 // part[0,na), CFL partials in part[2na, 2na+nc); both in one pass so the
 // loads of a thread are independent
 constexpr int kFinalBlock = 1024;
-// DIV partials (pa, qa)[0, na), CFL partials (pc, qc)[0, nc)
-__device__ __forceinline__ void final_dt_reduce(int na, const double* part, const int32_t* parti,
+// this thread's share of the DIV partials (part, parti)[0, na) and the CFL
+// partials (pc, qc)[0, nc): DIV maximum (va, xa), CFL minimum (vc, xc)
+__device__ __forceinline__ void final_dt_gather(int na, const double* part, const int32_t* parti,
                                                 int nc, const double* pc, const int32_t* qc,
-                                                DtResult* res) {
+                                                double& va, int& xa, double& vc, int& xc) {
+  va = -1.0; vc = DBL_MAX;
+  xa = 0x7fffffff; xc = 0x7fffffff;
+  const int nmax = na > nc ? na : nc;
+  if (na < 1 || nc < 1) {  // (not a case the step launches)
+    for (int k = threadIdx.x; k < nmax; k += blockDim.x) {
+      if (k < na) better_max(va, xa, part[k], parti[k]);
+      if (k < nc) better_min(vc, xc, pc[k], qc[k]);
+    }
+    return;
+  }
+  // kPer entries of each control per thread and pass, all loaded (from
+  // clamped indices, so with no branch) before any is compared
+  constexpr int kPer = 4;
+  const int stride = (int)blockDim.x;
+  for (int b = threadIdx.x; b < nmax; b += kPer * stride) {
+    double a[kPer], c[kPer];
+    int ai[kPer], ci[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int k = b + r * stride;
+      const int ka = k < na ? k : na - 1, kc = k < nc ? k : nc - 1;
+      a[r] = part[ka]; ai[r] = parti[ka];
+      c[r] = pc[kc]; ci[r] = qc[kc];
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int k = b + r * stride;
+      if (k < na) better_max(va, xa, a[r], ai[r]);
+      if (k < nc) better_min(vc, xc, c[r], ci[r]);
+    }
+  }
+}
+// the workgroup's combination of the threads' shares into res
+__device__ __forceinline__ void final_dt_combine(double va, int xa, double vc, int xc,
+                                                 DtResult* res) {
   __shared__ double sa[kFinalBlock / 64], sc[kFinalBlock / 64];
   __shared__ int ia[kFinalBlock / 64], ic[kFinalBlock / 64];
-  double va = -1.0, vc = DBL_MAX;
-  int xa = 0x7fffffff, xc = 0x7fffffff;
-  const int nmax = na > nc ? na : nc;
-#pragma unroll 4
-  for (int k = threadIdx.x; k < nmax; k += blockDim.x) {
-    if (k < na) better_max(va, xa, part[k], parti[k]);
-    if (k < nc) better_min(vc, xc, pc[k], qc[k]);
-  }
   wave_reduce<false>(va, xa);
   wave_reduce<true>(vc, xc);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -133,6 +161,15 @@ __device__ __forceinline__ void final_dt_reduce(int na, const double* part, cons
     res->cfl_idx = ic[0];
   }
   __syncthreads();
+}
+// DIV partials (pa, qa)[0, na), CFL partials (pc, qc)[0, nc)
+__device__ __forceinline__ void final_dt_reduce(int na, const double* part, const int32_t* parti,
+                                                int nc, const double* pc, const int32_t* qc,
+                                                DtResult* res) {
+  double va, vc;
+  int xa, xc;
+  final_dt_gather(na, part, parti, nc, pc, qc, va, xa, vc, xc);
+  final_dt_combine(va, xa, vc, xc, res);
 }
 __device__ __forceinline__ void final_dt_reduce(int na, int nc, const double* part,
                                                 const int32_t* parti, DtResult* res) {

@@ -109,12 +109,13 @@ int upload_vec(T** d, const std::vector<T>& v) {
 }  // namespace
 
 void free_tiles(Handle* h) {
-  void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_nodes, h->d_tile_ncm, h->d_tile_el,
+  void* bufs[] = {h->d_tile_eptr, h->d_tile_nptr, h->d_tile_bounds, h->d_tile_nodes, h->d_tile_ncm, h->d_tile_el,
                   h->d_tile_nd, h->d_rest8, h->d_ens_toff};
   for (void* p : bufs)
     if (p) hipFree(p);
   h->d_tile_eptr = h->d_tile_nptr = h->d_tile_nodes = nullptr;
   h->d_tile_ncm = nullptr;
+  h->d_tile_bounds = nullptr;
   h->d_tile_el = nullptr;
   h->d_tile_nd = nullptr;
   h->d_ens_toff = nullptr;
@@ -250,7 +251,10 @@ int build_tiles(Handle* h) {
   }
   nptr[ntile] = (int)nodes.size();
   eptr[ntile] = (int)ents.size();
+  std::vector<int4> bounds(ntile);
+  for (int t = 0; t < ntile; ++t) bounds[t] = make_int4(eptr[t], eptr[t + 1], nptr[t], nptr[t + 1]);
   if (upload_vec(&h->d_tile_eptr, eptr) || upload_vec(&h->d_tile_nptr, nptr) ||
+      upload_vec(&h->d_tile_bounds, bounds) ||
       upload_vec(&h->d_tile_nodes, nodes) || upload_vec(&h->d_tile_ncm, ncm) ||
       upload_vec(&h->d_tile_el, ents) ||
       upload_vec(&h->d_tile_nd, ent_nd) ||
