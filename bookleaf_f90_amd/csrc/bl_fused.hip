@@ -378,10 +378,12 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
   int bi = 0x7fffffff, di = 0x7fffffff;
   const int per = (nb1 + (int)gridDim.x - 1) / (int)gridDim.x;
   const int lo = (int)blockIdx.x * per, hi = min(nb1, lo + per);
-  for (int k = lo + (int)threadIdx.x; k < hi; k += (int)blockDim.x) {
-    better_max(dv, di, A.part[k], A.parti[k]);
-    better_min(best, bi, A.part[2 * nb1 + k], A.parti[2 * nb1 + k]);
-  }
+  // this thread's first partial of each control, loaded now (from a clamped
+  // index) and compared after the scan, so that the scan's first loads do
+  // not wait for these
+  const int k0 = lo + (int)threadIdx.x, kc = min(k0, nb1 - 1);
+  const double fdv = A.part[kc], fcv = A.part[2 * nb1 + kc];
+  const int fdi = A.parti[kc], fci = A.parti[2 * nb1 + kc];
   const int nbe = (A.nel + kBlock - 1) / kBlock;
   const int lb = xcd_block();
   const int e0 = (int)(((long long)lb * nbe) / gridDim.x) * kBlock;
@@ -439,9 +441,18 @@ k_f1_limiter(FusedArgs A, FusedNodes N, int nb1) {
     __syncthreads();  // the list is refilled
   }
   if (threadIdx.x == 0 && listed) atomicAdd(A.vstats, (unsigned long long)listed);
-  block_argmin_first(best, bi, A.part + 3 * nb1, A.parti + 3 * nb1);
-  __syncthreads();
-  block_reduce_store<false>(dv, di, A.part + nb1, A.parti + nb1);
+  if (k0 < hi) {
+    better_max(dv, di, fdv, fdi);
+    better_min(best, bi, fcv, fci);
+  }
+  for (int k = k0 + (int)blockDim.x; k < hi; k += (int)blockDim.x) {
+    better_max(dv, di, A.part[k], A.parti[k]);
+    better_min(best, bi, A.part[2 * nb1 + k], A.parti[2 * nb1 + k]);
+  }
+  // (one pass of shuffles and one barrier for both)
+  if (!(best == best)) best = DBL_MAX;  // a NaN never wins the strict '<' scan
+  block_reduce_div_cfl(dv, di, best, bi, A.part + nb1 + blockIdx.x, A.parti + nb1 + blockIdx.x,
+                       A.part + 3 * nb1 + blockIdx.x, A.parti + 3 * nb1 + blockIdx.x);
 }
 
 // F1b: one workgroup -- combine the partials of F1 and F1L, then the
