@@ -106,13 +106,20 @@ __device__ __forceinline__ void ld4(const double* __restrict__ a, int e, double 
   v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
 }
 
+// a node's (u, v, x, y)
+struct NodeUVXY {
+  double u, v, x, y;
+};
+__device__ __forceinline__ NodeUVXY node_uvxy(const FusedNodes& N, int a) {
+  return NodeUVXY{N.u[a], N.v[a], N.x[a], N.y[a]};
+}
 // differences (b - a) of node data along the edge a -> b
-__device__ __forceinline__ void edge_nodes(const FusedNodes& N, int a, int b, double& du,
-                                           double& dv, double& dx, double& dy) {
-  du = N.u[b] - N.u[a];
-  dv = N.v[b] - N.v[a];
-  dx = N.x[b] - N.x[a];
-  dy = N.y[b] - N.y[a];
+__device__ __forceinline__ void edge_diff(const NodeUVXY& a, const NodeUVXY& b, double& du,
+                                          double& dv, double& dx, double& dy) {
+  du = b.u - a.u;
+  dv = b.v - a.v;
+  dx = b.x - a.x;
+  dy = b.y - a.y;
 }
 
 // ---------------------------------------------------------------- F1
@@ -139,6 +146,12 @@ __device__ __forceinline__ void f1_viscosity(const FusedArgs& A, const FusedNode
     int g = (f + 1) & 3;
     ou[f] = u[g] - u[f]; ov[f] = v[g] - v[f]; ox[f] = x[g] - x[f]; oy[f] = y[g] - y[f];
   }
+  // the element's own nodes come from registers (the same values its
+  // corners loaded); only the outer nodes are gathered, the four of a
+  // limiter direction together
+  NodeUVXY own[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) own[k] = NodeUVXY{u[k], v[k], x[k], y[k]};
   double ed[4];
 #pragma unroll
   for (int iside = 0; iside < 2; ++iside) {
@@ -147,14 +160,17 @@ __device__ __forceinline__ void f1_viscosity(const FusedArgs& A, const FusedNode
     double s1, s2, s3, s4;
     double au, av, ax, ay, bu, bv, bx, by;
     const int* o = on + 4 * iside;
+    NodeUVXY out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = node_uvxy(N, o[k]);
     // in1 edge f1+1, in2 edge f2+3
-    edge_nodes(N, nd[iside], o[0], au, av, ax, ay);
-    edge_nodes(N, o[2], nd[(iside + 3) & 3], bu, bv, bx, by);
+    edge_diff(own[iside], out[0], au, av, ax, ay);
+    edge_diff(out[2], own[(iside + 3) & 3], bu, bv, bx, by);
     limiter_ratios(ou[is1], ov[is1], ox[is1], oy[is1], au, av, ax, ay, bu, bv, bx, by,
                    A.zerocut, s1, s2);
     // in1 edge f1+3, in2 edge f2+1
-    edge_nodes(N, o[1], nd[iside + 1], au, av, ax, ay);
-    edge_nodes(N, nd[iside + 2], o[3], bu, bv, bx, by);
+    edge_diff(out[1], own[iside + 1], au, av, ax, ay);
+    edge_diff(own[iside + 2], out[3], bu, bv, bx, by);
     limiter_ratios(ou[is2], ov[is2], ox[is2], oy[is2], au, av, ax, ay, bu, bv, bx, by,
                    A.zerocut, s3, s4);
     if (in1 == e) {
