@@ -147,9 +147,10 @@ TILE_KERNELS = ("lag_tile", "f1_seg")
 
 
 def tile_bytes(k, nel, nnd, moving, zeroed, tiles):
-    _, _, nent = tiles
+    ntile, _, nent = tiles
     if k == "lag_tile":
-        return 24 * nent + 98 * nel + 88 * nnd + 64 * moving + 72 * zeroed
+        # (per node: its entry's corner-mask byte; per tile: its bounds)
+        return 24 * nent + 98 * nel + 89 * nnd + 16 * ntile + 64 * moving + 72 * zeroed
     return 8 * nent + 25 * nel + 32 * nnd
 
 

@@ -1225,8 +1225,7 @@ k_lag_tile(FusedArgs A, FusedNodes N, FusedNodesOut O, EnsArgs E) {
   const int nb = tb.z, nn = tb.w - tb.z;
   // every thread loads a node entry and an element entry (a tile has at
   // least one of each; threads past the end reload the last), so that these
-  // loads, and the node values after them, are in flight together; no
-  // barrier comes before phase A
+  // loads are in flight together; no barrier comes before phase A
   const bool own = tid < nn, active = tid < ne;
   const uint32_t nv = (uint32_t)A.tile_nodes[nb + min(tid, nn - 1)];
   // the node's corner mask: the slots of F2 that phase B sums
