@@ -68,10 +68,11 @@ __device__ __forceinline__ bool step_skipped(const FusedArgs& A, const DevTimest
 
 // the limiter pass's at-rest flags per thread and scan pass (a byte word)
 constexpr int kScanPer = 8;
-// F1b's workgroup: two partials of each control per thread at the usual
-// 1024 limiter workgroups (512 measured 1-1.5 us a step faster than 1024
-// threads, which take longer to start)
-constexpr int kF1bThreads = 512;
+// F1b's workgroup: four partials of each control per thread at the usual
+// 1024 limiter workgroups, all loaded at once (512 threads were 1-1.5 us a
+// step faster than 1024, which take longer to start, and 256 another
+// 0.9 us faster than 512)
+constexpr int kF1bThreads = 256;
 
 // a wave's count of elements whose zero viscosity it wrote, to its
 // workgroup's counter slot
